@@ -1,0 +1,383 @@
+"""MI355X-native strongly-connected-components engine (Python host API).
+
+Thin ctypes binding of ``libscc_amd.so`` (C ABI in ``include/scc_amd.h``).
+Every SCC computation runs in the HIP library on the GPU; this module only
+moves numpy arrays across the boundary.  Importing the package does not
+touch the GPU; creating a :class:`Context` does, and fails loudly when the
+library or a device is missing.
+
+Names mirror the reference (Amit104/DetectingSCC):
+  - :func:`load_full_graph`   <- ``loadFullGraph`` (src/load.cpp:12-140)
+  - :func:`vHong`, :func:`wHong`, :func:`vSlota`, :func:`wSlota`,
+    :func:`detectSCC`         <- the drivers of include/scc.h:49-53
+Graphs use the reference layout: vertices 1..V, dummy vertex 0, row arrays of
+V+2 entries, column arrays of E entries.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from dataclasses import dataclass, field
+
+import numpy as np
+
+_PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_PKG, "lib", "libscc_amd.so")
+CLI_PATH = os.path.join(_PKG, "bin", "scc")
+
+_u32p = ctypes.POINTER(ctypes.c_uint32)
+_u8p = ctypes.POINTER(ctypes.c_uint8)
+_u64p = ctypes.POINTER(ctypes.c_uint64)
+
+EXPAND_AUTO, EXPAND_VERTEX, EXPAND_VWARP, EXPAND_LB = 0, 1, 2, 3
+PHASES = ["trim1", "trim2", "pivot", "reach", "update", "wcc", "color", "local", "merge", "label"]
+WARP_SIZES = (1, 2, 4, 8, 16, 32, 64)
+
+
+class SCCError(RuntimeError):
+    """A non-zero scc_status from the library."""
+
+    def __init__(self, status: int, message: str):
+        super().__init__(f"[{status}] {message}")
+        self.status = status
+
+
+class _HostGraph(ctypes.Structure):
+    _fields_ = [("V", ctypes.c_uint32), ("E", ctypes.c_uint32),
+                ("Fr", _u32p), ("Fc", _u32p), ("Br", _u32p), ("Bc", _u32p)]
+
+
+class _Opts(ctypes.Structure):
+    _fields_ = [("algo", ctypes.c_char), ("trim1", ctypes.c_int), ("trim2", ctypes.c_int),
+                ("warp", ctypes.c_int), ("expand", ctypes.c_int), ("block", ctypes.c_int),
+                ("check_every", ctypes.c_int)]
+
+
+class _Stats(ctypes.Structure):
+    _fields_ = [("num_sccs", ctypes.c_uint32), ("num_trim1", ctypes.c_uint32),
+                ("num_trim2", ctypes.c_uint32), ("num_pivot", ctypes.c_uint32),
+                ("largest", ctypes.c_uint32), ("rounds", ctypes.c_uint32),
+                ("levels", ctypes.c_uint32), ("first_pivot", ctypes.c_uint32),
+                ("first_fw_vertices", ctypes.c_uint64), ("first_fw_arcs", ctypes.c_uint64),
+                ("first_bw_vertices", ctypes.c_uint64), ("first_bw_arcs", ctypes.c_uint64),
+                ("ms_scc", ctypes.c_double), ("ms_first_reach", ctypes.c_double),
+                ("ms_phase", ctypes.c_double * len(PHASES))]
+
+
+_LIB = None
+
+
+def lib() -> ctypes.CDLL:
+    """Load libscc_amd.so once; raise if it has not been built."""
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: run `python -m detectingscc_amd.build`")
+        L = ctypes.CDLL(LIB_PATH)
+        for name in ["scc_status_string", "scc_last_error", "scc_version"]:
+            getattr(L, name).restype = ctypes.c_char_p
+        L.scc_status_string.argtypes = [ctypes.c_int]
+        _LIB = L
+    return _LIB
+
+
+def _check(status: int) -> None:
+    if status != 0:
+        msg = lib().scc_last_error().decode(errors="replace")
+        raise SCCError(status, msg or lib().scc_status_string(status).decode())
+
+
+def _p32(a: np.ndarray):
+    return a.ctypes.data_as(_u32p)
+
+
+def _p8(a):
+    return None if a is None else a.ctypes.data_as(_u8p)
+
+
+@dataclass
+class Graph:
+    """Host CSR + reverse CSR in the reference layout (uint32 arrays)."""
+    V: int
+    E: int
+    Fr: np.ndarray
+    Fc: np.ndarray
+    Br: np.ndarray
+    Bc: np.ndarray
+    part: np.ndarray | None = field(default=None, repr=False)
+
+    def out_degree(self) -> np.ndarray:
+        return np.diff(self.Fr).astype(np.int64)
+
+    def in_degree(self) -> np.ndarray:
+        return np.diff(self.Br).astype(np.int64)
+
+
+def _take(hg: _HostGraph) -> Graph:
+    V, E = int(hg.V), int(hg.E)
+    def arr(p, n):
+        return np.ctypeslib.as_array(p, shape=(max(n, 1),))[:n].copy() if n else np.zeros(0, np.uint32)
+    g = Graph(V, E, arr(hg.Fr, V + 2), arr(hg.Fc, E), arr(hg.Br, V + 2), arr(hg.Bc, E))
+    lib().scc_host_graph_free(ctypes.byref(hg))
+    return g
+
+
+def load_full_graph(path: str) -> Graph:
+    """GTgraph text -> Graph (loadFullGraph, src/load.cpp:12-113)."""
+    hg = _HostGraph()
+    _check(lib().scc_load_gtgraph(os.fsencode(path), ctypes.byref(hg)))
+    return _take(hg)
+
+
+def load_partition(path: str, V: int) -> np.ndarray:
+    """Partition vector file -> uint32[V+2] with ids at 1..V (load.cpp:117-130)."""
+    part = np.zeros(V + 2, np.uint32)
+    _check(lib().scc_load_partition(os.fsencode(path), ctypes.c_uint32(V), _p32(part)))
+    return part
+
+
+def from_arcs(V: int, src, dst) -> Graph:
+    """Arc list (1-based ids) -> Graph in the reference layout."""
+    s = np.ascontiguousarray(src, dtype=np.uint32)
+    d = np.ascontiguousarray(dst, dtype=np.uint32)
+    if s.shape != d.shape:
+        raise ValueError("src and dst differ in length")
+    hg = _HostGraph()
+    _check(lib().scc_csr_from_arcs(ctypes.c_uint32(V), ctypes.c_uint64(s.size), _p32(s), _p32(d),
+                                   ctypes.byref(hg)))
+    return _take(hg)
+
+
+def gen_er(V: int, draws: int, cycles: int = 0, cmin: int = 2, cmax: int = 256, seed: int = 1) -> Graph:
+    hg = _HostGraph()
+    _check(lib().scc_gen_er(ctypes.c_uint32(V), ctypes.c_uint64(draws), ctypes.c_uint32(cycles),
+                            ctypes.c_uint32(cmin), ctypes.c_uint32(cmax), ctypes.c_uint64(seed),
+                            ctypes.byref(hg)))
+    return _take(hg)
+
+
+def gen_rmat(scale: int, E: int | None = None, a: float = 0.57, b: float = 0.19, c: float = 0.19,
+             seed: int = 22) -> Graph:
+    E = 16 * (1 << scale) if E is None else E
+    hg = _HostGraph()
+    _check(lib().scc_gen_rmat(ctypes.c_uint32(scale), ctypes.c_uint64(E), ctypes.c_double(a),
+                              ctypes.c_double(b), ctypes.c_double(c), ctypes.c_uint64(seed),
+                              ctypes.byref(hg)))
+    return _take(hg)
+
+
+def gen_grid(side: int, p: float = 0.7, seed: int = 4096) -> Graph:
+    hg = _HostGraph()
+    _check(lib().scc_gen_grid(ctypes.c_uint32(side), ctypes.c_double(p), ctypes.c_uint64(seed),
+                              ctypes.byref(hg)))
+    return _take(hg)
+
+
+def gen_communities(V: int, K: int, deg: float = 12.0, inter: float = 0.02, seed: int = 55) -> Graph:
+    hg = _HostGraph()
+    part = np.zeros(V + 2, np.uint32)
+    _check(lib().scc_gen_communities(ctypes.c_uint32(V), ctypes.c_uint32(K), ctypes.c_double(deg),
+                                     ctypes.c_double(inter), ctypes.c_uint64(seed), ctypes.byref(hg),
+                                     _p32(part)))
+    g = _take(hg)
+    g.part = part
+    return g
+
+
+def write_gtgraph(path: str, g: Graph) -> None:
+    hg = _HostGraph(g.V, g.E, _p32(g.Fr), _p32(g.Fc), _p32(g.Br), _p32(g.Bc))
+    _check(lib().scc_write_gtgraph(os.fsencode(path), ctypes.byref(hg)))
+
+
+def write_metis(path: str, g: Graph) -> None:
+    hg = _HostGraph(g.V, g.E, _p32(g.Fr), _p32(g.Fc), _p32(g.Br), _p32(g.Bc))
+    _check(lib().scc_write_metis(os.fsencode(path), ctypes.byref(hg)))
+
+
+def size_histogram(labels: np.ndarray) -> list[tuple[int, int]]:
+    """(size, count) pairs ascending by size from canonical labels (V+1 entries)."""
+    lab = np.ascontiguousarray(labels, dtype=np.uint32)
+    V = lab.size - 1
+    n = ctypes.c_uint32(0)
+    _check(lib().scc_size_histogram(ctypes.c_uint32(V), _p32(lab), None, ctypes.c_uint32(0), ctypes.byref(n)))
+    pairs = np.zeros(2 * max(n.value, 1), np.uint64)
+    _check(lib().scc_size_histogram(ctypes.c_uint32(V), _p32(lab), pairs.ctypes.data_as(_u64p),
+                                    ctypes.c_uint32(n.value), ctypes.byref(n)))
+    return [(int(pairs[2 * i]), int(pairs[2 * i + 1])) for i in range(n.value)]
+
+
+@dataclass
+class Stats:
+    num_sccs: int
+    num_trim1: int
+    num_trim2: int
+    num_pivot: int
+    largest: int
+    rounds: int
+    levels: int
+    first_pivot: int
+    first_fw_vertices: int
+    first_fw_arcs: int
+    first_bw_vertices: int
+    first_bw_arcs: int
+    ms_scc: float
+    ms_first_reach: float
+    ms_phase: dict
+
+
+def _stats(s: _Stats) -> Stats:
+    return Stats(s.num_sccs, s.num_trim1, s.num_trim2, s.num_pivot, s.largest, s.rounds, s.levels,
+                 s.first_pivot, s.first_fw_vertices, s.first_fw_arcs, s.first_bw_vertices,
+                 s.first_bw_arcs, s.ms_scc, s.ms_first_reach,
+                 {PHASES[i]: float(s.ms_phase[i]) for i in range(len(PHASES))})
+
+
+def make_opts(algo: str = "g", trim1: bool = True, trim2: bool = True, warp: int = 1,
+              expand: int = EXPAND_AUTO, check_every: int = 0) -> _Opts:
+    o = _Opts()
+    lib().scc_opts_default(ctypes.byref(o))
+    o.algo = algo.encode()[:1]
+    o.trim1, o.trim2, o.warp, o.expand, o.check_every = int(trim1), int(trim2), int(warp), int(expand), int(check_every)
+    return o
+
+
+class Context:
+    """A device context: resident graph + state on one HIP device."""
+
+    def __init__(self, device: int = 0):
+        self._ctx = ctypes.c_void_p()
+        _check(lib().scc_ctx_create(ctypes.c_int(device), ctypes.byref(self._ctx)))
+        self.graph: Graph | None = None
+
+    def close(self) -> None:
+        if self._ctx:
+            lib().scc_ctx_destroy(self._ctx)
+            self._ctx = ctypes.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def upload(self, g: Graph) -> None:
+        _check(lib().scc_graph_upload(self._ctx, ctypes.c_uint32(g.V), ctypes.c_uint32(g.E),
+                                      _p32(g.Fr), _p32(g.Fc), _p32(g.Br), _p32(g.Bc)))
+        self.graph = g
+        if g.part is not None:
+            self.upload_partition(g.part)
+
+    def upload_partition(self, part: np.ndarray) -> None:
+        part = np.ascontiguousarray(part, dtype=np.uint32)
+        _check(lib().scc_partition_upload(self._ctx, _p32(part)))
+
+    def run(self, algo: str = "g", trim1: bool = True, trim2: bool = True, warp: int = 1,
+            expand: int = EXPAND_AUTO, check_every: int = 0) -> Stats:
+        o = make_opts(algo, trim1, trim2, warp, expand, check_every)
+        s = _Stats()
+        _check(lib().scc_run(self._ctx, ctypes.byref(o), ctypes.byref(s)))
+        return _stats(s)
+
+    def labels(self) -> np.ndarray:
+        lab = np.zeros(self.graph.V + 1, np.uint32)
+        _check(lib().scc_labels(self._ctx, _p32(lab)))
+        return lab
+
+    def solve(self, g: Graph, **kw) -> tuple[np.ndarray, Stats]:
+        self.upload(g)
+        st = self.run(**kw)
+        return self.labels(), st
+
+    # -- single phases (kernel-level checks) --
+    def _dead(self, dead):
+        if dead is None:
+            return None
+        d = np.ascontiguousarray(dead, dtype=np.uint8)
+        if d.size != self.graph.V + 1:
+            raise ValueError("dead must have V+1 entries")
+        return d
+
+    def phase_trim1(self, dead=None) -> np.ndarray:
+        d = self._dead(dead)
+        out = np.zeros(self.graph.V + 1, np.uint8)
+        _check(lib().scc_phase_trim1(self._ctx, _p8(d), _p8(out)))
+        return out
+
+    def phase_trim2(self, dead=None) -> np.ndarray:
+        d = self._dead(dead)
+        out = np.zeros(self.graph.V + 1, np.uint32)
+        _check(lib().scc_phase_trim2(self._ctx, _p8(d), _p32(out)))
+        return out
+
+    def phase_first_pivot(self, dead=None) -> int:
+        d = self._dead(dead)
+        p = ctypes.c_uint32(0)
+        _check(lib().scc_phase_first_pivot(self._ctx, _p8(d), ctypes.byref(p)))
+        return p.value
+
+    def phase_reach(self, src: int, direction: int = 0, expand: int = EXPAND_VERTEX, warp: int = 1,
+                    dead=None) -> tuple[np.ndarray, int, float]:
+        d = self._dead(dead)
+        out = np.zeros(self.graph.V + 1, np.uint8)
+        lv = ctypes.c_uint32(0)
+        ms = ctypes.c_double(0)
+        _check(lib().scc_phase_reach(self._ctx, ctypes.c_uint32(src), ctypes.c_int(direction),
+                                     ctypes.c_int(expand), ctypes.c_int(warp), _p8(d), _p8(out),
+                                     ctypes.byref(lv), ctypes.byref(ms)))
+        return out, lv.value, ms.value
+
+    def phase_min_label(self, dead=None) -> np.ndarray:
+        d = self._dead(dead)
+        out = np.zeros(self.graph.V + 1, np.uint32)
+        _check(lib().scc_phase_min_label(self._ctx, _p8(d), _p32(out)))
+        return out
+
+    def phase_max_color(self, dead=None) -> np.ndarray:
+        d = self._dead(dead)
+        out = np.zeros(self.graph.V + 1, np.uint32)
+        _check(lib().scc_phase_max_color(self._ctx, _p8(d), _p32(out)))
+        return out
+
+
+def scc(g: Graph, algo: str = "g", trim1: bool = True, trim2: bool = True, warp: int = 1,
+        expand: int = EXPAND_AUTO, device: int = 0) -> tuple[np.ndarray, Stats]:
+    """One-shot SCC decomposition: canonical labels (V+1) and run statistics."""
+    with Context(device) as ctx:
+        return ctx.solve(g, algo=algo, trim1=trim1, trim2=trim2, warp=warp, expand=expand)
+
+
+# -- drop-in driver names (include/scc.h:49-53); they print the reference's lines --
+def _dropin(name, *args):
+    fn = getattr(lib(), name)
+    return fn(*args)
+
+
+def vHong(CSize, RSize, Fc, Fr, Bc, Br, t1, t2) -> int:
+    return _dropin("vHong", ctypes.c_uint32(CSize), ctypes.c_uint32(RSize), _p32(Fc), _p32(Fr),
+                   _p32(Bc), _p32(Br), ctypes.c_bool(t1), ctypes.c_bool(t2))
+
+
+def wHong(CSize, RSize, Fc, Fr, Bc, Br, t1, t2, warpSize) -> int:
+    return _dropin("wHong", ctypes.c_uint32(CSize), ctypes.c_uint32(RSize), _p32(Fc), _p32(Fr),
+                   _p32(Bc), _p32(Br), ctypes.c_bool(t1), ctypes.c_bool(t2), ctypes.c_int(warpSize))
+
+
+def vSlota(CSize, RSize, Fc, Fr, Bc, Br, t1, t2) -> int:
+    return _dropin("vSlota", ctypes.c_uint32(CSize), ctypes.c_uint32(RSize), _p32(Fc), _p32(Fr),
+                   _p32(Bc), _p32(Br), ctypes.c_bool(t1), ctypes.c_bool(t2))
+
+
+def wSlota(CSize, RSize, Fc, Fr, Bc, Br, t1, t2, warpSize) -> int:
+    return _dropin("wSlota", ctypes.c_uint32(CSize), ctypes.c_uint32(RSize), _p32(Fc), _p32(Fr),
+                   _p32(Bc), _p32(Br), ctypes.c_bool(t1), ctypes.c_bool(t2), ctypes.c_int(warpSize))
+
+
+def detectSCC(CSize, RSize, Fc, Fr, Bc, Br, Pr, t1, t2) -> int:
+    return _dropin("detectSCC", ctypes.c_uint32(CSize), ctypes.c_uint32(RSize), _p32(Fc), _p32(Fr),
+                   _p32(Bc), _p32(Br), _p32(Pr), ctypes.c_bool(t1), ctypes.c_bool(t2))

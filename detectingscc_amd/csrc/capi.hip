@@ -1,0 +1,262 @@
+// capi.hip -- extern "C" entry points of libscc_amd.so: context management,
+// the device engine, phase entries, and the drop-in wrappers with the
+// reference driver names (include/scc.h:49-53, include/load.h:6).
+#include <chrono>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <new>
+
+#include "engine.h"
+#include "host_util.h"
+
+using namespace sccamd;
+
+#define CTX_CHECK(ctx)                                                                      \
+    do {                                                                                    \
+        if (!(ctx)) return fail(SCC_ERR_INVALID, "null context");                           \
+    } while (0)
+
+extern "C" {
+
+const char *scc_version(void) { return "scc-mi355x 0.1 (gfx950)"; }
+
+void scc_opts_default(scc_opts *o)
+{
+    if (!o) return;
+    std::memset(o, 0, sizeof *o);
+    o->algo = 'g';
+    o->trim1 = 1;
+    o->trim2 = 1;
+    o->warp = 1;
+    o->expand = SCC_EXPAND_AUTO;
+}
+
+scc_status scc_ctx_create(int device, scc_ctx **out)
+{
+    clear_error();
+    if (!out) return fail(SCC_ERR_INVALID, "null output pointer");
+    *out = nullptr;
+    scc_ctx *c = new (std::nothrow) scc_ctx();
+    if (!c) return fail(SCC_ERR_NOMEM, "host allocation failed");
+    scc_status s = c->eng.init(device);
+    if (s != SCC_OK) {
+        delete c;
+        return s;
+    }
+    *out = c;
+    return SCC_OK;
+}
+
+void scc_ctx_destroy(scc_ctx *ctx) { delete ctx; }
+
+scc_status scc_graph_upload(scc_ctx *ctx, uint32_t V, uint32_t E, const uint32_t *Fr,
+                            const uint32_t *Fc, const uint32_t *Br, const uint32_t *Bc)
+{
+    clear_error();
+    CTX_CHECK(ctx);
+    return ctx->eng.upload(V, E, Fr, Fc, Br, Bc);
+}
+
+scc_status scc_partition_upload(scc_ctx *ctx, const uint32_t *part)
+{
+    clear_error();
+    CTX_CHECK(ctx);
+    if (!part) return fail(SCC_ERR_INVALID, "null partition");
+    return ctx->eng.upload_partition(part);
+}
+
+scc_status scc_run(scc_ctx *ctx, const scc_opts *opts, scc_stats *stats)
+{
+    clear_error();
+    CTX_CHECK(ctx);
+    scc_opts o;
+    if (opts) o = *opts;
+    else scc_opts_default(&o);
+    return ctx->eng.run(o, stats);
+}
+
+scc_status scc_labels(scc_ctx *ctx, uint32_t *lab)
+{
+    clear_error();
+    CTX_CHECK(ctx);
+    if (!lab) return fail(SCC_ERR_INVALID, "null label buffer");
+    return ctx->eng.labels(lab);
+}
+
+scc_status scc_solve(scc_ctx *ctx, const scc_host_graph *g, const uint32_t *part,
+                     const scc_opts *opts, uint32_t *lab, scc_stats *stats)
+{
+    clear_error();
+    CTX_CHECK(ctx);
+    if (!g) return fail(SCC_ERR_INVALID, "null graph");
+    scc_status s = ctx->eng.upload(g->V, g->E, g->Fr, g->Fc, g->Br, g->Bc);
+    if (s == SCC_OK && part) s = ctx->eng.upload_partition(part);
+    if (s == SCC_OK) s = scc_run(ctx, opts, stats);
+    if (s == SCC_OK && lab) s = ctx->eng.labels(lab);
+    return s;
+}
+
+scc_status scc_phase_trim1(scc_ctx *ctx, const uint8_t *dead, uint8_t *removed)
+{
+    clear_error();
+    CTX_CHECK(ctx);
+    if (!removed) return fail(SCC_ERR_INVALID, "null output");
+    return ctx->eng.phase_trim1(dead, removed);
+}
+
+scc_status scc_phase_trim2(scc_ctx *ctx, const uint8_t *dead, uint32_t *pair)
+{
+    clear_error();
+    CTX_CHECK(ctx);
+    if (!pair) return fail(SCC_ERR_INVALID, "null output");
+    return ctx->eng.phase_trim2(dead, pair);
+}
+
+scc_status scc_phase_first_pivot(scc_ctx *ctx, const uint8_t *dead, uint32_t *pivot)
+{
+    clear_error();
+    CTX_CHECK(ctx);
+    if (!pivot) return fail(SCC_ERR_INVALID, "null output");
+    return ctx->eng.phase_first_pivot(dead, pivot);
+}
+
+scc_status scc_phase_reach(scc_ctx *ctx, uint32_t src, int dir, int expand, int warp,
+                           const uint8_t *dead, uint8_t *out, uint32_t *levels, double *ms)
+{
+    clear_error();
+    CTX_CHECK(ctx);
+    if (!out) return fail(SCC_ERR_INVALID, "null output");
+    return ctx->eng.phase_reach(src, dir, expand, warp, dead, out, levels, ms);
+}
+
+scc_status scc_phase_min_label(scc_ctx *ctx, const uint8_t *dead, uint32_t *out)
+{
+    clear_error();
+    CTX_CHECK(ctx);
+    if (!out) return fail(SCC_ERR_INVALID, "null output");
+    return ctx->eng.phase_min_label(dead, out);
+}
+
+scc_status scc_phase_max_color(scc_ctx *ctx, const uint8_t *dead, uint32_t *out)
+{
+    clear_error();
+    CTX_CHECK(ctx);
+    if (!out) return fail(SCC_ERR_INVALID, "null output");
+    return ctx->eng.phase_max_color(dead, out);
+}
+
+// ------------------------------------------------- drop-in wrappers ---
+
+static std::mutex g_dropin_mu;
+static scc_ctx *g_dropin_ctx = nullptr;
+
+// One driver call in the reference's shape: H2D, all phases, D2H of the
+// result, then the stdout lines.  "Time" covers the same span as the
+// reference's cudaEvent pair (scc_WCC.cu:577-581 -> 951-953): it starts
+// before device allocation and ends after the count.
+static int dropin(char algo, uint32_t CSize, uint32_t RSize, uint32_t *Fc, uint32_t *Fr,
+                  uint32_t *Bc, uint32_t *Br, uint32_t *Pr, bool t1, bool t2, int warp)
+{
+    std::lock_guard<std::mutex> lk(g_dropin_mu);
+    auto t0 = std::chrono::steady_clock::now();
+    scc_status s = SCC_OK;
+    if (!g_dropin_ctx) s = scc_ctx_create(0, &g_dropin_ctx);
+    scc_stats st;
+    std::memset(&st, 0, sizeof st);
+    if (s == SCC_OK) s = g_dropin_ctx->eng.upload(RSize, CSize, Fr, Fc, Br, Bc);
+    if (s == SCC_OK && algo == 'd') {
+        if (!Pr) s = fail(SCC_ERR_INVALID, "detectSCC needs a partition vector");
+        else s = g_dropin_ctx->eng.upload_partition(Pr);
+    }
+    if (s == SCC_OK) {
+        scc_opts o;
+        scc_opts_default(&o);
+        o.algo = algo;
+        o.trim1 = t1;
+        o.trim2 = t2;
+        o.warp = warp;
+        s = g_dropin_ctx->eng.run(o, &st);
+    }
+    if (s != SCC_OK) {
+        printf("%s\n", scc_last_error());
+        return (int)s;
+    }
+    auto t1c = std::chrono::steady_clock::now();
+    double ms = std::chrono::duration<double, std::milli>(t1c - t0).count();
+    if (algo == 'g') printf("numberOf1Sccs: %u\n", st.num_trim1);
+    if (algo == 'd') {
+        printf("LOCAL SCC's IDENTIFIED! NODES WITH SAME RANGE VALUES BELONG TO THE SAME SCC!!\n");
+        printf("STARTING MERGE!\n");
+        printf("MERGING DONE! ^_^\n");
+    }
+    printf("\nNumber Of Sccs : %u", st.num_sccs);
+    printf("\nTime : %f", ms);
+    if (algo == 'd') printf("\n");
+    fflush(stdout);
+    return 0;
+}
+
+int vHong(uint32_t CSize, uint32_t RSize, uint32_t *Fc, uint32_t *Fr, uint32_t *Bc, uint32_t *Br,
+          bool t1, bool t2)
+{
+    return dropin('g', CSize, RSize, Fc, Fr, Bc, Br, nullptr, t1, t2, 1);
+}
+
+int wHong(uint32_t CSize, uint32_t RSize, uint32_t *Fc, uint32_t *Fr, uint32_t *Bc, uint32_t *Br,
+          bool t1, bool t2, int warpSize)
+{
+    return dropin('x', CSize, RSize, Fc, Fr, Bc, Br, nullptr, t1, t2, warpSize);
+}
+
+int vSlota(uint32_t CSize, uint32_t RSize, uint32_t *Fc, uint32_t *Fr, uint32_t *Bc, uint32_t *Br,
+           bool t1, bool t2)
+{
+    return dropin('h', CSize, RSize, Fc, Fr, Bc, Br, nullptr, t1, t2, 1);
+}
+
+int wSlota(uint32_t CSize, uint32_t RSize, uint32_t *Fc, uint32_t *Fr, uint32_t *Bc, uint32_t *Br,
+           bool t1, bool t2, int warpSize)
+{
+    return dropin('y', CSize, RSize, Fc, Fr, Bc, Br, nullptr, t1, t2, warpSize);
+}
+
+int detectSCC(uint32_t CSize, uint32_t RSize, uint32_t *Fc, uint32_t *Fr, uint32_t *Bc,
+              uint32_t *Br, uint32_t *Pr, bool t1, bool t2)
+{
+    return dropin('d', CSize, RSize, Fc, Fr, Bc, Br, Pr, t1, t2, 1);
+}
+
+void loadFullGraph(const char *filename, uint32_t *oCSize, uint32_t *oRSize, uint32_t **oFc,
+                   uint32_t **oFr, uint32_t **oBc, uint32_t **oBr, uint32_t **oPr)
+{
+    scc_host_graph g;
+    std::memset(&g, 0, sizeof g);
+    scc_status s = scc_load_gtgraph(filename, &g);
+    if (oPr) *oPr = nullptr;
+    if (s != SCC_OK) {
+        printf("Error Reading graph file\n");
+        if (oCSize) *oCSize = 0;
+        if (oRSize) *oRSize = 0;
+        if (oFc) *oFc = nullptr;
+        if (oFr) *oFr = nullptr;
+        if (oBc) *oBc = nullptr;
+        if (oBr) *oBr = nullptr;
+        return;
+    }
+    // the reference's caller releases these with delete[] (scc.cu:110-113)
+    auto dup = [](const uint32_t *p, size_t n) {
+        uint32_t *q = new uint32_t[n ? n : 1];
+        if (n) std::memcpy(q, p, n * sizeof(uint32_t));
+        return q;
+    };
+    if (oCSize) *oCSize = g.E;
+    if (oRSize) *oRSize = g.V;
+    if (oFc) *oFc = dup(g.Fc, g.E);
+    if (oFr) *oFr = dup(g.Fr, (size_t)g.V + 2);
+    if (oBc) *oBc = dup(g.Bc, g.E);
+    if (oBr) *oBr = dup(g.Br, (size_t)g.V + 2);
+    scc_host_graph_free(&g);
+}
+
+} // extern "C"

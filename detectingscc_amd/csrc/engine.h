@@ -1,0 +1,72 @@
+// engine.h -- the device-resident SCC engine behind the C ABI.
+#pragma once
+#include <vector>
+
+#include "scc_amd.h"
+#include "scc_dev.h"
+
+struct scc_ctx;
+
+namespace sccamd {
+
+const char *phase_name(int p);
+
+class Engine {
+  public:
+    ~Engine();
+    scc_status init(int device);
+    scc_status upload(uint32_t V, uint32_t E, const uint32_t *Fr, const uint32_t *Fc,
+                      const uint32_t *Br, const uint32_t *Bc);
+    scc_status upload_partition(const uint32_t *part);
+    scc_status run(const scc_opts &o, scc_stats *out);
+    scc_status labels(uint32_t *lab);
+
+    scc_status phase_trim1(const uint8_t *dead, uint8_t *removed);
+    scc_status phase_trim2(const uint8_t *dead, uint32_t *pair);
+    scc_status phase_first_pivot(const uint8_t *dead, uint32_t *pivot);
+    scc_status phase_reach(uint32_t src, int dir, int expand, int W, const uint8_t *dead,
+                           uint8_t *out, uint32_t *levels, double *ms);
+    scc_status phase_min_label(const uint8_t *dead, uint32_t *out);
+    scc_status phase_max_color(const uint8_t *dead, uint32_t *out);
+
+    int device = -1;
+    hipStream_t stream = nullptr;
+    DevGraph g;
+    DevState st;
+    bool has_graph = false, has_part = false, ran = false;
+
+  private:
+    void release();
+    scc_status reserve(uint32_t V, uint32_t E);
+    scc_status read_ctr();
+    void zero_ctr(int first, int n);
+    void mark(int phase);
+    void collect_times(scc_stats *s);
+    uint32_t bpd(int W) const;
+    scc_status reach(bool fw, bool bw, bool single, int expand, int W, uint32_t *levels_out);
+    scc_status trim1(bool fixpoint);
+    uint64_t limit() const;
+    scc_status no_convergence(const char *what);
+    scc_status pivot_round(bool first, int expand, int W, scc_stats *stats, uint32_t *live);
+    scc_status min_label();
+    scc_status color_loop(int expand, int W, scc_stats *stats);
+    scc_status run_detect(const scc_opts &o, scc_stats *out);
+    scc_status phase_begin(const uint8_t *dead);
+    scc_status fetch8(uint8_t *out);
+    scc_status fetch32(uint32_t *out, const uint32_t *src);
+
+    uint32_t capV = 0, capE = 0;
+    uint32_t *h_ctr = nullptr;        // pinned mirror of st.ctr
+    uint8_t *d_tmp8 = nullptr;        // V+1 bytes for phase inputs/outputs
+    int check_every = 0;
+    std::vector<hipEvent_t> events;
+    std::vector<int> mark_phase;
+    size_t nmarks = 0;
+    int first_reach_marks[2] = {-1, -1};
+};
+
+} // namespace sccamd
+
+struct scc_ctx {
+    sccamd::Engine eng;
+};

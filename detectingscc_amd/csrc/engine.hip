@@ -1,0 +1,599 @@
+// engine.hip -- the SCC phase engine: device-resident graph and state, the
+// vHong / wHong / vSlota / wSlota phase orders, phase timing, and the C ABI
+// for the device engine.
+//
+// The reference drives every phase from the host with a 1-byte D2H after
+// each kernel (scc_WCC.cu:683-686).  Here a BFS search queues `check_every`
+// levels back to back (queue counts live on the device) before the host
+// reads the counters once; sweeps that run to a fixpoint read one counter
+// word per pass.
+#include <chrono>
+#include <cstring>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "engine.h"
+#include "host_util.h"
+
+namespace sccamd {
+
+#define HIP_TRY(call)                                                                       \
+    do {                                                                                    \
+        hipError_t _e = (call);                                                             \
+        if (_e != hipSuccess)                                                               \
+            return fail(SCC_ERR_DEVICE, "%s failed: %s (%s:%d)", #call, hipGetErrorString(_e), \
+                        __FILE__, __LINE__);                                                \
+    } while (0)
+
+static const char *const kPhaseNames[SCC_PHASE_COUNT] = {
+    "trim1", "trim2", "pivot", "reach", "update", "wcc", "color", "local", "merge", "label"};
+
+const char *phase_name(int p) { return (p >= 0 && p < SCC_PHASE_COUNT) ? kPhaseNames[p] : "?"; }
+
+Engine::~Engine() { release(); }
+
+void Engine::release()
+{
+    if (device >= 0) (void)hipSetDevice(device);
+    auto f = [](void *p) {
+        if (p) (void)hipFree(p);
+    };
+    f(g.Fr); f(g.Fc); f(g.Br); f(g.Bc);
+    f(st.cls); f(st.lab); f(st.aux); f(st.fvis); f(st.bvis); f(st.piv);
+    for (int i = 0; i < 2; i++) {
+        f(st.qf[i]); f(st.qb[i]); f(st.bigf[i]); f(st.bigb[i]);
+        f(st.bigoff_f[i]); f(st.bigoff_b[i]);
+    }
+    f(st.pivlist); f(st.part); f(st.ctr); f(d_tmp8);
+    if (h_ctr) (void)hipHostFree(h_ctr);
+    for (auto e : events) (void)hipEventDestroy(e);
+    events.clear();
+    if (stream) (void)hipStreamDestroy(stream);
+    g = DevGraph();
+    st = DevState();
+    d_tmp8 = nullptr;
+    h_ctr = nullptr;
+    stream = nullptr;
+    capV = capE = 0;
+}
+
+scc_status Engine::init(int dev)
+{
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0)
+        return fail(SCC_ERR_DEVICE, "no HIP device available");
+    if (dev < 0 || dev >= n) return fail(SCC_ERR_INVALID, "device %d out of range (0..%d)", dev, n - 1);
+    device = dev;
+    HIP_TRY(hipSetDevice(dev));
+    HIP_TRY(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    HIP_TRY(hipHostMalloc((void **)&h_ctr, C_COUNT * sizeof(uint32_t), hipHostMallocDefault));
+    HIP_TRY(hipMalloc((void **)&st.ctr, C_COUNT * sizeof(uint32_t)));
+    HIP_TRY(hipMemset(st.ctr, 0, C_COUNT * sizeof(uint32_t)));
+    return SCC_OK;
+}
+
+template <class T> static hipError_t dalloc(T **p, size_t n)
+{
+    return hipMalloc((void **)p, sizeof(T) * (n ? n : 1));
+}
+
+// (Re)allocate device arrays for V vertices and E arcs.
+scc_status Engine::reserve(uint32_t V, uint32_t E)
+{
+    HIP_TRY(hipSetDevice(device));
+    if (V <= capV && E <= capE) return SCC_OK;
+    uint32_t nV = V > capV ? V : capV, nE = E > capE ? E : capE;
+    // keep the context, drop the arrays
+    auto f = [](void *p) {
+        if (p) (void)hipFree(p);
+    };
+    f(g.Fr); f(g.Fc); f(g.Br); f(g.Bc);
+    f(st.cls); f(st.lab); f(st.aux); f(st.fvis); f(st.bvis); f(st.piv);
+    for (int i = 0; i < 2; i++) {
+        f(st.qf[i]); f(st.qb[i]); f(st.bigf[i]); f(st.bigb[i]);
+        f(st.bigoff_f[i]); f(st.bigoff_b[i]);
+    }
+    f(st.pivlist); f(st.part); f(d_tmp8);
+    uint32_t *ctr = st.ctr;
+    g = DevGraph();
+    st = DevState();
+    st.ctr = ctr;
+    capV = capE = 0;
+    const size_t nv = (size_t)nV + 1;
+    const size_t nwords = 2 * ((nv + 63) / 64);
+    hipError_t e = hipSuccess;
+    auto A = [&](hipError_t r) {
+        if (r != hipSuccess && e == hipSuccess) e = r;
+    };
+    A(dalloc(&g.Fr, nv + 1)); A(dalloc(&g.Br, nv + 1));
+    A(dalloc(&g.Fc, nE)); A(dalloc(&g.Bc, nE));
+    A(dalloc(&st.cls, nv)); A(dalloc(&st.lab, nv)); A(dalloc(&st.aux, nv));
+    A(dalloc(&st.fvis, nwords)); A(dalloc(&st.bvis, nwords));
+    A(dalloc(&st.piv, 3 * nv + 3));
+    for (int i = 0; i < 2; i++) {
+        A(dalloc(&st.qf[i], nv)); A(dalloc(&st.qb[i], nv));
+        A(dalloc(&st.bigf[i], nv)); A(dalloc(&st.bigb[i], nv));
+        A(dalloc(&st.bigoff_f[i], nv)); A(dalloc(&st.bigoff_b[i], nv));
+    }
+    A(dalloc(&st.pivlist, nv)); A(dalloc(&st.part, nv + 1)); A(dalloc(&d_tmp8, nv));
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        return fail(e == hipErrorOutOfMemory ? SCC_ERR_NOMEM : SCC_ERR_DEVICE,
+                    "Error: Not enough memory on GPU (V=%u E=%u): %s", nV, nE, hipGetErrorString(e));
+    }
+    st.nwords = (uint32_t)nwords;
+    capV = nV;
+    capE = nE;
+    return SCC_OK;
+}
+
+// Host-side validation of the reference layout: a malformed CSR must be
+// rejected before any kernel indexes with it.
+static scc_status validate_csr(uint32_t V, uint32_t E, const uint32_t *R, const uint32_t *C,
+                               const char *name)
+{
+    if (R[0] != 0 || R[1] != 0 || R[V + 1] != E)
+        return fail(SCC_ERR_INVALID, "%s row array must start 0,0 and end at E=%u", name, E);
+    std::atomic<int> bad{0};
+    parallel_chunks((uint64_t)V + 1, [&](uint64_t b, uint64_t e, int) {
+        for (uint64_t v = b; v < e; v++)
+            if (R[v] > R[v + 1]) bad = 1;
+    });
+    if (bad) return fail(SCC_ERR_INVALID, "%s row array is not monotone", name);
+    parallel_chunks(E, [&](uint64_t b, uint64_t e, int) {
+        for (uint64_t i = b; i < e; i++)
+            if (C[i] < 1 || C[i] > V) bad = 1;
+    });
+    if (bad) return fail(SCC_ERR_INVALID, "%s column entry outside [1, %u]", name, V);
+    return SCC_OK;
+}
+
+scc_status Engine::upload(uint32_t V, uint32_t E, const uint32_t *Fr, const uint32_t *Fc,
+                          const uint32_t *Br, const uint32_t *Bc)
+{
+    if (!Fr || !Br || (E && (!Fc || !Bc))) return fail(SCC_ERR_INVALID, "null graph array");
+    if (V >= (1u << 30)) return fail(SCC_ERR_INVALID, "V=%u exceeds 2^30-1", V);
+    scc_status s;
+    if ((s = validate_csr(V, E, Fr, Fc, "Fr/Fc")) != SCC_OK) return s;
+    if ((s = validate_csr(V, E, Br, Bc, "Br/Bc")) != SCC_OK) return s;
+    if ((s = reserve(V, E)) != SCC_OK) return s;
+    g.V = V;
+    g.E = E;
+    HIP_TRY(hipMemcpyAsync(g.Fr, Fr, sizeof(uint32_t) * ((size_t)V + 2), hipMemcpyHostToDevice, stream));
+    HIP_TRY(hipMemcpyAsync(g.Br, Br, sizeof(uint32_t) * ((size_t)V + 2), hipMemcpyHostToDevice, stream));
+    if (E) {
+        HIP_TRY(hipMemcpyAsync(g.Fc, Fc, sizeof(uint32_t) * E, hipMemcpyHostToDevice, stream));
+        HIP_TRY(hipMemcpyAsync(g.Bc, Bc, sizeof(uint32_t) * E, hipMemcpyHostToDevice, stream));
+    }
+    HIP_TRY(hipStreamSynchronize(stream));
+    has_graph = true;
+    has_part = false;
+    ran = false;
+    return SCC_OK;
+}
+
+scc_status Engine::upload_partition(const uint32_t *part)
+{
+    if (!has_graph) return fail(SCC_ERR_NOTREADY, "no graph uploaded");
+    HIP_TRY(hipMemcpy(st.part, part, sizeof(uint32_t) * ((size_t)g.V + 2), hipMemcpyHostToDevice));
+    has_part = true;
+    return SCC_OK;
+}
+
+scc_status Engine::read_ctr()
+{
+    HIP_TRY(hipMemcpyAsync(h_ctr, st.ctr, C_COUNT * sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
+    HIP_TRY(hipStreamSynchronize(stream));
+    return SCC_OK;
+}
+
+__global__ void zero_ctr_kernel(uint32_t *ctr, uint32_t first, uint32_t n)
+{
+    if (threadIdx.x < n) ctr[first + threadIdx.x] = 0;
+}
+
+void Engine::zero_ctr(int first, int n)
+{
+    zero_ctr_kernel<<<1, 64, 0, stream>>>(st.ctr, (uint32_t)first, (uint32_t)n);
+}
+
+void Engine::mark(int phase)
+{
+    if (nmarks == events.size()) {
+        hipEvent_t e;
+        if (hipEventCreate(&e) != hipSuccess) return;
+        events.push_back(e);
+    }
+    (void)hipEventRecord(events[nmarks], stream);
+    mark_phase.resize(nmarks + 1);
+    mark_phase[nmarks] = phase;
+    nmarks++;
+}
+
+// Phase times: the interval between consecutive marks belongs to the
+// phase of the earlier mark (-1 = untimed).
+void Engine::collect_times(scc_stats *s)
+{
+    for (int p = 0; p < SCC_PHASE_COUNT; p++) s->ms_phase[p] = 0;
+    s->ms_scc = 0;
+    if (nmarks < 2) return;
+    (void)hipEventSynchronize(events[nmarks - 1]);
+    for (size_t i = 0; i + 1 < nmarks; i++) {
+        float ms = 0;
+        (void)hipEventElapsedTime(&ms, events[i], events[i + 1]);
+        if (mark_phase[i] >= 0) s->ms_phase[mark_phase[i]] += ms;
+        s->ms_scc += ms;
+    }
+}
+
+uint32_t Engine::bpd(int W) const
+{
+    uint64_t b = ((uint64_t)g.V + 1) * (uint64_t)W / 256 + 1;
+    return (uint32_t)(b > 2048 ? 2048 : b);
+}
+
+// FW and/or BW search from the seeded queues until both frontiers are empty.
+scc_status Engine::reach(bool fw, bool bw, bool single, int expand, int W, uint32_t *levels_out)
+{
+    LevelArgs a;
+    std::memset(&a, 0, sizeof a);
+    DirArgs F{g.Fr, g.Fc, st.fvis, nullptr, nullptr, st.ctr + C_FWCNT, nullptr, nullptr, nullptr, nullptr};
+    DirArgs B{g.Br, g.Bc, st.bvis, nullptr, nullptr, st.ctr + C_BWCNT, nullptr, nullptr, nullptr, nullptr};
+    a.cls = st.cls;
+    a.ndirs = (fw ? 1 : 0) + (bw ? 1 : 0);
+    if (!a.ndirs) return SCC_OK;
+    const int K = check_every > 0 ? check_every : 2;
+    const uint32_t blocks = bpd(expand == SCC_EXPAND_VWARP ? W : 1);
+    uint32_t level = 0;
+    for (;;) {
+        const int in = level & 1, out = in ^ 1;
+        F.qin = st.qf[in]; F.qout = st.qf[out];
+        B.qin = st.qb[in]; B.qout = st.qb[out];
+        F.big = st.bigf[in]; F.bigoff = st.bigoff_f[in];
+        B.big = st.bigb[in]; B.bigoff = st.bigoff_b[in];
+        F.bigc = reinterpret_cast<unsigned long long *>(st.ctr + C_BIGF) + in;
+        F.bigc_next = reinterpret_cast<unsigned long long *>(st.ctr + C_BIGF) + out;
+        B.bigc = reinterpret_cast<unsigned long long *>(st.ctr + C_BIGB) + in;
+        B.bigc_next = reinterpret_cast<unsigned long long *>(st.ctr + C_BIGB) + out;
+        if (fw && bw) {
+            a.d[0] = F;
+            a.d[1] = B;
+        } else {
+            a.d[0] = fw ? F : B;
+        }
+        a.level = (int)level;
+        k_expand_level(stream, a, expand, W, single, blocks);
+        level++;
+        if (level % K == 0) {
+            scc_status s = read_ctr();
+            if (s != SCC_OK) return s;
+            const uint32_t nf = fw ? h_ctr[C_FWCNT + level % 3] : 0;
+            const uint32_t nb = bw ? h_ctr[C_BWCNT + level % 3] : 0;
+            if (!nf && !nb) break;
+            // every level claims at least one vertex, so V+1 levels bound a search
+            if (level > limit()) return no_convergence("reachability");
+        }
+    }
+    if (levels_out) *levels_out += level;
+    return SCC_OK;
+}
+
+// trim-1 to its fixpoint (scc_WCC.cu:680-687), or one pass.
+scc_status Engine::trim1(bool fixpoint)
+{
+    mark(SCC_PHASE_TRIM1);
+    uint32_t prev;
+    scc_status s = read_ctr();
+    if (s != SCC_OK) return s;
+    uint64_t pass = 0;
+    do {
+        prev = h_ctr[C_TRIM1];
+        k_trim1(stream, g, st);
+        if ((s = read_ctr()) != SCC_OK) return s;
+        if (++pass > limit()) return no_convergence("trim-1");
+    } while (fixpoint && h_ctr[C_TRIM1] != prev);
+    return SCC_OK;
+}
+
+uint64_t Engine::limit() const { return (uint64_t)g.V + 8; }
+
+scc_status Engine::no_convergence(const char *what)
+{
+    return fail(SCC_ERR_DEVICE, "%s did not converge within %llu iterations", what,
+                (unsigned long long)limit());
+}
+
+// One pivot round: elect, seed, FW||BW, update.  Returns live count.
+scc_status Engine::pivot_round(bool first, int expand, int W, scc_stats *stats, uint32_t *live)
+{
+    mark(SCC_PHASE_PIVOT);
+    k_init_vis(stream, g, st);
+    zero_ctr(C_NPIV, 1);
+    k_elect(stream, g, st);
+    k_select(stream, g, st, 1);
+    mark(SCC_PHASE_REACH);
+    const int first_mark = (int)nmarks - 1;
+    scc_status s = reach(true, true, first, expand, W, &stats->levels);
+    if (s != SCC_OK) return s;
+    mark(SCC_PHASE_UPDATE);
+    if (first) {
+        first_reach_marks[0] = first_mark;
+        first_reach_marks[1] = (int)nmarks - 1;
+        k_reach_stats(stream, g, st);
+    }
+    zero_ctr(C_LIVE, 1);
+    k_update(stream, g, st);
+    k_clear_piv(stream, st, g.V + 1);
+    if ((s = read_ctr()) != SCC_OK) return s;
+    *live = h_ctr[C_LIVE];
+    if (first) stats->first_pivot = h_ctr[C_PIV0];
+    return SCC_OK;
+}
+
+scc_status Engine::min_label()
+{
+    mark(SCC_PHASE_WCC);
+    k_minlabel_init(stream, g, st);
+    scc_status s;
+    uint64_t it = 0;
+    do {
+        zero_ctr(C_CHANGED, 1);
+        k_minlabel_prop(stream, g, st);
+        if ((s = read_ctr()) != SCC_OK) return s;
+        if (++it > limit()) return no_convergence("min-label propagation");
+    } while (h_ctr[C_CHANGED]);
+    k_minlabel_finish(stream, g, st);
+    return SCC_OK;
+}
+
+scc_status Engine::color_loop(int expand, int W, scc_stats *stats)
+{
+    scc_status s;
+    for (;;) {
+        if ((s = read_ctr()) != SCC_OK) return s;
+        if (!h_ctr[C_LIVE]) break;
+        if (++stats->rounds > limit()) return no_convergence("colouring rounds");
+        mark(SCC_PHASE_COLOR);
+        k_color_init(stream, g, st);
+        uint64_t it = 0;
+        do {
+            zero_ctr(C_CHANGED, 1);
+            k_color_prop(stream, g, st);
+            if ((s = read_ctr()) != SCC_OK) return s;
+            if (++it > limit()) return no_convergence("colour propagation");
+        } while (h_ctr[C_CHANGED]);
+        k_init_vis(stream, g, st);
+        zero_ctr(C_NPIV, 1);
+        k_color_roots(stream, g, st);
+        mark(SCC_PHASE_REACH);
+        if ((s = reach(true, false, false, expand, W, &stats->levels)) != SCC_OK) return s;
+        mark(SCC_PHASE_UPDATE);
+        zero_ctr(C_LIVE, 1);
+        k_color_finish(stream, g, st);
+    }
+    return SCC_OK;
+}
+
+scc_status Engine::run(const scc_opts &o, scc_stats *out)
+{
+    if (!has_graph) return fail(SCC_ERR_NOTREADY, "no graph uploaded");
+    HIP_TRY(hipSetDevice(device));
+    const char A = o.algo;
+    if (A != 'g' && A != 'h' && A != 'x' && A != 'y' && A != 'd')
+        return fail(SCC_ERR_INVALID, "unknown algorithm '%c'", A);
+    if (A == 'd') return run_detect(o, out);
+    int W = o.warp;
+    if (W != 1 && W != 2 && W != 4 && W != 8 && W != 16 && W != 32 && W != 64)
+        return fail(SCC_ERR_INVALID, "warp size %d not in {1,2,4,8,16,32,64}", W);
+    int expand = o.expand;
+    if (expand == SCC_EXPAND_AUTO) expand = (A == 'x' || A == 'y') ? SCC_EXPAND_VWARP : SCC_EXPAND_VERTEX;
+    if (expand != SCC_EXPAND_VERTEX && expand != SCC_EXPAND_VWARP && expand != SCC_EXPAND_LB)
+        return fail(SCC_ERR_INVALID, "unknown expand mode %d", expand);
+    // the reference uses the virtual warp only for the first pivot's search
+    // (scc_WCC.cu:209-317); the main loop uses vertex-per-thread
+    const int first_expand = expand;
+    const int loop_expand = (expand == SCC_EXPAND_VWARP) ? SCC_EXPAND_VERTEX : expand;
+    check_every = o.check_every;
+
+    scc_stats stats;
+    std::memset(&stats, 0, sizeof stats);
+    nmarks = 0;
+    first_reach_marks[0] = first_reach_marks[1] = -1;
+    mark(-1);
+    k_init_state(stream, g, st, nullptr);
+    zero_ctr(0, C_COUNT);
+    scc_status s;
+    uint32_t live = g.V;
+    const bool hong = (A == 'g' || A == 'x');
+    if (hong) {
+        if (o.trim1 && (s = trim1(true)) != SCC_OK) return s;
+        if ((s = pivot_round(true, first_expand, W, &stats, &live)) != SCC_OK) return s;
+        if (o.trim1 && (s = trim1(true)) != SCC_OK) return s;
+        if (o.trim2) {
+            mark(SCC_PHASE_TRIM2);
+            k_trim2(stream, g, st);
+        }
+        if (o.trim1 && (s = trim1(true)) != SCC_OK) return s;
+        if ((s = min_label()) != SCC_OK) return s;
+        for (;;) {
+            // a round with no live vertex elects nothing and reports live = 0
+            if ((s = pivot_round(false, loop_expand, W, &stats, &live)) != SCC_OK) return s;
+            if (!live) break;
+            // each round finishes at least its pivots' SCCs
+            if (++stats.rounds > limit()) return no_convergence("pivot rounds");
+        }
+    } else {
+        // vSlota/wSlota: one trim-1 pass, -q ignored (scc_Coloring.cu:507-508)
+        if (o.trim1 && (s = trim1(false)) != SCC_OK) return s;
+        if ((s = pivot_round(true, first_expand, W, &stats, &live)) != SCC_OK) return s;
+        if ((s = color_loop(loop_expand, W, &stats)) != SCC_OK) return s;
+    }
+    mark(SCC_PHASE_LABEL);
+    k_canon(stream, g, st);
+    mark(-1);
+    if ((s = read_ctr()) != SCC_OK) return s;
+    if (h_ctr[C_ANOM])
+        return fail(SCC_ERR_DEVICE, "internal invariant violated %u times", h_ctr[C_ANOM]);
+    collect_times(&stats);
+    if (first_reach_marks[0] >= 0) {
+        float ms = 0;
+        (void)hipEventElapsedTime(&ms, events[first_reach_marks[0]], events[first_reach_marks[1]]);
+        stats.ms_first_reach = ms;
+    }
+    auto u64 = [&](int i) { return (uint64_t)h_ctr[i] | ((uint64_t)h_ctr[i + 1] << 32); };
+    stats.num_sccs = h_ctr[C_NSCC];
+    stats.num_trim1 = h_ctr[C_TRIM1];
+    stats.num_trim2 = h_ctr[C_TRIM2];
+    stats.num_pivot = h_ctr[C_PIVOT];
+    stats.largest = h_ctr[C_LARGEST];
+    stats.first_fw_vertices = u64(C_FWV);
+    stats.first_fw_arcs = u64(C_FWE);
+    stats.first_bw_vertices = u64(C_BWV);
+    stats.first_bw_arcs = u64(C_BWE);
+    ran = true;
+    if (out) *out = stats;
+    return SCC_OK;
+}
+
+scc_status Engine::run_detect(const scc_opts &, scc_stats *)
+{
+    return fail(SCC_ERR_UNSUPPORTED, "detectSCC is not available in this build");
+}
+
+scc_status Engine::labels(uint32_t *lab)
+{
+    if (!ran) return fail(SCC_ERR_NOTREADY, "no completed run");
+    HIP_TRY(hipMemcpyAsync(lab, st.lab, sizeof(uint32_t) * ((size_t)g.V + 1), hipMemcpyDeviceToHost, stream));
+    HIP_TRY(hipStreamSynchronize(stream));
+    return SCC_OK;
+}
+
+// --------------------------------------------------- phase entries ---
+
+scc_status Engine::phase_begin(const uint8_t *dead)
+{
+    if (!has_graph) return fail(SCC_ERR_NOTREADY, "no graph uploaded");
+    HIP_TRY(hipSetDevice(device));
+    ran = false;
+    if (dead) HIP_TRY(hipMemcpyAsync(d_tmp8, dead, (size_t)g.V + 1, hipMemcpyHostToDevice, stream));
+    k_init_state(stream, g, st, dead ? d_tmp8 : nullptr);
+    zero_ctr(0, C_COUNT);
+    return SCC_OK;
+}
+
+scc_status Engine::fetch8(uint8_t *out)
+{
+    HIP_TRY(hipMemcpyAsync(out, d_tmp8, (size_t)g.V + 1, hipMemcpyDeviceToHost, stream));
+    HIP_TRY(hipStreamSynchronize(stream));
+    return SCC_OK;
+}
+
+scc_status Engine::fetch32(uint32_t *out, const uint32_t *src)
+{
+    HIP_TRY(hipMemcpyAsync(out, src, sizeof(uint32_t) * ((size_t)g.V + 1), hipMemcpyDeviceToHost, stream));
+    HIP_TRY(hipStreamSynchronize(stream));
+    return SCC_OK;
+}
+
+scc_status Engine::phase_trim1(const uint8_t *dead, uint8_t *removed)
+{
+    scc_status s;
+    if ((s = phase_begin(dead)) != SCC_OK) return s;
+    nmarks = 0;
+    if ((s = trim1(true)) != SCC_OK) return s;
+    k_export_removed(stream, g, st, d_tmp8);
+    if ((s = fetch8(removed)) != SCC_OK) return s;
+    if (dead)
+        for (uint32_t v = 0; v <= g.V; v++) removed[v] = removed[v] && !dead[v];
+    return SCC_OK;
+}
+
+scc_status Engine::phase_trim2(const uint8_t *dead, uint32_t *pair)
+{
+    scc_status s;
+    if ((s = phase_begin(dead)) != SCC_OK) return s;
+    k_trim2(stream, g, st);
+    std::vector<uint32_t> cls((size_t)g.V + 1);
+    if ((s = fetch32(cls.data(), st.cls)) != SCC_OK) return s;
+    if ((s = fetch32(pair, st.lab)) != SCC_OK) return s;
+    for (uint32_t v = 0; v <= g.V; v++)
+        if (v == 0 || cls[v] != DEAD || (dead && dead[v])) pair[v] = 0;
+    return SCC_OK;
+}
+
+scc_status Engine::phase_first_pivot(const uint8_t *dead, uint32_t *pivot)
+{
+    scc_status s;
+    if ((s = phase_begin(dead)) != SCC_OK) return s;
+    k_init_vis(stream, g, st);
+    k_elect(stream, g, st);
+    k_select(stream, g, st, 1);
+    if ((s = read_ctr()) != SCC_OK) return s;
+    *pivot = 0;
+    if (h_ctr[C_NPIV] > 1) return fail(SCC_ERR_DEVICE, "first-pivot election chose %u vertices", h_ctr[C_NPIV]);
+    if (h_ctr[C_NPIV] == 1) HIP_TRY(hipMemcpy(pivot, st.pivlist, sizeof(uint32_t), hipMemcpyDeviceToHost));
+    return SCC_OK;
+}
+
+scc_status Engine::phase_reach(uint32_t src, int dir, int expand, int W, const uint8_t *dead,
+                               uint8_t *out, uint32_t *levels, double *ms)
+{
+    if (!has_graph) return fail(SCC_ERR_NOTREADY, "no graph uploaded");
+    if (src < 1 || src > g.V) return fail(SCC_ERR_INVALID, "source %u outside [1, %u]", src, g.V);
+    if (dead && dead[src]) return fail(SCC_ERR_INVALID, "source %u is marked dead", src);
+    if (expand == SCC_EXPAND_AUTO) expand = SCC_EXPAND_VERTEX;
+    if (W != 1 && W != 2 && W != 4 && W != 8 && W != 16 && W != 32 && W != 64)
+        return fail(SCC_ERR_INVALID, "warp size %d not in {1,2,4,8,16,32,64}", W);
+    scc_status s;
+    if ((s = phase_begin(dead)) != SCC_OK) return s;
+    k_init_vis(stream, g, st);
+    k_seed_single(stream, st, src, dir);
+    nmarks = 0;
+    mark(SCC_PHASE_REACH);
+    uint32_t lv = 0;
+    if ((s = reach(dir == 0, dir != 0, true, expand, W, &lv)) != SCC_OK) return s;
+    mark(-1);
+    scc_stats tmp;
+    collect_times(&tmp);
+    if (ms) *ms = tmp.ms_scc;
+    if (levels) *levels = lv;
+    k_export_visited(stream, g, st, dir, d_tmp8);
+    return fetch8(out);
+}
+
+scc_status Engine::phase_min_label(const uint8_t *dead, uint32_t *out)
+{
+    scc_status s;
+    if ((s = phase_begin(dead)) != SCC_OK) return s;
+    k_minlabel_init(stream, g, st);
+    uint64_t it = 0;
+    do {
+        zero_ctr(C_CHANGED, 1);
+        k_minlabel_prop(stream, g, st);
+        if ((s = read_ctr()) != SCC_OK) return s;
+        if (++it > limit()) return no_convergence("min-label propagation");
+    } while (h_ctr[C_CHANGED]);
+    if ((s = fetch32(out, st.cls)) != SCC_OK) return s;
+    for (uint32_t v = 0; v <= g.V; v++)
+        if (out[v] == DEAD || v == 0) out[v] = 0;
+    return SCC_OK;
+}
+
+scc_status Engine::phase_max_color(const uint8_t *dead, uint32_t *out)
+{
+    scc_status s;
+    if ((s = phase_begin(dead)) != SCC_OK) return s;
+    k_color_init(stream, g, st);
+    uint64_t it = 0;
+    do {
+        zero_ctr(C_CHANGED, 1);
+        k_color_prop(stream, g, st);
+        if ((s = read_ctr()) != SCC_OK) return s;
+        if (++it > limit()) return no_convergence("colour propagation");
+    } while (h_ctr[C_CHANGED]);
+    return fetch32(out, st.aux);
+}
+
+} // namespace sccamd

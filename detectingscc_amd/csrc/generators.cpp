@@ -1,0 +1,226 @@
+// generators.cpp -- seeded synthetic digraphs for tests, benchmarks, the
+// batched mix and the serving stream.  Every random choice is a pure
+// function rnd(seed, stream, index), so results do not depend on the
+// number of host threads.
+#include <cmath>
+#include <cstring>
+#include <unordered_set>
+
+#include "generators.h"
+#include "host_util.h"
+
+namespace sccamd {
+
+// Seeded Fisher-Yates permutation of [0, n).
+std::vector<uint32_t> seeded_permutation(uint32_t n, uint64_t seed)
+{
+    std::vector<uint32_t> p(n);
+    for (uint32_t i = 0; i < n; i++) p[i] = i;
+    for (uint32_t i = n; i-- > 1;) {
+        uint32_t j = (uint32_t)below(rnd(seed, 9, i), (uint64_t)i + 1);
+        std::swap(p[i], p[j]);
+    }
+    return p;
+}
+
+void er_arcs(uint32_t V, uint64_t draws, uint64_t seed, std::vector<uint32_t> &s,
+             std::vector<uint32_t> &d)
+{
+    size_t base = s.size();
+    std::vector<uint32_t> u(draws), v(draws);
+    parallel_chunks(draws, [&](uint64_t b, uint64_t e, int) {
+        for (uint64_t k = b; k < e; k++) {
+            u[k] = 1 + (uint32_t)below(rnd(seed, 1, 2 * k), V);
+            v[k] = 1 + (uint32_t)below(rnd(seed, 1, 2 * k + 1), V);
+        }
+    });
+    s.reserve(base + draws);
+    d.reserve(base + draws);
+    for (uint64_t k = 0; k < draws; k++)
+        if (u[k] != v[k]) {
+            s.push_back(u[k]);
+            d.push_back(v[k]);
+        }
+}
+
+void planted_cycles(uint32_t V, uint32_t cycles, uint32_t cmin, uint32_t cmax, uint64_t seed,
+                    std::vector<uint32_t> &s, std::vector<uint32_t> &d)
+{
+    if (cmax > V) cmax = V;
+    if (cmin < 2) cmin = 2;
+    if (cmin > cmax) return;
+    std::vector<uint32_t> cyc;
+    std::unordered_set<uint32_t> used;
+    for (uint32_t j = 0; j < cycles; j++) {
+        uint32_t L = cmin + (uint32_t)below(rnd(seed, 2, j), (uint64_t)(cmax - cmin + 1));
+        cyc.clear();
+        used.clear();
+        for (uint64_t t = 0; cyc.size() < L; t++) {
+            uint32_t x = 1 + (uint32_t)below(rnd(seed, 3 + (uint64_t)j, t), V);
+            if (used.insert(x).second) cyc.push_back(x);
+        }
+        for (uint32_t i = 0; i < L; i++) {
+            s.push_back(cyc[i]);
+            d.push_back(cyc[(i + 1) % L]);
+        }
+    }
+}
+
+// R-MAT over [1, 2^scale] with 16-bit quadrant thresholds per level; arcs
+// with an endpoint above vcap are redrawn (counter advanced), which gives
+// R-MAT graphs of any vertex count.
+void rmat_arcs(uint32_t scale, uint32_t vcap, uint64_t E, double a, double b, double c,
+               uint64_t seed, const std::vector<uint32_t> *perm, std::vector<uint32_t> &s,
+               std::vector<uint32_t> &d)
+{
+    const uint32_t ta = (uint32_t)std::lround(a * 65536.0);
+    const uint32_t tab = (uint32_t)std::lround((a + b) * 65536.0);
+    const uint32_t tabc = (uint32_t)std::lround((a + b + c) * 65536.0);
+    const uint32_t words = (scale + 3) / 4;
+    size_t base = s.size();
+    s.resize(base + E);
+    d.resize(base + E);
+    parallel_chunks(E, [&](uint64_t lo, uint64_t hi, int) {
+        for (uint64_t i = lo; i < hi; i++) {
+            for (uint64_t attempt = 0;; attempt++) {
+                uint64_t ctr = i + attempt * E;
+                uint32_t r = 0, q = 0, lvl = 0;
+                for (uint32_t w = 0; w < words; w++) {
+                    uint64_t bits = rnd(seed, 10 + w, ctr);
+                    for (int k = 0; k < 4 && lvl < scale; k++, lvl++) {
+                        uint32_t x = (uint32_t)(bits >> (16 * k)) & 0xFFFF;
+                        uint32_t rb = x >= tab, cb = (x >= ta && x < tab) || x >= tabc;
+                        r = (r << 1) | rb;
+                        q = (q << 1) | cb;
+                    }
+                }
+                if (r >= vcap || q >= vcap) continue;
+                if (perm) {
+                    r = (*perm)[r];
+                    q = (*perm)[q];
+                }
+                s[base + i] = r + 1;
+                d[base + i] = q + 1;
+                break;
+            }
+        }
+    });
+}
+
+void grid_arcs(uint32_t side, double p, uint64_t seed, std::vector<uint32_t> &s,
+               std::vector<uint32_t> &d)
+{
+    const uint64_t V = (uint64_t)side * side;
+    std::vector<uint8_t> mask(V);
+    parallel_chunks(V, [&](uint64_t b, uint64_t e, int) {
+        for (uint64_t x = b; x < e; x++) {
+            uint64_t r = x / side, c = x % side, id = x + 1;
+            uint8_t m = 0;
+            // directions in ascending target order: N, W, E, S
+            if (r > 0 && bernoulli(rnd(seed, 20, id * 4 + 0), p)) m |= 1;
+            if (c > 0 && bernoulli(rnd(seed, 20, id * 4 + 1), p)) m |= 2;
+            if (c + 1 < side && bernoulli(rnd(seed, 20, id * 4 + 2), p)) m |= 4;
+            if (r + 1 < side && bernoulli(rnd(seed, 20, id * 4 + 3), p)) m |= 8;
+            mask[x] = m;
+        }
+    });
+    uint64_t n = 0;
+    for (uint64_t x = 0; x < V; x++) n += __builtin_popcount(mask[x]);
+    s.reserve(s.size() + n);
+    d.reserve(d.size() + n);
+    for (uint64_t x = 0; x < V; x++) {
+        uint32_t id = (uint32_t)x + 1;
+        if (mask[x] & 1) { s.push_back(id); d.push_back(id - side); }
+        if (mask[x] & 2) { s.push_back(id); d.push_back(id - 1); }
+        if (mask[x] & 4) { s.push_back(id); d.push_back(id + 1); }
+        if (mask[x] & 8) { s.push_back(id); d.push_back(id + side); }
+    }
+}
+
+} // namespace sccamd
+
+using namespace sccamd;
+
+static scc_status finish(uint32_t V, std::vector<uint32_t> &s, std::vector<uint32_t> &d,
+                         scc_host_graph *out)
+{
+    if (s.size() > 0xFFFFFFFFull) return fail(SCC_ERR_INVALID, "too many arcs");
+    return scc_csr_from_arcs(V, s.size(), s.data(), d.data(), out);
+}
+
+extern "C" {
+
+scc_status scc_gen_er(uint32_t V, uint64_t draws, uint32_t cycles, uint32_t cmin, uint32_t cmax,
+                      uint64_t seed, scc_host_graph *out)
+{
+    clear_error();
+    if (!out || V < 1 || V >= (1u << 30)) return fail(SCC_ERR_INVALID, "bad ER arguments");
+    std::vector<uint32_t> s, d;
+    er_arcs(V, draws, seed, s, d);
+    planted_cycles(V, cycles, cmin, cmax, seed, s, d);
+    return finish(V, s, d, out);
+}
+
+scc_status scc_gen_rmat(uint32_t scale, uint64_t E, double a, double b, double c, uint64_t seed,
+                        scc_host_graph *out)
+{
+    clear_error();
+    if (!out || scale < 1 || scale > 29 || a < 0 || b < 0 || c < 0 || a + b + c > 1.0)
+        return fail(SCC_ERR_INVALID, "bad R-MAT arguments");
+    uint32_t V = 1u << scale;
+    std::vector<uint32_t> perm = seeded_permutation(V, seed);
+    std::vector<uint32_t> s, d;
+    rmat_arcs(scale, V, E, a, b, c, seed, &perm, s, d);
+    return finish(V, s, d, out);
+}
+
+scc_status scc_gen_grid(uint32_t side, double p, uint64_t seed, scc_host_graph *out)
+{
+    clear_error();
+    if (!out || side < 1 || (uint64_t)side * side >= (1u << 30) || p < 0 || p > 1)
+        return fail(SCC_ERR_INVALID, "bad grid arguments");
+    std::vector<uint32_t> s, d;
+    grid_arcs(side, p, seed, s, d);
+    return finish(side * side, s, d, out);
+}
+
+scc_status scc_gen_communities(uint32_t V, uint32_t K, double deg, double inter, uint64_t seed,
+                               scc_host_graph *out, uint32_t *part)
+{
+    clear_error();
+    if (!out || !part || V < 1 || K < 1 || K > V || V >= (1u << 30) || deg < 0 || inter < 0 ||
+        inter > 1)
+        return fail(SCC_ERR_INVALID, "bad community arguments");
+    std::vector<uint32_t> perm = seeded_permutation(V, seed);   // position -> vertex-1
+    std::vector<uint32_t> pos(V);
+    for (uint32_t i = 0; i < V; i++) pos[perm[i]] = i;
+    // community c holds positions [c*V/K, (c+1)*V/K)
+    auto lo = [&](uint64_t c) { return (uint32_t)(c * V / K); };
+    part[0] = 0;
+    part[V + 1] = 0;
+    for (uint32_t v = 0; v < V; v++) part[v + 1] = (uint32_t)(((uint64_t)pos[v] * K) / V);
+    for (uint32_t v = 0; v < V; v++)   // integer rounding can misplace the boundary; fix it
+        while (part[v + 1] + 1 < K && pos[v] >= lo(part[v + 1] + 1)) part[v + 1]++;
+    for (uint32_t v = 0; v < V; v++)
+        while (part[v + 1] > 0 && pos[v] < lo(part[v + 1])) part[v + 1]--;
+    const uint64_t E = (uint64_t)std::llround(deg * V);
+    std::vector<uint32_t> s(E), d(E);
+    parallel_chunks(E, [&](uint64_t b, uint64_t e, int) {
+        for (uint64_t i = b; i < e; i++) {
+            uint32_t u = (uint32_t)below(rnd(seed, 30, i), V);
+            uint32_t v;
+            if (bernoulli(rnd(seed, 31, i), inter)) {
+                v = (uint32_t)below(rnd(seed, 32, i), V);
+            } else {
+                uint32_t c = part[u + 1];
+                uint32_t a0 = lo(c), n = lo(c + 1) - a0;
+                v = perm[a0 + (uint32_t)below(rnd(seed, 32, i), n)];
+            }
+            s[i] = u + 1;
+            d[i] = v + 1;
+        }
+    });
+    return finish(V, s, d, out);
+}
+
+} // extern "C"

@@ -1,0 +1,669 @@
+// kernels_sweep.hip -- vertex-sweep kernels of the SCC engine for gfx950.
+//
+// Every kernel walks vertices 1..V in wave-uniform strides: a wave owns 64
+// consecutive vertices per step, so ballots and cross-lane reductions always
+// see a full wave.  Class and state updates that other lanes may race on use
+// atomics (the reference's byte read-modify-writes, scc.h:28-42, are not).
+#include "scc_dev.h"
+
+namespace sccamd {
+
+#define WAVE_LOOP(v, valid, V)                                                              \
+    for (uint32_t _base = blockIdx.x * blockDim.x + (threadIdx.x & ~63u); _base < (V);     \
+         _base += gridDim.x * blockDim.x)                                                   \
+        for (uint32_t v = _base + (threadIdx.x & 63u) + 1, _once = 1; _once; _once = 0)     \
+            for (bool valid = v <= (V); _once; _once = 0)
+
+__device__ __forceinline__ uint32_t lane() { return threadIdx.x & 63u; }
+
+__device__ __forceinline__ uint64_t pivot_key(const uint32_t *__restrict__ Fr,
+                                              const uint32_t *__restrict__ Br, uint32_t v)
+{
+    uint64_t p = (uint64_t)(Fr[v + 1] - Fr[v]) * (uint64_t)(Br[v + 1] - Br[v]);
+    p = p > 0xFFFFFFFFull ? 0xFFFFFFFFull : p;
+    return (p << 32) | (uint64_t)(0xFFFFFFFFu - v);
+}
+
+__device__ __forceinline__ uint32_t key_vertex(uint64_t k) { return 0xFFFFFFFFu - (uint32_t)k; }
+
+__device__ __forceinline__ bool bit_of(const uint32_t *bm, uint32_t v)
+{
+    return (bm[v >> 5] >> (v & 31)) & 1u;
+}
+
+// Sum over the (fully active) wave, added once by lane 0.
+__device__ __forceinline__ void wave_add32(uint32_t *p, uint32_t x)
+{
+    for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o);
+    if (lane() == 0 && x) atomicAdd(p, x);
+}
+
+__device__ __forceinline__ void wave_add64(uint32_t *p32, uint64_t x)
+{
+    for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o);
+    if (lane() == 0 && x) atomicAdd(reinterpret_cast<unsigned long long *>(p32), (unsigned long long)x);
+}
+
+uint32_t sweep_blocks(uint32_t V)
+{
+    uint64_t b = ((uint64_t)V + SWEEP_BLOCK) / SWEEP_BLOCK;
+    return (uint32_t)(b < 1 ? 1 : (b > 4096 ? 4096 : b));
+}
+
+// ------------------------------------------------------------ init ---
+
+__global__ void fill_u32_kernel(uint32_t *p, uint32_t val, uint64_t n)
+{
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
+         i += (uint64_t)gridDim.x * blockDim.x)
+        p[i] = val;
+}
+
+void k_fill_u32(hipStream_t s, uint32_t *p, uint32_t val, uint64_t n)
+{
+    uint64_t b = (n + SWEEP_BLOCK - 1) / SWEEP_BLOCK;
+    fill_u32_kernel<<<(uint32_t)(b < 1 ? 1 : (b > 4096 ? 4096 : b)), SWEEP_BLOCK, 0, s>>>(p, val, n);
+}
+
+// All vertices start live in class 0 (range 0, scc_WCC.cu:630-631) unless
+// `dead` marks them finished; lab[v] = v.
+__global__ void init_state_kernel(uint32_t *cls, uint32_t *lab, uint64_t *piv, const uint8_t *dead,
+                                  uint32_t V)
+{
+    uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
+    if (tid == 0) {
+        cls[0] = DEAD;
+        lab[0] = 0;
+        piv[0] = 0;
+    }
+    for (uint32_t v = tid + 1; v <= V; v += gridDim.x * blockDim.x) {
+        cls[v] = (dead && dead[v]) ? DEAD : 0u;
+        lab[v] = v;
+    }
+}
+
+void k_init_state(hipStream_t s, const DevGraph &g, DevState &st, const uint8_t *dead_dev)
+{
+    init_state_kernel<<<sweep_blocks(g.V), SWEEP_BLOCK, 0, s>>>(st.cls, st.lab, st.piv, dead_dev, g.V);
+}
+
+// ---------------------------------------------------------- trim-1 ---
+// scc_kernels.cu:187-233: a live vertex with no live same-class
+// in-neighbour, or no live same-class out-neighbour, is a singleton SCC.
+
+__global__ void trim1_kernel(const uint32_t *__restrict__ Fr, const uint32_t *__restrict__ Fc,
+                             const uint32_t *__restrict__ Br, const uint32_t *__restrict__ Bc,
+                             uint32_t *cls, uint32_t *lab, uint32_t V, uint32_t *ctr)
+{
+    uint32_t removed = 0;
+    WAVE_LOOP(v, valid, V)
+    {
+        if (!valid) continue;
+        const uint32_t c = cls[v];
+        if (c == DEAD) continue;
+        bool keep = false;
+        for (uint32_t i = Br[v], e = Br[v + 1]; i < e; i++)
+            if (cls[Bc[i]] == c) { keep = true; break; }
+        if (keep) {
+            keep = false;
+            for (uint32_t i = Fr[v], e = Fr[v + 1]; i < e; i++)
+                if (cls[Fc[i]] == c) { keep = true; break; }
+        }
+        if (!keep) {
+            lab[v] = v;
+            __hip_atomic_store(&cls[v], DEAD, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            removed++;
+        }
+    }
+    wave_add32(&ctr[C_TRIM1], removed);
+}
+
+void k_trim1(hipStream_t s, const DevGraph &g, DevState &st)
+{
+    trim1_kernel<<<sweep_blocks(g.V), SWEEP_BLOCK, 0, s>>>(g.Fr, g.Fc, g.Br, g.Bc, st.cls, st.lab,
+                                                           g.V, st.ctr);
+}
+
+// ---------------------------------------------------------- trim-2 ---
+// scc_kernels.cu:236-357.  Live same-class degree capped at 2; *k gets
+// the last live neighbour seen.
+__device__ __forceinline__ int live_deg2(const uint32_t *__restrict__ R,
+                                         const uint32_t *__restrict__ C, const uint32_t *cls,
+                                         uint32_t v, uint32_t c, uint32_t *k)
+{
+    int d = 0;
+    for (uint32_t i = R[v], e = R[v + 1]; i < e; i++) {
+        uint32_t w = C[i];
+        if (cls[w] != c) continue;
+        if (++d == 2) return 2;
+        *k = w;
+    }
+    return d;
+}
+
+__device__ __forceinline__ bool has_arc(const uint32_t *__restrict__ R,
+                                        const uint32_t *__restrict__ C, uint32_t v, uint32_t k)
+{
+    for (uint32_t i = R[v], e = R[v + 1]; i < e; i++)
+        if (C[i] == k) return true;
+    return false;
+}
+
+// A pair {v,k} whose only live same-class in-arcs (or out-arcs) are the
+// ones between them is an SCC.  The claim is a CAS on the smaller id, so the
+// pair is recorded once and no flag update is lost (defect D5).
+__global__ void trim2_kernel(const uint32_t *__restrict__ Fr, const uint32_t *__restrict__ Fc,
+                             const uint32_t *__restrict__ Br, const uint32_t *__restrict__ Bc,
+                             uint32_t *cls, uint32_t *lab, uint32_t V, uint32_t *ctr)
+{
+    uint32_t pairs = 0;
+    WAVE_LOOP(v, valid, V)
+    {
+        if (!valid) continue;
+        const uint32_t c = cls[v];
+        if (c == DEAD) continue;
+        uint32_t k = 0, kk = 0;
+        bool hit = live_deg2(Br, Bc, cls, v, c, &k) == 1 && has_arc(Fr, Fc, v, k) &&
+                   live_deg2(Br, Bc, cls, k, c, &kk) == 1;
+        if (!hit) {
+            k = 0;
+            hit = live_deg2(Fr, Fc, cls, v, c, &k) == 1 && has_arc(Br, Bc, v, k) &&
+                  live_deg2(Fr, Fc, cls, k, c, &kk) == 1;
+        }
+        if (hit) {
+            uint32_t lo = v < k ? v : k, hi = v < k ? k : v;
+            if (atomicCAS(&cls[lo], c, DEAD) == c) {
+                lab[lo] = lo;
+                lab[hi] = lo;
+                __hip_atomic_store(&cls[hi], DEAD, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                pairs++;
+            }
+        }
+    }
+    wave_add32(&ctr[C_TRIM2], pairs);
+}
+
+void k_trim2(hipStream_t s, const DevGraph &g, DevState &st)
+{
+    trim2_kernel<<<sweep_blocks(g.V), SWEEP_BLOCK, 0, s>>>(g.Fr, g.Fc, g.Br, g.Bc, st.cls, st.lab,
+                                                           g.V, st.ctr);
+}
+
+// ------------------------------------------------- visited bitmaps ---
+// Bits of finished vertices (and of slot 0 and the padding) start set.
+__global__ void init_vis_kernel(const uint32_t *__restrict__ cls, uint32_t *fvis, uint32_t *bvis,
+                                uint32_t V, uint32_t nchunks)
+{
+    const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const uint32_t nwaves = (gridDim.x * blockDim.x) >> 6;
+    for (uint32_t ch = wave; ch < nchunks; ch += nwaves) {
+        uint32_t v = ch * 64 + lane();
+        bool dead = v > V || cls[v] == DEAD;
+        uint64_t m = __ballot(dead);
+        if (lane() == 0) {
+            fvis[2 * ch] = (uint32_t)m;
+            fvis[2 * ch + 1] = (uint32_t)(m >> 32);
+        } else if (lane() == 1) {
+            bvis[2 * ch] = (uint32_t)m;
+            bvis[2 * ch + 1] = (uint32_t)(m >> 32);
+        }
+    }
+}
+
+void k_init_vis(hipStream_t s, const DevGraph &g, DevState &st)
+{
+    uint32_t nchunks = st.nwords / 2;
+    uint32_t blocks = (nchunks + 3) / 4;
+    blocks = blocks < 1 ? 1 : (blocks > 4096 ? 4096 : blocks);
+    init_vis_kernel<<<blocks, SWEEP_BLOCK, 0, s>>>(st.cls, st.fvis, st.bvis, g.V, nchunks);
+}
+
+// ------------------------------------------------------------ pivots ---
+// pollForPivots / pollForFirstPivot (scc_kernels.cu:57-74, 643-656) with a
+// 64-bit key so the winner is the true argmax and every class elects one
+// vertex (defects D4, D10).  A wave whose live lanes share a class does one
+// atomic.
+__global__ void elect_kernel(const uint32_t *__restrict__ Fr, const uint32_t *__restrict__ Br,
+                             const uint32_t *__restrict__ cls, unsigned long long *piv, uint32_t V)
+{
+    WAVE_LOOP(v, valid, V)
+    {
+        const uint32_t c = valid ? cls[v] : DEAD;
+        const bool ok = c != DEAD;
+        const uint64_t live = __ballot(ok);
+        if (!live) continue;
+        const int L = __ffsll((unsigned long long)live) - 1;
+        const uint32_t c0 = __shfl(c, L);
+        const unsigned long long key = ok ? pivot_key(Fr, Br, v) : 0ull;
+        if (__all(!ok || c == c0)) {
+            unsigned long long m = key;
+            for (int o = 32; o > 0; o >>= 1) {
+                unsigned long long x = __shfl_xor(m, o);
+                m = x > m ? x : m;
+            }
+            if ((int)lane() == L && m > piv[c0]) atomicMax(&piv[c0], m);
+        } else if (ok && key > piv[c]) {
+            atomicMax(&piv[c], key);
+        }
+    }
+}
+
+void k_elect(hipStream_t s, const DevGraph &g, DevState &st)
+{
+    elect_kernel<<<sweep_blocks(g.V), SWEEP_BLOCK, 0, s>>>(
+        g.Fr, g.Br, st.cls, reinterpret_cast<unsigned long long *>(st.piv), g.V);
+}
+
+// selectPivots (scc_kernels.cu:4-19): the winner of each class seeds the
+// FW (and BW) frontier.
+__global__ void select_kernel(const uint32_t *__restrict__ Fr, const uint32_t *__restrict__ Br,
+                              const uint32_t *__restrict__ cls, const uint64_t *__restrict__ piv,
+                              uint32_t *fvis, uint32_t *bvis, uint32_t *qf, uint32_t *qb,
+                              uint32_t *pivlist, uint32_t *ctr, uint32_t V, int seed_bw)
+{
+    WAVE_LOOP(v, valid, V)
+    {
+        if (!valid) continue;
+        const uint32_t c = cls[v];
+        if (c == DEAD || piv[c] != pivot_key(Fr, Br, v)) continue;
+        uint32_t i = atomicAdd(&ctr[C_NPIV], 1u);
+        pivlist[i] = v;
+        qf[i] = v;
+        atomicOr(&fvis[v >> 5], 1u << (v & 31));
+        if (seed_bw) {
+            qb[i] = v;
+            atomicOr(&bvis[v >> 5], 1u << (v & 31));
+        }
+    }
+}
+
+// Queue counts for level 0 and the two counters after it.
+__global__ void seed_counts_kernel(uint32_t *ctr, const uint32_t *pivlist, int seed_bw)
+{
+    uint32_t n = ctr[C_NPIV];
+    ctr[C_PIV0] = n ? pivlist[0] : 0u;
+    ctr[C_FWCNT + 0] = n;
+    ctr[C_FWCNT + 1] = 0;
+    ctr[C_FWCNT + 2] = 0;
+    ctr[C_BWCNT + 0] = seed_bw ? n : 0;
+    ctr[C_BWCNT + 1] = 0;
+    ctr[C_BWCNT + 2] = 0;
+    unsigned long long *b = reinterpret_cast<unsigned long long *>(ctr + C_BIGF);
+    b[0] = b[1] = b[2] = b[3] = 0;
+}
+
+void k_select(hipStream_t s, const DevGraph &g, DevState &st, int seed_bw)
+{
+    select_kernel<<<sweep_blocks(g.V), SWEEP_BLOCK, 0, s>>>(g.Fr, g.Br, st.cls, st.piv, st.fvis,
+                                                            st.bvis, st.qf[0], st.qb[0],
+                                                            st.pivlist, st.ctr, g.V, seed_bw);
+    seed_counts_kernel<<<1, 1, 0, s>>>(st.ctr, st.pivlist, seed_bw);
+}
+
+// A single source for the phase entry point: vertex src seeds direction dir.
+__global__ void seed_single_kernel(uint32_t *vis, uint32_t *q, uint32_t *ctr, uint32_t src, int dir)
+{
+    q[0] = src;
+    vis[src >> 5] |= 1u << (src & 31);
+    for (int i = 0; i < 3; i++) ctr[C_FWCNT + i] = ctr[C_BWCNT + i] = 0;
+    ctr[(dir ? C_BWCNT : C_FWCNT) + 0] = 1;
+    unsigned long long *b = reinterpret_cast<unsigned long long *>(ctr + C_BIGF);
+    b[0] = b[1] = b[2] = b[3] = 0;
+}
+
+void k_seed_single(hipStream_t s, DevState &st, uint32_t src, int dir)
+{
+    seed_single_kernel<<<1, 1, 0, s>>>(dir ? st.bvis : st.fvis, dir ? st.qb[0] : st.qf[0], st.ctr,
+                                       src, dir);
+}
+
+// ------------------------------------------------------------ update ---
+// update (scc_kernels.cu:124-142): FW and BW visited -> the pivot's SCC;
+// otherwise the class splits into 3*pivot + (fw + 2*bw), which never wraps
+// (defect D9) because each vertex is a pivot at most once.
+__global__ void update_kernel(uint32_t *cls, uint32_t *lab, const uint64_t *__restrict__ piv,
+                              const uint32_t *__restrict__ fvis, const uint32_t *__restrict__ bvis,
+                              uint32_t *ctr, uint32_t V)
+{
+    uint32_t live = 0, found = 0;
+    WAVE_LOOP(v, valid, V)
+    {
+        if (!valid) continue;
+        const uint32_t c = cls[v];
+        if (c == DEAD) continue;
+        uint32_t p = key_vertex(piv[c]);
+        if (p < 1 || p > V) {          // no election for this class: cannot happen
+            atomicAdd(&ctr[C_ANOM], 1u);
+            p = v;
+        }
+        const uint32_t f = bit_of(fvis, v), b = bit_of(bvis, v);
+        if (f && b) {
+            lab[v] = p;
+            cls[v] = DEAD;
+            found += v == p;
+        } else {
+            cls[v] = 3u * p + f + 2u * b;
+            live++;
+        }
+    }
+    wave_add32(&ctr[C_LIVE], live);
+    wave_add32(&ctr[C_PIVOT], found);
+}
+
+void k_update(hipStream_t s, const DevGraph &g, DevState &st)
+{
+    update_kernel<<<sweep_blocks(g.V), SWEEP_BLOCK, 0, s>>>(st.cls, st.lab, st.piv, st.fvis,
+                                                            st.bvis, st.ctr, g.V);
+}
+
+// The next round's classes are 3p+k for this round's pivots p: clear them.
+__global__ void clear_piv_kernel(const uint32_t *__restrict__ pivlist, uint64_t *piv,
+                                 const uint32_t *ctr)
+{
+    const uint32_t n = ctr[C_NPIV];
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        uint32_t p = pivlist[i];
+        piv[3ull * p] = 0;
+        piv[3ull * p + 1] = 0;
+        piv[3ull * p + 2] = 0;
+    }
+}
+
+void k_clear_piv(hipStream_t s, DevState &st, uint32_t max_pivots)
+{
+    uint32_t b = (max_pivots + SWEEP_BLOCK - 1) / SWEEP_BLOCK;
+    b = b < 1 ? 1 : (b > 1024 ? 1024 : b);
+    clear_piv_kernel<<<b, SWEEP_BLOCK, 0, s>>>(st.pivlist, st.piv, st.ctr);
+}
+
+// |R| and |E_R| of the first FW and BW searches (before update).
+__global__ void reach_stats_kernel(const uint32_t *__restrict__ Fr, const uint32_t *__restrict__ Br,
+                                   const uint32_t *__restrict__ cls,
+                                   const uint32_t *__restrict__ fvis,
+                                   const uint32_t *__restrict__ bvis, uint32_t *ctr, uint32_t V)
+{
+    uint64_t fv = 0, fe = 0, bv = 0, be = 0;
+    WAVE_LOOP(v, valid, V)
+    {
+        if (!valid || cls[v] == DEAD) continue;
+        if (bit_of(fvis, v)) { fv++; fe += Fr[v + 1] - Fr[v]; }
+        if (bit_of(bvis, v)) { bv++; be += Br[v + 1] - Br[v]; }
+    }
+    wave_add64(&ctr[C_FWV], fv);
+    wave_add64(&ctr[C_FWE], fe);
+    wave_add64(&ctr[C_BWV], bv);
+    wave_add64(&ctr[C_BWE], be);
+}
+
+void k_reach_stats(hipStream_t s, const DevGraph &g, DevState &st)
+{
+    reach_stats_kernel<<<sweep_blocks(g.V), SWEEP_BLOCK, 0, s>>>(g.Fr, g.Br, st.cls, st.fvis,
+                                                                 st.bvis, st.ctr, g.V);
+}
+
+// -------------------------------------------------- "WCC" min label ---
+// assignUniqueRange + propagateRange1/2 (scc_kernels.cu:477-536): label =
+// min id reachable over live out-arcs; pointer jumping follows label
+// chains.  DEAD is the largest value, so finished vertices never win a min.
+__global__ void minlabel_init_kernel(uint32_t *cls, uint32_t V)
+{
+    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x + 1; v <= V; v += gridDim.x * blockDim.x)
+        if (cls[v] != DEAD) cls[v] = v;
+}
+
+void k_minlabel_init(hipStream_t s, const DevGraph &g, DevState &st)
+{
+    minlabel_init_kernel<<<sweep_blocks(g.V), SWEEP_BLOCK, 0, s>>>(st.cls, g.V);
+}
+
+__global__ void minlabel_prop_kernel(const uint32_t *__restrict__ Fr,
+                                     const uint32_t *__restrict__ Fc, uint32_t *cls, uint32_t *ctr,
+                                     uint32_t V)
+{
+    uint32_t changed = 0;
+    WAVE_LOOP(v, valid, V)
+    {
+        if (!valid) continue;
+        const uint32_t m0 = cls[v];
+        if (m0 == DEAD) continue;
+        uint32_t m = m0;
+        for (uint32_t i = Fr[v], e = Fr[v + 1]; i < e; i++) {
+            uint32_t x = cls[Fc[i]];
+            m = x < m ? x : m;
+        }
+        for (int it = 0; it < 32; it++) {
+            uint32_t x = cls[m];
+            if (x >= m) break;
+            m = x;
+        }
+        if (m < m0) {
+            __hip_atomic_store(&cls[v], m, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            changed = 1;
+        }
+    }
+    wave_add32(&ctr[C_CHANGED], changed);
+}
+
+void k_minlabel_prop(hipStream_t s, const DevGraph &g, DevState &st)
+{
+    minlabel_prop_kernel<<<sweep_blocks(g.V), SWEEP_BLOCK, 0, s>>>(g.Fr, g.Fc, st.cls, st.ctr, g.V);
+}
+
+// class = 3 * label; the class representative clears its pivot slot.
+__global__ void minlabel_finish_kernel(uint32_t *cls, uint64_t *piv, uint32_t V)
+{
+    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x + 1; v <= V; v += gridDim.x * blockDim.x) {
+        uint32_t m = cls[v];
+        if (m == DEAD) continue;
+        if (m == v) piv[3ull * v] = 0;
+        cls[v] = 3u * m;
+    }
+}
+
+void k_minlabel_finish(hipStream_t s, const DevGraph &g, DevState &st)
+{
+    minlabel_finish_kernel<<<sweep_blocks(g.V), SWEEP_BLOCK, 0, s>>>(st.cls, st.piv, g.V);
+}
+
+// ----------------------------------------------------- colouring ---
+// colorPropagation (scc_kernels.cu:539-564): colour = max id reachable over
+// live out-arcs, with pointer jumping added.  aux[v] = 0 for finished v.
+__global__ void color_init_kernel(const uint32_t *__restrict__ cls, uint32_t *aux, uint32_t V)
+{
+    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v <= V; v += gridDim.x * blockDim.x)
+        aux[v] = (v && cls[v] != DEAD) ? v : 0u;
+}
+
+void k_color_init(hipStream_t s, const DevGraph &g, DevState &st)
+{
+    color_init_kernel<<<sweep_blocks(g.V), SWEEP_BLOCK, 0, s>>>(st.cls, st.aux, g.V);
+}
+
+__global__ void color_prop_kernel(const uint32_t *__restrict__ Fr, const uint32_t *__restrict__ Fc,
+                                  uint32_t *aux, uint32_t *ctr, uint32_t V)
+{
+    uint32_t changed = 0;
+    WAVE_LOOP(v, valid, V)
+    {
+        if (!valid) continue;
+        const uint32_t m0 = aux[v];
+        if (!m0) continue;
+        uint32_t m = m0;
+        for (uint32_t i = Fr[v], e = Fr[v + 1]; i < e; i++) {
+            uint32_t x = aux[Fc[i]];
+            m = x > m ? x : m;
+        }
+        for (int it = 0; it < 32; it++) {
+            uint32_t x = aux[m];
+            if (x <= m) break;
+            m = x;
+        }
+        if (m > m0) {
+            __hip_atomic_store(&aux[v], m, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            changed = 1;
+        }
+    }
+    wave_add32(&ctr[C_CHANGED], changed);
+}
+
+void k_color_prop(hipStream_t s, const DevGraph &g, DevState &st)
+{
+    color_prop_kernel<<<sweep_blocks(g.V), SWEEP_BLOCK, 0, s>>>(g.Fr, g.Fc, st.aux, st.ctr, g.V);
+}
+
+// selectPivotColoring (scc_kernels.cu:567-580): class = colour; a vertex
+// whose colour is its own id is a root and seeds the FW frontier.
+__global__ void color_roots_kernel(uint32_t *cls, const uint32_t *__restrict__ aux, uint32_t *fvis,
+                                   uint32_t *qf, uint32_t *pivlist, uint32_t *ctr, uint32_t V)
+{
+    WAVE_LOOP(v, valid, V)
+    {
+        if (!valid || cls[v] == DEAD) continue;
+        const uint32_t c = aux[v];
+        cls[v] = c;
+        if (c == v) {
+            uint32_t i = atomicAdd(&ctr[C_NPIV], 1u);
+            pivlist[i] = v;
+            qf[i] = v;
+            atomicOr(&fvis[v >> 5], 1u << (v & 31));
+        }
+    }
+}
+
+void k_color_roots(hipStream_t s, const DevGraph &g, DevState &st)
+{
+    color_roots_kernel<<<sweep_blocks(g.V), SWEEP_BLOCK, 0, s>>>(st.cls, st.aux, st.fvis, st.qf[0],
+                                                                 st.pivlist, st.ctr, g.V);
+    seed_counts_kernel<<<1, 1, 0, s>>>(st.ctr, st.pivlist, 0);
+}
+
+// fwdColoring marks each vertex it expands as finished (scc_kernels.cu:607):
+// the vertices of colour r reached from root r form SCC(r).
+__global__ void color_finish_kernel(uint32_t *cls, uint32_t *lab, const uint32_t *__restrict__ fvis,
+                                    uint32_t *ctr, uint32_t V)
+{
+    uint32_t live = 0, found = 0;
+    WAVE_LOOP(v, valid, V)
+    {
+        if (!valid) continue;
+        const uint32_t c = cls[v];
+        if (c == DEAD) continue;
+        if (bit_of(fvis, v)) {
+            lab[v] = c;
+            cls[v] = DEAD;
+            found += c == v;
+        } else {
+            live++;
+        }
+    }
+    wave_add32(&ctr[C_LIVE], live);
+    wave_add32(&ctr[C_PIVOT], found);
+}
+
+void k_color_finish(hipStream_t s, const DevGraph &g, DevState &st)
+{
+    color_finish_kernel<<<sweep_blocks(g.V), SWEEP_BLOCK, 0, s>>>(st.cls, st.lab, st.fvis, st.ctr,
+                                                                  g.V);
+}
+
+// ------------------------------------------------ canonical labels ---
+// lab[v] -> min vertex id with the same representative; count roots and
+// the largest SCC.  A wave's lanes that share the first lane's label
+// combine into one atomic (the giant SCC would otherwise serialise).
+__global__ void canon_reset_kernel(uint32_t *aux, uint32_t *cnt, uint32_t V)
+{
+    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v <= V; v += gridDim.x * blockDim.x) {
+        aux[v] = DEAD;
+        cnt[v] = 0;
+    }
+}
+
+__global__ void canon_min_kernel(uint32_t *lab, uint32_t *aux, uint32_t *ctr, uint32_t V)
+{
+    WAVE_LOOP(v, valid, V)
+    {
+        uint32_t r = valid ? lab[v] : DEAD;
+        if (valid && (r < 1 || r > V)) {   // every vertex must hold a representative
+            atomicAdd(&ctr[C_ANOM], 1u);
+            lab[v] = r = v;
+        }
+        const uint32_t r0 = __shfl(r, 0);
+        const bool mine = valid && r == r0;
+        uint32_t m = mine ? v : DEAD;
+        for (int o = 32; o > 0; o >>= 1) {
+            uint32_t x = __shfl_xor(m, o);
+            m = x < m ? x : m;
+        }
+        if (lane() == 0 && m != DEAD && m < aux[r0]) atomicMin(&aux[r0], m);
+        if (valid && !mine && v < aux[r]) atomicMin(&aux[r], v);
+    }
+}
+
+__global__ void canon_apply_kernel(uint32_t *lab, const uint32_t *__restrict__ aux, uint32_t *cnt,
+                                   uint32_t *ctr, uint32_t V)
+{
+    uint32_t roots = 0;
+    WAVE_LOOP(v, valid, V)
+    {
+        const uint32_t r = valid ? aux[lab[v]] : DEAD;
+        if (valid) {
+            lab[v] = r;
+            roots += r == v;
+        }
+        const uint32_t r0 = __shfl(r, 0);
+        const bool mine = valid && r == r0;
+        const uint32_t n = (uint32_t)__popcll(__ballot(mine));
+        if (lane() == 0 && n) atomicAdd(&cnt[r0], n);
+        if (valid && !mine) atomicAdd(&cnt[r], 1u);
+    }
+    wave_add32(&ctr[C_NSCC], roots);
+}
+
+__global__ void canon_largest_kernel(const uint32_t *__restrict__ cnt, uint32_t *ctr, uint32_t V)
+{
+    uint32_t m = 0;
+    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x + 1; v <= V; v += gridDim.x * blockDim.x)
+        m = cnt[v] > m ? cnt[v] : m;
+    for (int o = 32; o > 0; o >>= 1) {
+        uint32_t x = __shfl_xor(m, o);
+        m = x > m ? x : m;
+    }
+    if (lane() == 0 && m) atomicMax(&ctr[C_LARGEST], m);
+}
+
+void k_canon(hipStream_t s, const DevGraph &g, DevState &st)
+{
+    const uint32_t b = sweep_blocks(g.V);
+    // cls is all DEAD at this point; it holds the SCC sizes from here on
+    canon_reset_kernel<<<b, SWEEP_BLOCK, 0, s>>>(st.aux, st.cls, g.V);
+    canon_min_kernel<<<b, SWEEP_BLOCK, 0, s>>>(st.lab, st.aux, st.ctr, g.V);
+    canon_apply_kernel<<<b, SWEEP_BLOCK, 0, s>>>(st.lab, st.aux, st.cls, st.ctr, g.V);
+    canon_largest_kernel<<<b, SWEEP_BLOCK, 0, s>>>(st.cls, st.ctr, g.V);
+}
+
+// ------------------------------------------------ phase exports ---
+__global__ void export_visited_kernel(const uint32_t *__restrict__ cls,
+                                      const uint32_t *__restrict__ vis, uint8_t *out, uint32_t V)
+{
+    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v <= V; v += gridDim.x * blockDim.x)
+        out[v] = (v && cls[v] != DEAD && bit_of(vis, v)) ? 1 : 0;
+}
+
+void k_export_visited(hipStream_t s, const DevGraph &g, DevState &st, int dir, uint8_t *out_dev)
+{
+    export_visited_kernel<<<sweep_blocks(g.V), SWEEP_BLOCK, 0, s>>>(st.cls, dir ? st.bvis : st.fvis,
+                                                                    out_dev, g.V);
+}
+
+__global__ void export_removed_kernel(const uint32_t *__restrict__ cls, uint8_t *out, uint32_t V)
+{
+    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v <= V; v += gridDim.x * blockDim.x)
+        out[v] = (v && cls[v] == DEAD) ? 1 : 0;
+}
+
+void k_export_removed(hipStream_t s, const DevGraph &g, DevState &st, uint8_t *out_dev)
+{
+    export_removed_kernel<<<sweep_blocks(g.V), SWEEP_BLOCK, 0, s>>>(st.cls, out_dev, g.V);
+}
+
+} // namespace sccamd

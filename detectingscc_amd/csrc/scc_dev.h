@@ -1,0 +1,121 @@
+// scc_dev.h -- device-side data layout of the SCC engine and the host
+// launchers of its kernels.
+//
+// Per-vertex state in HBM (vertices 1..V, slot 0 is the dummy vertex):
+//   cls[v]  uint32  class id of a live vertex; DEAD (0xFFFFFFFF) once v's SCC
+//                   is known.  After a FW-BW split the ids are
+//                   3*pivot + (fw + 2*bw), after "WCC" 3*min_reachable_id,
+//                   during colouring the colour (a vertex id).  A neighbour
+//                   probe "live and in my class" is one compare.
+//   lab[v]  uint32  representative of v's SCC once finished (pivot id, own id
+//                   for trim-1, pair minimum for trim-2); canonicalised to the
+//                   minimum vertex id at the end.
+//   aux[v]  uint32  scratch: colour during colouring, min id per label at the
+//                   end.
+//   fvis/bvis       one bit per vertex per direction; bits of finished
+//                   vertices are pre-set at the start of a search, so a
+//                   probe of a finished vertex never reads cls.
+//   piv[c]  uint64  per-class pivot key max(min(outdeg*indeg, 2^32-1) << 32 |
+//                   (2^32-1-v)), indexed by class id (3(V+1)+3 entries).
+//   q*              frontier queues, V+1 entries each, two per direction.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace sccamd {
+
+constexpr uint32_t DEAD = 0xFFFFFFFFu;
+constexpr int SWEEP_BLOCK = 256;
+
+// counters block (uint32 words)
+enum : int {
+    C_FWCNT = 0,     // 3 rotating FW queue counts
+    C_BWCNT = 3,     // 3 rotating BW queue counts
+    C_CHANGED = 6,   // fixpoint flag
+    C_LIVE = 7,      // live vertices after an update
+    C_TRIM1 = 8,     // vertices removed by trim-1
+    C_TRIM2 = 9,     // pairs removed by trim-2
+    C_PIVOT = 10,    // SCCs found by pivot / colour-root searches
+    C_NPIV = 11,     // pivots elected this round
+    C_NSCC = 12,     // canonical roots
+    C_LARGEST = 13,  // largest SCC size
+    C_FWV = 14,      // first-search statistics: |R| and |E_R| (64-bit pairs)
+    C_FWE = 16,
+    C_BWV = 18,
+    C_BWE = 20,
+    C_BIGF = 22,     // 2 x 64-bit packed (items, chunks) counts of the FW hub list
+    C_BIGB = 26,     // same for BW
+    C_PIV0 = 30,     // first pivot of the latest election
+    C_ANOM = 31,     // invariant violations caught by kernel guards (must stay 0)
+    C_COUNT = 32
+};
+
+struct DevGraph {
+    uint32_t V = 0, E = 0;
+    uint32_t *Fr = nullptr, *Fc = nullptr, *Br = nullptr, *Bc = nullptr;
+};
+
+struct DevState {
+    uint32_t *cls = nullptr, *lab = nullptr, *aux = nullptr;
+    uint32_t *fvis = nullptr, *bvis = nullptr;
+    uint64_t *piv = nullptr;
+    uint32_t *qf[2] = {nullptr, nullptr}, *qb[2] = {nullptr, nullptr};
+    uint32_t *pivlist = nullptr;   // pivots of the current round
+    uint32_t *bigf[2] = {nullptr, nullptr}, *bigb[2] = {nullptr, nullptr};  // hub lists
+    uint32_t *bigoff_f[2] = {nullptr, nullptr}, *bigoff_b[2] = {nullptr, nullptr};
+    uint32_t *part = nullptr;      // partition ids (detectSCC)
+    uint32_t *ctr = nullptr;       // C_COUNT words
+    uint32_t nwords = 0;           // bitmap words
+};
+
+// One search direction of a BFS level.
+struct DirArgs {
+    const uint32_t *R, *C;       // CSR of this direction
+    uint32_t *vis;               // visited bitmap
+    const uint32_t *qin;
+    uint32_t *qout;
+    uint32_t *cnt;               // the 3 rotating counts
+    uint32_t *big;               // hub list of this level (LB mode)
+    uint32_t *bigoff;            // chunk offset of each hub
+    unsigned long long *bigc;    // packed (items << 32 | chunks) of this level
+    unsigned long long *bigc_next;
+};
+
+struct LevelArgs {
+    DirArgs d[2];
+    const uint32_t *cls;
+    int ndirs;                   // 1 = FW only, 2 = FW and BW
+    int level;                   // selects the rotating counters
+};
+
+// ---- sweep kernels (kernels_sweep.hip) ----
+void k_fill_u32(hipStream_t s, uint32_t *p, uint32_t val, uint64_t n);
+void k_init_state(hipStream_t s, const DevGraph &g, DevState &st, const uint8_t *dead_dev);
+void k_trim1(hipStream_t s, const DevGraph &g, DevState &st);
+void k_trim2(hipStream_t s, const DevGraph &g, DevState &st);
+void k_init_vis(hipStream_t s, const DevGraph &g, DevState &st);
+void k_elect(hipStream_t s, const DevGraph &g, DevState &st);
+void k_select(hipStream_t s, const DevGraph &g, DevState &st, int seed_bw);
+void k_update(hipStream_t s, const DevGraph &g, DevState &st);
+void k_clear_piv(hipStream_t s, DevState &st, uint32_t max_pivots);
+void k_reach_stats(hipStream_t s, const DevGraph &g, DevState &st);
+void k_minlabel_init(hipStream_t s, const DevGraph &g, DevState &st);
+void k_minlabel_prop(hipStream_t s, const DevGraph &g, DevState &st);
+void k_minlabel_finish(hipStream_t s, const DevGraph &g, DevState &st);
+void k_color_init(hipStream_t s, const DevGraph &g, DevState &st);
+void k_color_prop(hipStream_t s, const DevGraph &g, DevState &st);
+void k_color_roots(hipStream_t s, const DevGraph &g, DevState &st);
+void k_color_finish(hipStream_t s, const DevGraph &g, DevState &st);
+void k_canon(hipStream_t s, const DevGraph &g, DevState &st);
+void k_seed_single(hipStream_t s, DevState &st, uint32_t src, int dir);
+void k_export_visited(hipStream_t s, const DevGraph &g, DevState &st, int dir, uint8_t *out_dev);
+void k_export_removed(hipStream_t s, const DevGraph &g, DevState &st, uint8_t *out_dev);
+
+// ---- reachability (kernels_reach.hip) ----
+// expand: SCC_EXPAND_VERTEX / VWARP / LB; warp = lanes per vertex for VWARP
+void k_expand_level(hipStream_t s, const LevelArgs &a, int expand, int warp, bool single_class,
+                    uint32_t blocks_per_dir);
+
+uint32_t sweep_blocks(uint32_t V);
+
+} // namespace sccamd

@@ -1,0 +1,236 @@
+/*
+ * scc_amd.h -- C ABI of the MI355X strongly-connected-components engine.
+ *
+ * Reference: Amit104/DetectingSCC.  Each entry point below names the
+ * reference interface it replaces (file:line).  Conventions:
+ *   - every vertex-indexed array follows the reference layout (A1):
+ *     vertices are 1..V, index 0 is a dummy vertex with no arcs,
+ *     row arrays (Fr, Br) have V+2 entries, column arrays (Fc, Bc) E entries,
+ *     Fc sorted by (src, dst), each in-list of Bc ascending by source;
+ *   - SCC labels are canonical: lab[v] = the smallest vertex id in v's SCC,
+ *     lab[0] = 0; arrays of labels have V+1 entries;
+ *   - functions return scc_status (0 = success) and never exit or throw;
+ *     scc_last_error() gives a message for the calling thread;
+ *   - host buffers are owned by the caller unless documented otherwise;
+ *     device memory is owned by the context; one context per host thread.
+ */
+#ifndef SCC_AMD_H
+#define SCC_AMD_H
+
+#include <stdbool.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+#pragma GCC visibility push(default)
+
+typedef enum {
+    SCC_OK = 0,
+    SCC_ERR_INVALID = 1,     /* bad argument */
+    SCC_ERR_IO = 2,          /* file cannot be opened or read */
+    SCC_ERR_PARSE = 3,       /* malformed GTgraph / partition file */
+    SCC_ERR_NOMEM = 4,       /* host or device allocation failed */
+    SCC_ERR_DEVICE = 5,      /* HIP runtime error */
+    SCC_ERR_BUDGET = 6,      /* request larger than the HBM budget */
+    SCC_ERR_UNSUPPORTED = 7, /* option combination not supported */
+    SCC_ERR_NOTREADY = 8     /* nothing to return yet */
+} scc_status;
+
+const char *scc_status_string(int status);
+const char *scc_last_error(void);
+const char *scc_version(void);
+
+/* ------------------------------------------------------------ graphs --- */
+
+/* Host CSR + reverse CSR in the reference layout; arrays from malloc. */
+typedef struct {
+    uint32_t V;
+    uint32_t E;
+    uint32_t *Fr, *Fc, *Br, *Bc;
+} scc_host_graph;
+
+void scc_host_graph_free(scc_host_graph *g);
+
+/*
+ * GTgraph loader.  Replaces loadFullGraph (include/load.h:6,
+ * src/load.cpp:12-113): skips 7 header lines, reads "p sp V E" and E arcs
+ * "a u v w" (weight ignored), sorts arcs by (u, v), builds CSR and reverse
+ * CSR.  Unlike the reference, ids outside [1, V], short files and bad
+ * headers return SCC_ERR_PARSE, and no partition file is opened (the
+ * hard-coded path of load.cpp:117-122 becomes scc_load_partition).
+ */
+scc_status scc_load_gtgraph(const char *path, scc_host_graph *out);
+
+/* Partition vector (src/load.cpp:117-130): V ids, one per line, into
+ * part[1..V]; part has V+2 entries. */
+scc_status scc_load_partition(const char *path, uint32_t V, uint32_t *part);
+
+/* Arc list -> reference CSR layout (src/load.cpp:47-113), multithreaded. */
+scc_status scc_csr_from_arcs(uint32_t V, uint64_t E, const uint32_t *src, const uint32_t *dst,
+                             scc_host_graph *out);
+
+/* Writers for the on-disk formats: GTgraph text (the loader's input) and
+ * the METIS .graph of src/MetisInput.cpp:115-131 (out-list then in-list). */
+scc_status scc_write_gtgraph(const char *path, const scc_host_graph *g);
+scc_status scc_write_metis(const char *path, const scc_host_graph *g);
+
+/*
+ * Seeded synthetic graphs (no reference counterpart; the reference ships
+ * only sample files).  Counter-based RNG: output does not depend on the
+ * number of host threads.
+ *   ER:    `draws` arcs (u, v) uniform in [1, V]^2, u == v dropped,
+ *          duplicates kept, plus `cycles` directed cycles of length
+ *          uniform in [cmin, cmax] over distinct vertices.
+ *   R-MAT: V = 2^scale, E arcs, quadrant probabilities (a, b, c, 1-a-b-c),
+ *          seeded vertex permutation, self-loops and duplicates kept.
+ *   grid:  side x side, id = r*side + c + 1, each N/S/E/W arc inside the
+ *          grid present with probability p.
+ *   communities: V vertices in K equal planted communities (after a seeded
+ *          permutation); arc count E = V*deg; a fraction `inter` of arcs
+ *          join uniform random vertices, the rest stay in the source's
+ *          community.  part[1..V] = community id (0-based), V+2 entries.
+ */
+scc_status scc_gen_er(uint32_t V, uint64_t draws, uint32_t cycles, uint32_t cmin, uint32_t cmax,
+                      uint64_t seed, scc_host_graph *out);
+scc_status scc_gen_rmat(uint32_t scale, uint64_t E, double a, double b, double c, uint64_t seed,
+                        scc_host_graph *out);
+scc_status scc_gen_grid(uint32_t side, double p, uint64_t seed, scc_host_graph *out);
+scc_status scc_gen_communities(uint32_t V, uint32_t K, double deg, double inter, uint64_t seed,
+                               scc_host_graph *out, uint32_t *part);
+
+/* ------------------------------------------------------- device engine --- */
+
+typedef struct scc_ctx scc_ctx;
+
+/* Algorithms of the reference CLI (src/scc.cu:78-102). */
+#define SCC_ALGO_VHONG  'g' /* vHong  scc_WCC.cu:573-998 */
+#define SCC_ALGO_VSLOTA 'h' /* vSlota scc_Coloring.cu:403-701 */
+#define SCC_ALGO_WHONG  'x' /* wHong  scc_WCC.cu:53-570 */
+#define SCC_ALGO_WSLOTA 'y' /* wSlota scc_Coloring.cu:6-400 */
+#define SCC_ALGO_DETECT 'd' /* detectSCC scc_WCC.cu:1000-1617 */
+
+/* Frontier expansion strategy of the reachability kernels. */
+#define SCC_EXPAND_AUTO   0 /* algorithm default: vertex (g,h), vwarp (x,y) */
+#define SCC_EXPAND_VERTEX 1 /* one lane per frontier vertex (fwd/bwd, scc_kernels.cu:360-445) */
+#define SCC_EXPAND_VWARP  2 /* W lanes per vertex (fwd_warp/bwd_warp, scc_kernels.h:46-105) */
+#define SCC_EXPAND_LB     3 /* workgroup scan-based gather + hub splitting */
+
+typedef struct {
+    char algo;        /* SCC_ALGO_* */
+    int trim1;        /* -p */
+    int trim2;        /* -q */
+    int warp;         /* -w: 1,2,4,8,16,32,64 */
+    int expand;       /* SCC_EXPAND_* */
+    int block;        /* workgroup size for sweep kernels, 0 = 256 */
+    int check_every;  /* BFS levels between host checks, 0 = auto */
+} scc_opts;
+
+void scc_opts_default(scc_opts *o);
+
+#define SCC_PHASE_TRIM1   0
+#define SCC_PHASE_TRIM2   1
+#define SCC_PHASE_PIVOT   2
+#define SCC_PHASE_REACH   3
+#define SCC_PHASE_UPDATE  4
+#define SCC_PHASE_WCC     5
+#define SCC_PHASE_COLOR   6
+#define SCC_PHASE_LOCAL   7
+#define SCC_PHASE_MERGE   8
+#define SCC_PHASE_LABEL   9
+#define SCC_PHASE_COUNT  10
+
+typedef struct {
+    uint32_t num_sccs;       /* "Number Of Sccs" (scc_WCC.cu:966) */
+    uint32_t num_trim1;      /* "numberOf1Sccs" (scc_WCC.cu:948) */
+    uint32_t num_trim2;      /* 2-SCCs removed by trim-2 */
+    uint32_t num_pivot;      /* SCCs found by a pivot / colour-root search */
+    uint32_t largest;        /* size of the largest SCC */
+    uint32_t rounds;         /* outer rounds of the main loop */
+    uint32_t levels;         /* BFS levels launched, both directions */
+    uint32_t first_pivot;    /* vertex elected by the first-pivot rule */
+    uint64_t first_fw_vertices, first_fw_arcs; /* |R|, |E_R| of the first FW */
+    uint64_t first_bw_vertices, first_bw_arcs; /* |R|, |E_R| of the first BW */
+    double ms_scc;           /* CSR resident -> labels resident (device clock) */
+    double ms_first_reach;   /* first-pivot FW||BW (device clock) */
+    double ms_phase[SCC_PHASE_COUNT];
+} scc_stats;
+
+/* Create a context on HIP device `device` (the reference hard-codes
+ * cudaSetDevice(7), scc_WCC.cu:575; here it is a parameter). */
+scc_status scc_ctx_create(int device, scc_ctx **out);
+void scc_ctx_destroy(scc_ctx *ctx);
+
+/* Copy a graph to the device and keep it resident (replaces the
+ * per-driver cudaMalloc + H2D, scc_WCC.cu:608-628). */
+scc_status scc_graph_upload(scc_ctx *ctx, uint32_t V, uint32_t E, const uint32_t *Fr,
+                            const uint32_t *Fc, const uint32_t *Br, const uint32_t *Bc);
+/* Partition vector for SCC_ALGO_DETECT (the Pr of scc_WCC.cu:1072). */
+scc_status scc_partition_upload(scc_ctx *ctx, const uint32_t *part);
+
+/* Run one SCC decomposition on the resident graph; labels stay resident. */
+scc_status scc_run(scc_ctx *ctx, const scc_opts *opts, scc_stats *stats);
+/* Copy the canonical labels of the last run (V+1 entries). */
+scc_status scc_labels(scc_ctx *ctx, uint32_t *lab);
+
+/* Upload + run + labels in one call. */
+scc_status scc_solve(scc_ctx *ctx, const scc_host_graph *g, const uint32_t *part,
+                     const scc_opts *opts, uint32_t *lab, scc_stats *stats);
+
+/* SCC size histogram from labels: pairs (size, count) ascending by size,
+ * at most `cap` pairs; *npairs receives the number of distinct sizes. */
+scc_status scc_size_histogram(uint32_t V, const uint32_t *lab, uint64_t *pairs, uint32_t cap,
+                              uint32_t *npairs);
+
+/* ------------------------------------------------ phase entry points --- */
+/*
+ * Single phases on the resident graph, for kernel-level checks.  `dead`
+ * (V+1 bytes, may be NULL) marks vertices already finished; all other
+ * vertices form one class.
+ */
+/* trim-1 fixpoint (scc_kernels.cu:187-233 iterated); removed[v]=1 */
+scc_status scc_phase_trim1(scc_ctx *ctx, const uint8_t *dead, uint8_t *removed);
+/* one trim-2 pass (scc_kernels.cu:236-357); pair[v] = min id of v's pair, 0 if kept */
+scc_status scc_phase_trim2(scc_ctx *ctx, const uint8_t *dead, uint32_t *pair);
+/* first-pivot rule (scc_kernels.cu:626-656, deterministic key) */
+scc_status scc_phase_first_pivot(scc_ctx *ctx, const uint8_t *dead, uint32_t *pivot);
+/* reachability from src; dir 0 = FW over Fc, 1 = BW over Bc (scc_kernels.cu:360-445,
+ * scc_kernels.h:46-105); out[v]=1 if reached; *levels = BFS levels */
+scc_status scc_phase_reach(scc_ctx *ctx, uint32_t src, int dir, int expand, int warp,
+                           const uint8_t *dead, uint8_t *out, uint32_t *levels, double *ms);
+/* min reachable id (propagateRange1/2, scc_kernels.cu:496-536); 0 for dead */
+scc_status scc_phase_min_label(scc_ctx *ctx, const uint8_t *dead, uint32_t *out);
+/* max reachable id (colorPropagation, scc_kernels.cu:539-564); 0 for dead */
+scc_status scc_phase_max_color(scc_ctx *ctx, const uint8_t *dead, uint32_t *out);
+
+/* --------------------------------------------------- drop-in wrappers --- */
+/*
+ * The five drivers of include/scc.h:49-53 with the same parameters:
+ * CSize = E, RSize = V, reference-layout host arrays.  They run on device 0
+ * and print the reference's stdout lines (scc_WCC.cu:948, 966-967;
+ * scc_Coloring.cu:669-670; scc_WCC.cu:1576-1577).  They return 0 on
+ * success (a caller declaring them void ignores it) and print the error
+ * text instead of throwing (scc_WCC.cu:618-623, scc.cu:104-108).
+ */
+int vHong(uint32_t CSize, uint32_t RSize, uint32_t *Fc, uint32_t *Fr, uint32_t *Bc, uint32_t *Br,
+          bool t1, bool t2);
+int wHong(uint32_t CSize, uint32_t RSize, uint32_t *Fc, uint32_t *Fr, uint32_t *Bc, uint32_t *Br,
+          bool t1, bool t2, int warpSize);
+int vSlota(uint32_t CSize, uint32_t RSize, uint32_t *Fc, uint32_t *Fr, uint32_t *Bc, uint32_t *Br,
+           bool t1, bool t2);
+int wSlota(uint32_t CSize, uint32_t RSize, uint32_t *Fc, uint32_t *Fr, uint32_t *Bc, uint32_t *Br,
+           bool t1, bool t2, int warpSize);
+int detectSCC(uint32_t CSize, uint32_t RSize, uint32_t *Fc, uint32_t *Fr, uint32_t *Bc,
+              uint32_t *Br, uint32_t *Pr, bool t1, bool t2);
+/* loadFullGraph (include/load.h:6) with the reference parameters; *oPr is
+ * set to NULL (no hard-coded partition path); on error it prints
+ * "Error Reading graph file" as load.cpp:26 and zeroes the outputs. */
+void loadFullGraph(const char *filename, uint32_t *oCSize, uint32_t *oRSize, uint32_t **oFc,
+                   uint32_t **oFr, uint32_t **oBc, uint32_t **oBr, uint32_t **oPr);
+
+#pragma GCC visibility pop
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,228 @@
+"""Host-side checks that need no GPU: the C-ABI library loads and exports
+every declared symbol, the GTgraph loader and CSR builder reproduce the
+reference layout (checked against the oracle's restatement of load.cpp),
+generators are seeded and exact, and the CLI's error paths match the
+reference (help text and exit code 1)."""
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+import detectingscc_amd as scc
+from oracle import pyoracle as orc
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+FIXTURES = ["sample0", "sample4", "sample5", "sample1", "sample128", "sample2"]
+
+
+def _declared_functions():
+    text = open(os.path.join(ROOT, "include", "scc_amd.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    text = "\n".join(ln for ln in text.splitlines() if not ln.lstrip().startswith("#"))
+    names = re.findall(r"\b([A-Za-z_][A-Za-z0-9_]*)\s*\(", text)
+    skip = {"if", "while", "for", "sizeof", "return", "defined"}
+    return sorted({n for n in names if n not in skip and not n.startswith("SCC_")})
+
+
+def test_library_exports_every_declared_symbol():
+    L = scc.lib()
+    names = _declared_functions()
+    assert len(names) >= 30
+    missing = [n for n in names if not hasattr(L, n)]
+    assert not missing, missing
+
+
+def test_status_strings_and_version():
+    L = scc.lib()
+    assert L.scc_status_string(0) == b"ok"
+    assert b"gfx950" in L.scc_version()
+
+
+@pytest.mark.parametrize("name", FIXTURES)
+def test_loader_matches_oracle_layout(known, fixture_path, name):
+    path = fixture_path(known["graphs"][name]["file"])
+    g = scc.load_full_graph(path)
+    o = orc.load_gtgraph(path)
+    assert (g.V, g.E) == (o.V, o.E)
+    for a in ["Fr", "Fc", "Br", "Bc"]:
+        assert np.array_equal(getattr(g, a), getattr(o, a)), a
+    # closed form of the layout (load.cpp:47-113)
+    assert g.Fr[0] == 0 and g.Fr[1] == 0 and g.Fr[-1] == g.E and g.Br[-1] == g.E
+    src = np.repeat(np.arange(g.V + 1), np.diff(g.Fr.astype(np.int64)))
+    key = src.astype(np.int64) * (g.V + 1) + g.Fc
+    assert (np.diff(key) >= 0).all()                      # sorted by (src, dst)
+    dst = np.repeat(np.arange(g.V + 1), np.diff(g.Br.astype(np.int64)))
+    key = dst.astype(np.int64) * (g.V + 1) + g.Bc
+    assert (np.diff(key) >= 0).all()                      # in-lists ascending by source
+    assert np.array_equal(np.sort(src.astype(np.int64) * (g.V + 1) + g.Fc),
+                          np.sort(g.Bc.astype(np.int64) * (g.V + 1) + dst))
+
+
+def test_loader_keeps_duplicate_arcs(known, fixture_path):
+    g = scc.load_full_graph(fixture_path("sample0.gr"))
+    # data/sample0.gr lists "a 4 1 1" twice
+    assert list(g.Fc[g.Fr[4]:g.Fr[5]]) == [1, 1]
+
+
+def _write(tmp_path, text, name="g.gr"):
+    p = tmp_path / name
+    p.write_text(text)
+    return str(p)
+
+
+HEADER = "c\nc\nc\nc\nc\nc\nc\n"
+
+
+@pytest.mark.parametrize("body,why", [
+    ("", "missing problem line"),
+    ("p sp 3 2\na 1 2 1\n", "short arc list"),
+    ("p sp 3 1\na 1 4 1\n", "id above V"),
+    ("p sp 3 1\na 0 2 1\n", "id zero"),
+    ("p sp 3 1\na 1 x 1\n", "non-numeric id"),
+    ("p sp x 1\n", "bad V"),
+])
+def test_loader_rejects_malformed(tmp_path, body, why):
+    with pytest.raises(scc.SCCError) as e:
+        scc.load_full_graph(_write(tmp_path, HEADER + body))
+    assert e.value.status == 3, why
+
+
+def test_loader_missing_file():
+    with pytest.raises(scc.SCCError) as e:
+        scc.load_full_graph("/nonexistent/graph.gr")
+    assert e.value.status == 2
+
+
+def test_loader_empty_graph(tmp_path):
+    g = scc.load_full_graph(_write(tmp_path, HEADER + "p sp 4 0\n"))
+    assert g.V == 4 and g.E == 0 and (g.Fr == 0).all() and (g.Br == 0).all()
+
+
+def test_gtgraph_roundtrip(tmp_path, known, fixture_path):
+    g = scc.load_full_graph(fixture_path("sample128.gr"))
+    p = str(tmp_path / "rt.gr")
+    scc.write_gtgraph(p, g)
+    h = scc.load_full_graph(p)
+    for a in ["Fr", "Fc", "Br", "Bc"]:
+        assert np.array_equal(getattr(g, a), getattr(h, a))
+
+
+def test_metis_writer_format(tmp_path, fixture_path):
+    g = scc.load_full_graph(fixture_path("sample0.gr"))
+    p = str(tmp_path / "m.graph")
+    scc.write_metis(p, g)
+    lines = open(p).read().split("\n")
+    assert lines[0] == "6 5"
+    # vertex 1: out {2}, in {4,4}; vertex 4: out {1,1}, in {2}
+    assert lines[1] == "2 4 4 "
+    assert lines[4] == "1 1 2 "
+    assert lines[3] == ""   # vertex 3 is isolated
+
+
+@pytest.mark.parametrize("item", range(3))
+def test_partition_parse_only_files(known, fixture_path, item):
+    pp = known["partition_parse_only"][item]
+    part = scc.load_partition(fixture_path(pp["part"]), pp["lines"])
+    assert part.shape == (pp["lines"] + 2,)
+    o = orc.load_partition(fixture_path(pp["part"]), pp["lines"])
+    assert np.array_equal(part, o)
+
+
+def test_partition_too_short(fixture_path):
+    with pytest.raises(scc.SCCError):
+        scc.load_partition(fixture_path("MeTiSInput0.graph.part.2"), 7)
+
+
+def test_csr_from_arcs_matches_oracle():
+    rng = np.random.default_rng(3)
+    V, E = 3000, 20000
+    s = rng.integers(1, V + 1, E)
+    d = rng.integers(1, V + 1, E)
+    g = scc.from_arcs(V, s, d)
+    o = orc.build_csr(V, s, d)
+    for a in ["Fr", "Fc", "Br", "Bc"]:
+        assert np.array_equal(getattr(g, a), getattr(o, a)), a
+
+
+def test_csr_rejects_bad_ids():
+    with pytest.raises(scc.SCCError):
+        scc.from_arcs(5, [1, 6], [2, 3])
+
+
+def test_generators_seeded_and_thread_independent(monkeypatch):
+    a = scc.gen_rmat(12, seed=5)
+    monkeypatch.setenv("SCC_HOST_THREADS", "3")
+    b = scc.gen_rmat(12, seed=5)
+    monkeypatch.setenv("SCC_HOST_THREADS", "1")
+    c = scc.gen_rmat(12, seed=5)
+    for x in (b, c):
+        assert np.array_equal(a.Fc, x.Fc) and np.array_equal(a.Fr, x.Fr)
+    d = scc.gen_rmat(12, seed=6)
+    assert not np.array_equal(a.Fc, d.Fc)
+
+
+def test_generator_sizes():
+    r = scc.gen_rmat(10)
+    assert r.V == 1024 and r.E == 16 * 1024
+    e = scc.gen_er(1000, 5000, cycles=3, cmin=4, cmax=4, seed=2)
+    assert e.V == 1000 and 5000 - 20 < e.E - 12 <= 5000
+    src = np.repeat(np.arange(e.V + 1), np.diff(e.Fr.astype(np.int64)))
+    loops = int((src == e.Fc).sum())
+    assert loops == 0
+    gr = scc.gen_grid(64, 0.7, seed=1)
+    assert gr.V == 4096
+    expect = 0.7 * 4 * 64 * 63
+    assert abs(gr.E - expect) < 5 * np.sqrt(expect)
+    src = np.repeat(np.arange(gr.V + 1), np.diff(gr.Fr.astype(np.int64))).astype(np.int64)
+    diff = np.abs(src - gr.Fc.astype(np.int64))
+    assert set(np.unique(diff)) <= {1, 64}
+
+
+def test_grid_full_is_one_scc():
+    g = scc.gen_grid(20, 1.0, seed=1)
+    lab, n = orc.tarjan(g)
+    assert n == 1 and g.E == 4 * 20 * 19
+
+
+def test_communities_partition():
+    g = scc.gen_communities(10000, 10, deg=8.0, inter=0.02, seed=5)
+    part = g.part[1:g.V + 1]
+    counts = np.bincount(part)
+    assert counts.size == 10 and counts.min() == counts.max() == 1000
+    src = np.repeat(np.arange(g.V + 1), np.diff(g.Fr.astype(np.int64)))
+    cross = (g.part[src] != g.part[g.Fc]).mean()
+    assert 0.01 < cross < 0.03
+
+
+def test_size_histogram():
+    lab = np.array([0, 1, 1, 3, 1, 5, 5], np.uint32)
+    assert scc.size_histogram(lab) == [(1, 1), (2, 1), (3, 1)]
+    with pytest.raises(scc.SCCError):
+        scc.size_histogram(np.array([0, 2, 2], np.uint32))    # not canonical
+
+
+def _cli(*args):
+    return subprocess.run([scc.CLI_PATH, *args], stdout=subprocess.PIPE, stderr=subprocess.PIPE,
+                          text=True, timeout=60)
+
+
+def test_cli_help_on_too_few_args():
+    r = _cli("-a", "g")
+    assert r.returncode == 1 and r.stdout.startswith("To execute:")
+
+
+def test_cli_unknown_algorithm(fixture_path):
+    r = _cli("-a", "z", "-p", "1", "-q", "1", "-w", "1", "-f", fixture_path("sample0.gr"))
+    assert r.returncode == 1 and "To execute:" in r.stdout
+
+
+def test_cli_missing_file():
+    r = _cli("-a", "g", "-p", "1", "-q", "1", "-w", "1", "-f", "/nonexistent.gr")
+    assert r.returncode == 1 and "Error Reading graph file" in r.stdout
+
+
+def test_cli_detect_needs_partition(fixture_path):
+    r = _cli("-a", "d", "-p", "1", "-q", "1", "-w", "1", "-f", fixture_path("sample0.gr"))
+    assert r.returncode == 1 and "Error Reading metis graph file" in r.stdout
