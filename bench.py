@@ -79,9 +79,9 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="rmat22", choices=sorted(WORKLOADS))
-    ap.add_argument("--algo", default="x")
-    ap.add_argument("--warp", type=int, default=32)
-    ap.add_argument("--expand", type=int, default=0)
+    ap.add_argument("--algo", default="g")
+    ap.add_argument("--warp", type=int, default=1)
+    ap.add_argument("--expand", type=int, default=4, help="0 auto, 1 vertex, 2 vwarp, 3 LB, 4 DO")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-scale", type=int, default=20)
     args = ap.parse_args()
@@ -96,7 +96,7 @@ def main():
           file=sys.stderr, flush=True)
     ctx = scc.Context(0)
     ctx.upload(g)
-    kw = dict(algo=args.algo, warp=args.warp, expand=args.expand)
+    kw = dict(algo=args.algo, warp=args.warp, expand=args.expand, sizes=False)
     for i in range(args.warmup):
         st = ctx.run(**kw)
         print(f"[bench] warmup {i}: {st.ms_scc:.3f} ms device, {st.num_sccs} SCCs", file=sys.stderr, flush=True)

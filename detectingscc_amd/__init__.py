@@ -29,7 +29,7 @@ _u32p = ctypes.POINTER(ctypes.c_uint32)
 _u8p = ctypes.POINTER(ctypes.c_uint8)
 _u64p = ctypes.POINTER(ctypes.c_uint64)
 
-EXPAND_AUTO, EXPAND_VERTEX, EXPAND_VWARP, EXPAND_LB = 0, 1, 2, 3
+EXPAND_AUTO, EXPAND_VERTEX, EXPAND_VWARP, EXPAND_LB, EXPAND_DO = 0, 1, 2, 3, 4
 PHASES = ["trim1", "trim2", "pivot", "reach", "update", "wcc", "color", "local", "merge", "label"]
 WARP_SIZES = (1, 2, 4, 8, 16, 32, 64)
 
@@ -50,7 +50,7 @@ class _HostGraph(ctypes.Structure):
 class _Opts(ctypes.Structure):
     _fields_ = [("algo", ctypes.c_char), ("trim1", ctypes.c_int), ("trim2", ctypes.c_int),
                 ("warp", ctypes.c_int), ("expand", ctypes.c_int), ("block", ctypes.c_int),
-                ("check_every", ctypes.c_int)]
+                ("check_every", ctypes.c_int), ("sizes", ctypes.c_int)]
 
 
 class _Stats(ctypes.Structure):
@@ -233,11 +233,12 @@ def _stats(s: _Stats) -> Stats:
 
 
 def make_opts(algo: str = "g", trim1: bool = True, trim2: bool = True, warp: int = 1,
-              expand: int = EXPAND_AUTO, check_every: int = 0) -> _Opts:
+              expand: int = EXPAND_AUTO, check_every: int = 0, sizes: bool = True) -> _Opts:
     o = _Opts()
     lib().scc_opts_default(ctypes.byref(o))
     o.algo = algo.encode()[:1]
     o.trim1, o.trim2, o.warp, o.expand, o.check_every = int(trim1), int(trim2), int(warp), int(expand), int(check_every)
+    o.sizes = int(sizes)
     return o
 
 
@@ -278,8 +279,8 @@ class Context:
         _check(lib().scc_partition_upload(self._ctx, _p32(part)))
 
     def run(self, algo: str = "g", trim1: bool = True, trim2: bool = True, warp: int = 1,
-            expand: int = EXPAND_AUTO, check_every: int = 0) -> Stats:
-        o = make_opts(algo, trim1, trim2, warp, expand, check_every)
+            expand: int = EXPAND_AUTO, check_every: int = 0, sizes: bool = True) -> Stats:
+        o = make_opts(algo, trim1, trim2, warp, expand, check_every, sizes)
         s = _Stats()
         _check(lib().scc_run(self._ctx, ctypes.byref(o), ctypes.byref(s)))
         return _stats(s)

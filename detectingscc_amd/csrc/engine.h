@@ -45,6 +45,7 @@ class Engine {
     uint32_t bpd(int W) const;
     scc_status reach(bool fw, bool bw, bool single, int expand, int W, uint32_t *levels_out);
     scc_status trim1(bool fixpoint);
+    void init_vis();
     uint64_t limit() const;
     scc_status no_convergence(const char *what);
     scc_status pivot_round(bool first, int expand, int W, scc_stats *stats, uint32_t *live);

@@ -41,7 +41,7 @@ void Engine::release()
         if (p) (void)hipFree(p);
     };
     f(g.Fr); f(g.Fc); f(g.Br); f(g.Bc);
-    f(st.cls); f(st.lab); f(st.aux); f(st.fvis); f(st.bvis); f(st.piv);
+    f(st.cls); f(st.lab); f(st.aux); f(st.fvis); f(st.bvis); f(st.dbits); f(st.piv);
     for (int i = 0; i < 2; i++) {
         f(st.qf[i]); f(st.qb[i]); f(st.bigf[i]); f(st.bigb[i]);
         f(st.bigoff_f[i]); f(st.bigoff_b[i]);
@@ -68,9 +68,9 @@ scc_status Engine::init(int dev)
     device = dev;
     HIP_TRY(hipSetDevice(dev));
     HIP_TRY(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
-    HIP_TRY(hipHostMalloc((void **)&h_ctr, C_COUNT * sizeof(uint32_t), hipHostMallocDefault));
-    HIP_TRY(hipMalloc((void **)&st.ctr, C_COUNT * sizeof(uint32_t)));
-    HIP_TRY(hipMemset(st.ctr, 0, C_COUNT * sizeof(uint32_t)));
+    HIP_TRY(hipHostMalloc((void **)&h_ctr, CT(C_COUNT) * sizeof(uint32_t), hipHostMallocDefault));
+    HIP_TRY(hipMalloc((void **)&st.ctr, CT(C_COUNT) * sizeof(uint32_t)));
+    HIP_TRY(hipMemset(st.ctr, 0, CT(C_COUNT) * sizeof(uint32_t)));
     return SCC_OK;
 }
 
@@ -90,7 +90,7 @@ scc_status Engine::reserve(uint32_t V, uint32_t E)
         if (p) (void)hipFree(p);
     };
     f(g.Fr); f(g.Fc); f(g.Br); f(g.Bc);
-    f(st.cls); f(st.lab); f(st.aux); f(st.fvis); f(st.bvis); f(st.piv);
+    f(st.cls); f(st.lab); f(st.aux); f(st.fvis); f(st.bvis); f(st.dbits); f(st.piv);
     for (int i = 0; i < 2; i++) {
         f(st.qf[i]); f(st.qb[i]); f(st.bigf[i]); f(st.bigb[i]);
         f(st.bigoff_f[i]); f(st.bigoff_b[i]);
@@ -110,7 +110,7 @@ scc_status Engine::reserve(uint32_t V, uint32_t E)
     A(dalloc(&g.Fr, nv + 1)); A(dalloc(&g.Br, nv + 1));
     A(dalloc(&g.Fc, nE)); A(dalloc(&g.Bc, nE));
     A(dalloc(&st.cls, nv)); A(dalloc(&st.lab, nv)); A(dalloc(&st.aux, nv));
-    A(dalloc(&st.fvis, nwords)); A(dalloc(&st.bvis, nwords));
+    A(dalloc(&st.fvis, nwords)); A(dalloc(&st.bvis, nwords)); A(dalloc(&st.dbits, nwords));
     A(dalloc(&st.piv, 3 * nv + 3));
     for (int i = 0; i < 2; i++) {
         A(dalloc(&st.qf[i], nv)); A(dalloc(&st.qb[i], nv));
@@ -184,14 +184,18 @@ scc_status Engine::upload_partition(const uint32_t *part)
 
 scc_status Engine::read_ctr()
 {
-    HIP_TRY(hipMemcpyAsync(h_ctr, st.ctr, C_COUNT * sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
+    HIP_TRY(hipMemcpyAsync(h_ctr, st.ctr, CT(C_COUNT) * sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
     HIP_TRY(hipStreamSynchronize(stream));
     return SCC_OK;
 }
 
+// Zero counters [first, first+n), both words of each (64-bit counters too).
 __global__ void zero_ctr_kernel(uint32_t *ctr, uint32_t first, uint32_t n)
 {
-    if (threadIdx.x < n) ctr[first + threadIdx.x] = 0;
+    if (threadIdx.x < n) {
+        ctr[CT(first + threadIdx.x)] = 0;
+        ctr[CT(first + threadIdx.x) + 1] = 0;
+    }
 }
 
 void Engine::zero_ctr(int first, int n)
@@ -234,44 +238,82 @@ uint32_t Engine::bpd(int W) const
     return (uint32_t)(b > 2048 ? 2048 : b);
 }
 
+void Engine::init_vis()
+{
+    zero_ctr(C_NLIVE, 1);
+    k_init_vis(stream, g, st);
+}
+
+// Direction-optimising switch points (Beamer et al.): go bottom-up when the
+// frontier holds more than 1/kToBottomUp of the vertices still unvisited,
+// back to top-down when it falls under 1/kToTopDown of the live vertices.
+static constexpr int64_t kToBottomUp = 16;
+static constexpr int64_t kToTopDown = 48;
+
 // FW and/or BW search from the seeded queues until both frontiers are empty.
 scc_status Engine::reach(bool fw, bool bw, bool single, int expand, int W, uint32_t *levels_out)
 {
-    LevelArgs a;
-    std::memset(&a, 0, sizeof a);
-    DirArgs F{g.Fr, g.Fc, st.fvis, nullptr, nullptr, st.ctr + C_FWCNT, nullptr, nullptr, nullptr, nullptr};
-    DirArgs B{g.Br, g.Bc, st.bvis, nullptr, nullptr, st.ctr + C_BWCNT, nullptr, nullptr, nullptr, nullptr};
-    a.cls = st.cls;
-    a.ndirs = (fw ? 1 : 0) + (bw ? 1 : 0);
-    if (!a.ndirs) return SCC_OK;
-    const int K = check_every > 0 ? check_every : 2;
-    const uint32_t blocks = bpd(expand == SCC_EXPAND_VWARP ? W : 1);
+    const bool dopt = expand == SCC_EXPAND_DO;
+    const int td_expand = dopt ? SCC_EXPAND_LB : expand;
+    bool active[2] = {fw, bw};
+    bool bottom_up[2] = {false, false};
+    int64_t visited[2] = {0, 0};
+    bool counted_seeds = false;
+    if (!fw && !bw) return SCC_OK;
+    // DO mode decides every level, so it reads the counters every level
+    const int K = dopt ? 1 : (check_every > 0 ? check_every : 2);
+    const uint32_t blocks = bpd(td_expand == SCC_EXPAND_VWARP ? W : 1);
     uint32_t level = 0;
     for (;;) {
         const int in = level & 1, out = in ^ 1;
-        F.qin = st.qf[in]; F.qout = st.qf[out];
-        B.qin = st.qb[in]; B.qout = st.qb[out];
-        F.big = st.bigf[in]; F.bigoff = st.bigoff_f[in];
-        B.big = st.bigb[in]; B.bigoff = st.bigoff_b[in];
-        F.bigc = reinterpret_cast<unsigned long long *>(st.ctr + C_BIGF) + in;
-        F.bigc_next = reinterpret_cast<unsigned long long *>(st.ctr + C_BIGF) + out;
-        B.bigc = reinterpret_cast<unsigned long long *>(st.ctr + C_BIGB) + in;
-        B.bigc_next = reinterpret_cast<unsigned long long *>(st.ctr + C_BIGB) + out;
-        if (fw && bw) {
-            a.d[0] = F;
-            a.d[1] = B;
-        } else {
-            a.d[0] = fw ? F : B;
+        DirArgs D[2];
+        for (int d = 0; d < 2; d++) {
+            const int hub = d ? C_BIGB : C_BIGF;
+            D[d].R = d ? g.Br : g.Fr;
+            D[d].C = d ? g.Bc : g.Fc;
+            D[d].vis = d ? st.bvis : st.fvis;
+            D[d].qin = d ? st.qb[in] : st.qf[in];
+            D[d].qout = d ? st.qb[out] : st.qf[out];
+            D[d].cnt = st.ctr + CT(d ? C_BWCNT : C_FWCNT);
+            D[d].big = d ? st.bigb[in] : st.bigf[in];
+            D[d].bigoff = d ? st.bigoff_b[in] : st.bigoff_f[in];
+            D[d].bigc = reinterpret_cast<unsigned long long *>(st.ctr + CT(hub + in));
+            D[d].bigc_next = reinterpret_cast<unsigned long long *>(st.ctr + CT(hub + out));
         }
+        LevelArgs a;
+        std::memset(&a, 0, sizeof a);
+        a.cls = st.cls;
         a.level = (int)level;
-        k_expand_level(stream, a, expand, W, single, blocks);
+        for (int d = 0; d < 2; d++)
+            if (active[d] && !bottom_up[d]) a.d[a.ndirs++] = D[d];
+        if (a.ndirs) k_expand_level(stream, a, td_expand, W, single, blocks);
+        for (int d = 0; d < 2; d++)
+            if (active[d] && bottom_up[d])
+                k_bottom_up(stream, D[d], d ? g.Fr : g.Br, d ? g.Fc : g.Bc, st.dbits, st.cls, g.V,
+                            (int)level, single);
         level++;
         if (level % K == 0) {
             scc_status s = read_ctr();
             if (s != SCC_OK) return s;
-            const uint32_t nf = fw ? h_ctr[C_FWCNT + level % 3] : 0;
-            const uint32_t nb = bw ? h_ctr[C_BWCNT + level % 3] : 0;
-            if (!nf && !nb) break;
+            const int64_t nlive = h_ctr[CT(C_NLIVE)];
+            bool any = false;
+            for (int d = 0; d < 2; d++) {
+                if (!active[d]) continue;
+                const int64_t nf = h_ctr[CT((d ? C_BWCNT : C_FWCNT) + level % 3)];
+                if (!nf) {
+                    if (dopt) active[d] = false;
+                    continue;
+                }
+                any = true;
+                if (!dopt) continue;
+                if (!counted_seeds) visited[d] += h_ctr[CT(C_NPIV)];
+                visited[d] += nf;
+                const int64_t remaining = nlive - visited[d];
+                if (!bottom_up[d] && nf * kToBottomUp > remaining) bottom_up[d] = true;
+                else if (bottom_up[d] && nf * kToTopDown < nlive) bottom_up[d] = false;
+            }
+            counted_seeds = true;
+            if (!any) break;
             // every level claims at least one vertex, so V+1 levels bound a search
             if (level > limit()) return no_convergence("reachability");
         }
@@ -289,11 +331,11 @@ scc_status Engine::trim1(bool fixpoint)
     if (s != SCC_OK) return s;
     uint64_t pass = 0;
     do {
-        prev = h_ctr[C_TRIM1];
+        prev = h_ctr[CT(C_TRIM1)];
         k_trim1(stream, g, st);
         if ((s = read_ctr()) != SCC_OK) return s;
         if (++pass > limit()) return no_convergence("trim-1");
-    } while (fixpoint && h_ctr[C_TRIM1] != prev);
+    } while (fixpoint && h_ctr[CT(C_TRIM1)] != prev);
     return SCC_OK;
 }
 
@@ -309,7 +351,7 @@ scc_status Engine::no_convergence(const char *what)
 scc_status Engine::pivot_round(bool first, int expand, int W, scc_stats *stats, uint32_t *live)
 {
     mark(SCC_PHASE_PIVOT);
-    k_init_vis(stream, g, st);
+    init_vis();
     zero_ctr(C_NPIV, 1);
     k_elect(stream, g, st);
     k_select(stream, g, st, 1);
@@ -327,8 +369,8 @@ scc_status Engine::pivot_round(bool first, int expand, int W, scc_stats *stats, 
     k_update(stream, g, st);
     k_clear_piv(stream, st, g.V + 1);
     if ((s = read_ctr()) != SCC_OK) return s;
-    *live = h_ctr[C_LIVE];
-    if (first) stats->first_pivot = h_ctr[C_PIV0];
+    *live = h_ctr[CT(C_LIVE)];
+    if (first) stats->first_pivot = h_ctr[CT(C_PIV0)];
     return SCC_OK;
 }
 
@@ -343,7 +385,7 @@ scc_status Engine::min_label()
         k_minlabel_prop(stream, g, st);
         if ((s = read_ctr()) != SCC_OK) return s;
         if (++it > limit()) return no_convergence("min-label propagation");
-    } while (h_ctr[C_CHANGED]);
+    } while (h_ctr[CT(C_CHANGED)]);
     k_minlabel_finish(stream, g, st);
     return SCC_OK;
 }
@@ -353,7 +395,7 @@ scc_status Engine::color_loop(int expand, int W, scc_stats *stats)
     scc_status s;
     for (;;) {
         if ((s = read_ctr()) != SCC_OK) return s;
-        if (!h_ctr[C_LIVE]) break;
+        if (!h_ctr[CT(C_LIVE)]) break;
         if (++stats->rounds > limit()) return no_convergence("colouring rounds");
         mark(SCC_PHASE_COLOR);
         k_color_init(stream, g, st);
@@ -363,8 +405,8 @@ scc_status Engine::color_loop(int expand, int W, scc_stats *stats)
             k_color_prop(stream, g, st);
             if ((s = read_ctr()) != SCC_OK) return s;
             if (++it > limit()) return no_convergence("colour propagation");
-        } while (h_ctr[C_CHANGED]);
-        k_init_vis(stream, g, st);
+        } while (h_ctr[CT(C_CHANGED)]);
+        init_vis();
         zero_ctr(C_NPIV, 1);
         k_color_roots(stream, g, st);
         mark(SCC_PHASE_REACH);
@@ -389,7 +431,8 @@ scc_status Engine::run(const scc_opts &o, scc_stats *out)
         return fail(SCC_ERR_INVALID, "warp size %d not in {1,2,4,8,16,32,64}", W);
     int expand = o.expand;
     if (expand == SCC_EXPAND_AUTO) expand = (A == 'x' || A == 'y') ? SCC_EXPAND_VWARP : SCC_EXPAND_VERTEX;
-    if (expand != SCC_EXPAND_VERTEX && expand != SCC_EXPAND_VWARP && expand != SCC_EXPAND_LB)
+    if (expand != SCC_EXPAND_VERTEX && expand != SCC_EXPAND_VWARP && expand != SCC_EXPAND_LB &&
+        expand != SCC_EXPAND_DO)
         return fail(SCC_ERR_INVALID, "unknown expand mode %d", expand);
     // the reference uses the virtual warp only for the first pivot's search
     // (scc_WCC.cu:209-317); the main loop uses vertex-per-thread
@@ -431,23 +474,23 @@ scc_status Engine::run(const scc_opts &o, scc_stats *out)
         if ((s = color_loop(loop_expand, W, &stats)) != SCC_OK) return s;
     }
     mark(SCC_PHASE_LABEL);
-    k_canon(stream, g, st);
+    k_canon(stream, g, st, o.sizes != 0);
     mark(-1);
     if ((s = read_ctr()) != SCC_OK) return s;
-    if (h_ctr[C_ANOM])
-        return fail(SCC_ERR_DEVICE, "internal invariant violated %u times", h_ctr[C_ANOM]);
+    if (h_ctr[CT(C_ANOM)])
+        return fail(SCC_ERR_DEVICE, "internal invariant violated %u times", h_ctr[CT(C_ANOM)]);
     collect_times(&stats);
     if (first_reach_marks[0] >= 0) {
         float ms = 0;
         (void)hipEventElapsedTime(&ms, events[first_reach_marks[0]], events[first_reach_marks[1]]);
         stats.ms_first_reach = ms;
     }
-    auto u64 = [&](int i) { return (uint64_t)h_ctr[i] | ((uint64_t)h_ctr[i + 1] << 32); };
-    stats.num_sccs = h_ctr[C_NSCC];
-    stats.num_trim1 = h_ctr[C_TRIM1];
-    stats.num_trim2 = h_ctr[C_TRIM2];
-    stats.num_pivot = h_ctr[C_PIVOT];
-    stats.largest = h_ctr[C_LARGEST];
+    auto u64 = [&](int i) { return (uint64_t)h_ctr[CT(i)] | ((uint64_t)h_ctr[CT(i) + 1] << 32); };
+    stats.num_sccs = h_ctr[CT(C_NSCC)];
+    stats.num_trim1 = h_ctr[CT(C_TRIM1)];
+    stats.num_trim2 = h_ctr[CT(C_TRIM2)];
+    stats.num_pivot = h_ctr[CT(C_PIVOT)];
+    stats.largest = h_ctr[CT(C_LARGEST)];
     stats.first_fw_vertices = u64(C_FWV);
     stats.first_fw_arcs = u64(C_FWE);
     stats.first_bw_vertices = u64(C_BWV);
@@ -527,13 +570,13 @@ scc_status Engine::phase_first_pivot(const uint8_t *dead, uint32_t *pivot)
 {
     scc_status s;
     if ((s = phase_begin(dead)) != SCC_OK) return s;
-    k_init_vis(stream, g, st);
+    init_vis();
     k_elect(stream, g, st);
     k_select(stream, g, st, 1);
     if ((s = read_ctr()) != SCC_OK) return s;
     *pivot = 0;
-    if (h_ctr[C_NPIV] > 1) return fail(SCC_ERR_DEVICE, "first-pivot election chose %u vertices", h_ctr[C_NPIV]);
-    if (h_ctr[C_NPIV] == 1) HIP_TRY(hipMemcpy(pivot, st.pivlist, sizeof(uint32_t), hipMemcpyDeviceToHost));
+    if (h_ctr[CT(C_NPIV)] > 1) return fail(SCC_ERR_DEVICE, "first-pivot election chose %u vertices", h_ctr[CT(C_NPIV)]);
+    if (h_ctr[CT(C_NPIV)] == 1) HIP_TRY(hipMemcpy(pivot, st.pivlist, sizeof(uint32_t), hipMemcpyDeviceToHost));
     return SCC_OK;
 }
 
@@ -544,11 +587,13 @@ scc_status Engine::phase_reach(uint32_t src, int dir, int expand, int W, const u
     if (src < 1 || src > g.V) return fail(SCC_ERR_INVALID, "source %u outside [1, %u]", src, g.V);
     if (dead && dead[src]) return fail(SCC_ERR_INVALID, "source %u is marked dead", src);
     if (expand == SCC_EXPAND_AUTO) expand = SCC_EXPAND_VERTEX;
+    if (expand < SCC_EXPAND_VERTEX || expand > SCC_EXPAND_DO)
+        return fail(SCC_ERR_INVALID, "unknown expand mode %d", expand);
     if (W != 1 && W != 2 && W != 4 && W != 8 && W != 16 && W != 32 && W != 64)
         return fail(SCC_ERR_INVALID, "warp size %d not in {1,2,4,8,16,32,64}", W);
     scc_status s;
     if ((s = phase_begin(dead)) != SCC_OK) return s;
-    k_init_vis(stream, g, st);
+    init_vis();
     k_seed_single(stream, st, src, dir);
     nmarks = 0;
     mark(SCC_PHASE_REACH);
@@ -574,7 +619,7 @@ scc_status Engine::phase_min_label(const uint8_t *dead, uint32_t *out)
         k_minlabel_prop(stream, g, st);
         if ((s = read_ctr()) != SCC_OK) return s;
         if (++it > limit()) return no_convergence("min-label propagation");
-    } while (h_ctr[C_CHANGED]);
+    } while (h_ctr[CT(C_CHANGED)]);
     if ((s = fetch32(out, st.cls)) != SCC_OK) return s;
     for (uint32_t v = 0; v <= g.V; v++)
         if (out[v] == DEAD || v == 0) out[v] = 0;
@@ -592,7 +637,7 @@ scc_status Engine::phase_max_color(const uint8_t *dead, uint32_t *out)
         k_color_prop(stream, g, st);
         if ((s = read_ctr()) != SCC_OK) return s;
         if (++it > limit()) return no_convergence("colour propagation");
-    } while (h_ctr[C_CHANGED]);
+    } while (h_ctr[CT(C_CHANGED)]);
     return fetch32(out, st.aux);
 }
 

@@ -4,12 +4,16 @@
 //
 // A level reads the input queue of each direction and writes the next one.
 // Claims are atomicOr on a visited bitmap (finished vertices pre-set), so a
-// vertex enters a queue at most once per search; pushes are aggregated per
-// wave with a ballot, a popcount prefix and one atomicAdd.  Queue counts
-// rotate through three device words so that no host step sits between
-// levels: level L reads cnt[L%3], appends to cnt[(L+1)%3] and zeroes
-// cnt[(L+2)%3] for level L+1.  With `single_class` every live vertex is in
-// one class and the class compare is skipped (the probe is one bitmap bit).
+// vertex enters a queue at most once per search.  Queue counts rotate
+// through three device counters so that no host step sits between levels:
+// level L reads cnt[L%3], appends to cnt[(L+1)%3] and zeroes cnt[(L+2)%3]
+// for level L+1.  With `single_class` every live vertex is in one class and
+// the class compare is skipped (the probe is one bitmap bit).
+//
+// Atomics on one address serialise (~10 ns each on MI355X), so the fast
+// kernels (LB, hub, bottom-up) stage pushes in LDS and reserve queue space
+// with one global atomic per ~1,800 pushes; the reference-mapping kernels
+// (vertex, virtual warp) aggregate per wave.
 #include "scc_dev.h"
 #include "scc_amd.h"
 
@@ -39,6 +43,52 @@ __device__ __forceinline__ void push_wave(bool claimed, uint32_t w, uint32_t *qo
     if (claimed) qout[base + (uint32_t)__popcll(m & ((1ull << wlane()) - 1))] = w;
 }
 
+// ------------------------------------------------- LDS-staged queue ---
+constexpr uint32_t QBUF = 2048;
+
+struct LdsQueue {
+    uint32_t *buf;   // QBUF entries
+    uint32_t *n;     // entries held
+    uint32_t *base;  // broadcast of the reserved global offset
+};
+
+// Wave-aggregated append to the workgroup's LDS buffer.  The caller
+// guarantees room for one push per lane since the last flush check.
+__device__ __forceinline__ void lq_push(const LdsQueue &q, bool p, uint32_t v)
+{
+    const uint64_t act = __ballot(1);
+    const uint64_t m = __ballot(p);
+    if (!m) return;
+    const int leader = __ffsll((unsigned long long)act) - 1;
+    uint32_t b = 0;
+    if ((int)wlane() == leader) b = atomicAdd(q.n, (uint32_t)__popcll(m));
+    b = __shfl(b, leader);
+    if (p) q.buf[b + (uint32_t)__popcll(m & ((1ull << wlane()) - 1))] = v;
+}
+
+// Flush when `force` or when another round of blockDim pushes might not
+// fit.  Must be reached by every thread of the workgroup.
+__device__ __forceinline__ void lq_flush(const LdsQueue &q, uint32_t *gcnt, uint32_t *gout, bool force)
+{
+    __syncthreads();
+    const uint32_t n = *q.n;
+    if (n == 0 || (!force && n + blockDim.x <= QBUF)) return;
+    if (threadIdx.x == 0) *q.base = atomicAdd(gcnt, n);
+    __syncthreads();
+    const uint32_t b = *q.base;
+    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) gout[b + i] = q.buf[i];
+    __syncthreads();
+    if (threadIdx.x == 0) *q.n = 0;
+    __syncthreads();
+}
+
+#define DECLARE_LDS_QUEUE(q)                                                                 \
+    __shared__ uint32_t q##_buf[QBUF];                                                       \
+    __shared__ uint32_t q##_n, q##_base;                                                     \
+    if (threadIdx.x == 0) q##_n = 0;                                                         \
+    __syncthreads();                                                                         \
+    const LdsQueue q{q##_buf, &q##_n, &q##_base}
+
 // W lanes per frontier vertex; W = 1 is the vertex-per-thread mapping of
 // fwd/bwd, W > 1 the virtual warp of fwd_warp<W> (lanes stride the list).
 template <int W, bool SINGLE>
@@ -48,9 +98,9 @@ __global__ __launch_bounds__(256) void expand_vw_kernel(LevelArgs a, uint32_t bp
     const uint32_t bid = blockIdx.x - (uint32_t)dir * bpd;
     const DirArgs &D = a.d[dir];
     const int L = a.level;
-    const uint32_t n = D.cnt[L % 3];
-    uint32_t *cnt_out = &D.cnt[(L + 1) % 3];
-    if (bid == 0 && threadIdx.x == 0) D.cnt[(L + 2) % 3] = 0;
+    const uint32_t n = D.cnt[CT(L % 3)];
+    uint32_t *cnt_out = &D.cnt[CT((L + 1) % 3)];
+    if (bid == 0 && threadIdx.x == 0) D.cnt[CT((L + 2) % 3)] = 0;
     if (n == 0) return;
     const uint32_t g = (bid * blockDim.x + threadIdx.x) / W;
     const uint32_t ng = bpd * blockDim.x / W;
@@ -69,6 +119,252 @@ __global__ __launch_bounds__(256) void expand_vw_kernel(LevelArgs a, uint32_t bp
     }
 }
 
+// ---------------------------------------------------------------------
+// Load-balanced expansion (SCC_EXPAND_LB).  A 256-lane workgroup takes 256
+// frontier vertices, scans their degrees in LDS and walks the concatenated
+// adjacency slots with all lanes, so consecutive lanes read consecutive
+// column entries whatever the degree mix.  Vertices of degree >= LB_BIG are
+// deferred to a hub list and split into LB_CHUNK-arc chunks spread over
+// every workgroup by expand_hub_kernel, launched right after.
+constexpr uint32_t LB_TILE = 256;
+constexpr uint32_t LB_BIG = 512;
+constexpr uint32_t LB_CHUNK = 2048;
+
+// Exclusive scan of x over the 256-lane workgroup; returns the prefix and
+// writes the total to *total.  `wsum` is LDS scratch of 4 words.
+__device__ __forceinline__ uint32_t block_exscan256(uint32_t x, uint32_t *wsum, uint32_t *total)
+{
+    const uint32_t l = wlane(), w = threadIdx.x >> 6;
+    uint32_t incl = x;
+    for (int o = 1; o < 64; o <<= 1) {
+        uint32_t y = __shfl_up(incl, o);
+        if (l >= (uint32_t)o) incl += y;
+    }
+    if (l == 63) wsum[w] = incl;
+    __syncthreads();
+    uint32_t base = 0, tot = 0;
+    for (uint32_t i = 0; i < 4; i++) {
+        uint32_t s = wsum[i];
+        base += i < w ? s : 0u;
+        tot += s;
+    }
+    *total = tot;
+    return base + incl - x;
+}
+
+template <bool SINGLE>
+__global__ __launch_bounds__(256) void expand_lb_kernel(LevelArgs a, uint32_t bpd)
+{
+    __shared__ uint32_t s_off[LB_TILE + 1];
+    __shared__ uint32_t s_beg[LB_TILE];
+    __shared__ uint32_t s_cls[LB_TILE];
+    __shared__ uint32_t s_wsum[4];
+    const int dir = (a.ndirs == 2 && blockIdx.x >= bpd) ? 1 : 0;
+    const uint32_t bid = blockIdx.x - (uint32_t)dir * bpd;
+    const DirArgs &D = a.d[dir];
+    const int L = a.level;
+    const uint32_t n = D.cnt[CT(L % 3)];
+    uint32_t *cnt_out = &D.cnt[CT((L + 1) % 3)];
+    if (bid == 0 && threadIdx.x == 0) {
+        D.cnt[CT((L + 2) % 3)] = 0;
+        *D.bigc_next = 0ull;         // the previous level's hub list is consumed
+    }
+    if (n == 0) return;
+    DECLARE_LDS_QUEUE(q);
+    const uint32_t *__restrict__ R = D.R;
+    const uint32_t *__restrict__ C = D.C;
+    for (uint32_t base = bid * LB_TILE; base < n; base += bpd * LB_TILE) {
+        const uint32_t i = base + threadIdx.x;
+        uint32_t v = 0, beg = 0, deg = 0, c = 0;
+        if (i < n) {
+            v = D.qin[i];
+            beg = R[v];
+            deg = R[v + 1] - beg;
+            if (!SINGLE) c = a.cls[v];
+            if (deg >= LB_BIG) {
+                const unsigned long long chunks = (deg + LB_CHUNK - 1) / LB_CHUNK;
+                const unsigned long long old = atomicAdd(D.bigc, (1ull << 32) | chunks);
+                const uint32_t item = (uint32_t)(old >> 32);
+                D.big[item] = v;
+                D.bigoff[item] = (uint32_t)old;
+                deg = 0;
+            }
+        }
+        uint32_t total;
+        const uint32_t off = block_exscan256(deg, s_wsum, &total);
+        s_off[threadIdx.x] = off;
+        s_beg[threadIdx.x] = beg;
+        s_cls[threadIdx.x] = c;
+        if (threadIdx.x == 0) s_off[LB_TILE] = total;
+        __syncthreads();
+        for (uint32_t j0 = 0; j0 < total; j0 += LB_TILE) {
+            const uint32_t j = j0 + threadIdx.x;
+            bool claimed = false;
+            uint32_t w = 0;
+            if (j < total) {
+                // owner = last k with s_off[k] <= j (degree-0 entries share offsets)
+                uint32_t lo = 0, hi = LB_TILE - 1;
+                while (lo < hi) {
+                    const uint32_t mid = (lo + hi + 1) >> 1;
+                    if (s_off[mid] <= j) lo = mid;
+                    else hi = mid - 1;
+                }
+                w = C[s_beg[lo] + (j - s_off[lo])];
+                claimed = try_claim<SINGLE>(D.vis, a.cls, w, SINGLE ? 0u : s_cls[lo]);
+            }
+            lq_push(q, claimed, w);
+            lq_flush(q, cnt_out, D.qout, false);
+        }
+        __syncthreads();   // LDS reuse by the next tile
+    }
+    lq_flush(q, cnt_out, D.qout, true);
+}
+
+// Chunks of hub adjacency lists: chunk g belongs to the hub item whose chunk
+// offset is the last one <= g (offsets grow with the item index because
+// both come from one 64-bit atomicAdd).
+template <bool SINGLE>
+__global__ __launch_bounds__(256) void expand_hub_kernel(LevelArgs a, uint32_t bpd)
+{
+    const int dir = (a.ndirs == 2 && blockIdx.x >= bpd) ? 1 : 0;
+    const uint32_t bid = blockIdx.x - (uint32_t)dir * bpd;
+    const DirArgs &D = a.d[dir];
+    const unsigned long long packed = *D.bigc;
+    const uint32_t items = (uint32_t)(packed >> 32), chunks = (uint32_t)packed;
+    if (!items || bid >= chunks) return;
+    DECLARE_LDS_QUEUE(q);
+    uint32_t *cnt_out = &D.cnt[CT((a.level + 1) % 3)];
+    const uint32_t *__restrict__ R = D.R;
+    const uint32_t *__restrict__ C = D.C;
+    for (uint32_t gch = bid; gch < chunks; gch += bpd) {
+        uint32_t lo = 0, hi = items - 1;
+        while (lo < hi) {
+            const uint32_t mid = (lo + hi + 1) >> 1;
+            if (D.bigoff[mid] <= gch) lo = mid;
+            else hi = mid - 1;
+        }
+        const uint32_t v = D.big[lo];
+        const uint32_t c = SINGLE ? 0u : a.cls[v];
+        const uint32_t b = R[v] + (gch - D.bigoff[lo]) * LB_CHUNK;
+        const uint32_t e = min(R[v + 1], b + LB_CHUNK);
+        for (uint32_t j0 = b; j0 < e; j0 += 256) {
+            const uint32_t j = j0 + threadIdx.x;
+            bool claimed = false;
+            uint32_t w = 0;
+            if (j < e) {
+                w = C[j];
+                claimed = try_claim<SINGLE>(D.vis, a.cls, w, c);
+            }
+            lq_push(q, claimed, w);
+            lq_flush(q, cnt_out, D.qout, false);
+        }
+    }
+    lq_flush(q, cnt_out, D.qout, true);
+}
+
+static void launch_lb(hipStream_t s, const LevelArgs &a, bool single, uint32_t bpd)
+{
+    const uint32_t grid = bpd * (uint32_t)a.ndirs;
+    if (single) {
+        expand_lb_kernel<true><<<grid, 256, 0, s>>>(a, bpd);
+        expand_hub_kernel<true><<<grid, 256, 0, s>>>(a, bpd);
+    } else {
+        expand_lb_kernel<false><<<grid, 256, 0, s>>>(a, bpd);
+        expand_hub_kernel<false><<<grid, 256, 0, s>>>(a, bpd);
+    }
+}
+
+// ---------------------------------------------------------------------
+// Bottom-up pass (direction-optimising reachability, Beamer et al. SC'12):
+// a live vertex u not yet visited scans its parents and joins the search as
+// soon as one of them is visited and in u's class.  Only u's own lane sets
+// u's bit, so there is one atomic per newly visited vertex instead of one
+// per probed arc.  A parent that became visited during this same pass may
+// be missed (stale cache line); the next pass sees it because kernel
+// boundaries make every write visible.  The pass pushes the vertices it
+// visits, so a following top-down level can continue from them.
+//
+// Two tiers: each lane probes the first BU_SHORT parents of its vertex with
+// independent loads; vertices with longer parent lists and no hit yet are
+// listed in LDS and scanned by whole waves, 64 parents per step, with a
+// ballot to stop at the first hit.
+constexpr uint32_t BU_SHORT = 4;
+
+template <bool SINGLE>
+__device__ __forceinline__ bool visited_parent(const uint32_t *vis, const uint32_t *dbits,
+                                               const uint32_t *cls, uint32_t p, uint32_t c)
+{
+    const uint32_t wi = p >> 5, pb = 1u << (p & 31);
+    return (vis[wi] & ~dbits[wi] & pb) && (SINGLE || cls[p] == c);
+}
+
+template <bool SINGLE>
+__global__ __launch_bounds__(256) void expand_bu_kernel(DirArgs D, const uint32_t *__restrict__ PR,
+                                                        const uint32_t *__restrict__ PC,
+                                                        const uint32_t *__restrict__ dbits,
+                                                        const uint32_t *cls, uint32_t V, int L)
+{
+    __shared__ uint32_t s_long[256];
+    __shared__ uint32_t s_nlong;
+    uint32_t *cnt_out = &D.cnt[CT((L + 1) % 3)];
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        D.cnt[CT((L + 2) % 3)] = 0;
+        *D.bigc_next = 0ull;
+    }
+    if (threadIdx.x == 0) s_nlong = 0;
+    DECLARE_LDS_QUEUE(q);
+    for (uint32_t base = blockIdx.x * 256; base < V; base += gridDim.x * 256) {
+        const uint32_t u = base + threadIdx.x + 1;
+        bool found = false;
+        if (u <= V && !(D.vis[u >> 5] & (1u << (u & 31)))) {
+            const uint32_t c = SINGLE ? 0u : cls[u];
+            const uint32_t b = PR[u], e = PR[u + 1];
+            uint32_t p[BU_SHORT];
+#pragma unroll
+            for (uint32_t t = 0; t < BU_SHORT; t++) p[t] = b + t < e ? PC[b + t] : 0u;
+#pragma unroll
+            for (uint32_t t = 0; t < BU_SHORT; t++)
+                found |= b + t < e && visited_parent<SINGLE>(D.vis, dbits, cls, p[t], c);
+            if (!found && e - b > BU_SHORT) s_long[atomicAdd(&s_nlong, 1u)] = u;
+            if (found) found = !(atomicOr(&D.vis[u >> 5], 1u << (u & 31)) & (1u << (u & 31)));
+        }
+        lq_push(q, found, u);
+        __syncthreads();
+        // long parent lists: one wave per vertex
+        const uint32_t nlong = s_nlong;
+        for (uint32_t k = threadIdx.x >> 6; k < nlong; k += blockDim.x >> 6) {
+            const uint32_t x = s_long[k];
+            const uint32_t c = SINGLE ? 0u : cls[x];
+            const uint32_t e = PR[x + 1];
+            bool hit = false;
+            for (uint32_t j0 = PR[x] + BU_SHORT; j0 < e && !hit; j0 += 64) {
+                const uint32_t j = j0 + wlane();
+                const bool h = j < e && visited_parent<SINGLE>(D.vis, dbits, cls, PC[j], c);
+                hit = __ballot(h) != 0;
+            }
+            bool claimed = false;
+            if (hit && wlane() == 0)
+                claimed = !(atomicOr(&D.vis[x >> 5], 1u << (x & 31)) & (1u << (x & 31)));
+            lq_push(q, claimed, x);
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) s_nlong = 0;
+        lq_flush(q, cnt_out, D.qout, false);
+    }
+    lq_flush(q, cnt_out, D.qout, true);
+}
+
+void k_bottom_up(hipStream_t s, const DirArgs &d, const uint32_t *PR, const uint32_t *PC,
+                 const uint32_t *dbits, const uint32_t *cls, uint32_t V, int level, bool single_class)
+{
+    uint64_t b = ((uint64_t)V + 256) / 256;
+    const uint32_t grid = (uint32_t)(b > 2048 ? 2048 : b);
+    if (single_class)
+        expand_bu_kernel<true><<<grid, 256, 0, s>>>(d, PR, PC, dbits, cls, V, level);
+    else
+        expand_bu_kernel<false><<<grid, 256, 0, s>>>(d, PR, PC, dbits, cls, V, level);
+}
+
 template <int W>
 static void launch_vw(hipStream_t s, const LevelArgs &a, bool single, uint32_t bpd)
 {
@@ -82,6 +378,10 @@ static void launch_vw(hipStream_t s, const LevelArgs &a, bool single, uint32_t b
 void k_expand_level(hipStream_t s, const LevelArgs &a, int expand, int warp, bool single_class,
                     uint32_t blocks_per_dir)
 {
+    if (expand == SCC_EXPAND_LB) {
+        launch_lb(s, a, single_class, blocks_per_dir);
+        return;
+    }
     int W = expand == SCC_EXPAND_VWARP ? warp : 1;
     switch (W) {
     case 1: launch_vw<1>(s, a, single_class, blocks_per_dir); break;

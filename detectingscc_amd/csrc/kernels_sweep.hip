@@ -31,17 +31,31 @@ __device__ __forceinline__ bool bit_of(const uint32_t *bm, uint32_t v)
     return (bm[v >> 5] >> (v & 31)) & 1u;
 }
 
-// Sum over the (fully active) wave, added once by lane 0.
+// Sum over the whole workgroup, added to *p by one atomic.  Every thread of
+// the block must call it (they all do: it ends the kernels that use it).
+__device__ __forceinline__ uint64_t block_sum(uint64_t x)
+{
+    __shared__ uint64_t s_part[16];
+    for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o);
+    if (lane() == 0) s_part[threadIdx.x >> 6] = x;
+    __syncthreads();
+    uint64_t t = 0;
+    if (threadIdx.x == 0)
+        for (uint32_t w = 0; w < (blockDim.x >> 6); w++) t += s_part[w];
+    __syncthreads();
+    return t;   // valid in thread 0
+}
+
 __device__ __forceinline__ void wave_add32(uint32_t *p, uint32_t x)
 {
-    for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o);
-    if (lane() == 0 && x) atomicAdd(p, x);
+    const uint64_t t = block_sum(x);
+    if (threadIdx.x == 0 && t) atomicAdd(p, (uint32_t)t);
 }
 
 __device__ __forceinline__ void wave_add64(uint32_t *p32, uint64_t x)
 {
-    for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o);
-    if (lane() == 0 && x) atomicAdd(reinterpret_cast<unsigned long long *>(p32), (unsigned long long)x);
+    const uint64_t t = block_sum(x);
+    if (threadIdx.x == 0 && t) atomicAdd(reinterpret_cast<unsigned long long *>(p32), (unsigned long long)t);
 }
 
 uint32_t sweep_blocks(uint32_t V)
@@ -115,7 +129,7 @@ __global__ void trim1_kernel(const uint32_t *__restrict__ Fr, const uint32_t *__
             removed++;
         }
     }
-    wave_add32(&ctr[C_TRIM1], removed);
+    wave_add32(&ctr[CT(C_TRIM1)], removed);
 }
 
 void k_trim1(hipStream_t s, const DevGraph &g, DevState &st)
@@ -180,7 +194,7 @@ __global__ void trim2_kernel(const uint32_t *__restrict__ Fr, const uint32_t *__
             }
         }
     }
-    wave_add32(&ctr[C_TRIM2], pairs);
+    wave_add32(&ctr[CT(C_TRIM2)], pairs);
 }
 
 void k_trim2(hipStream_t s, const DevGraph &g, DevState &st)
@@ -190,24 +204,25 @@ void k_trim2(hipStream_t s, const DevGraph &g, DevState &st)
 }
 
 // ------------------------------------------------- visited bitmaps ---
-// Bits of finished vertices (and of slot 0 and the padding) start set.
+// Bits of finished vertices (and of slot 0 and the padding) start set in
+// both visited bitmaps; dbits keeps the finished set alone so a bottom-up
+// pass can tell "visited" from "finished".  Counts the live vertices.
 __global__ void init_vis_kernel(const uint32_t *__restrict__ cls, uint32_t *fvis, uint32_t *bvis,
-                                uint32_t V, uint32_t nchunks)
+                                uint32_t *dbits, uint32_t *ctr, uint32_t V, uint32_t nchunks)
 {
     const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const uint32_t nwaves = (gridDim.x * blockDim.x) >> 6;
+    uint32_t live = 0;
     for (uint32_t ch = wave; ch < nchunks; ch += nwaves) {
         uint32_t v = ch * 64 + lane();
         bool dead = v > V || cls[v] == DEAD;
+        live += !dead;
         uint64_t m = __ballot(dead);
-        if (lane() == 0) {
-            fvis[2 * ch] = (uint32_t)m;
-            fvis[2 * ch + 1] = (uint32_t)(m >> 32);
-        } else if (lane() == 1) {
-            bvis[2 * ch] = (uint32_t)m;
-            bvis[2 * ch + 1] = (uint32_t)(m >> 32);
-        }
+        uint32_t w = (lane() & 1) ? (uint32_t)(m >> 32) : (uint32_t)m;
+        uint32_t *dst = lane() < 2 ? fvis : (lane() < 4 ? bvis : dbits);
+        if (lane() < 6) dst[2 * ch + (lane() & 1)] = w;
     }
+    wave_add32(&ctr[CT(C_NLIVE)], live);
 }
 
 void k_init_vis(hipStream_t s, const DevGraph &g, DevState &st)
@@ -215,7 +230,8 @@ void k_init_vis(hipStream_t s, const DevGraph &g, DevState &st)
     uint32_t nchunks = st.nwords / 2;
     uint32_t blocks = (nchunks + 3) / 4;
     blocks = blocks < 1 ? 1 : (blocks > 4096 ? 4096 : blocks);
-    init_vis_kernel<<<blocks, SWEEP_BLOCK, 0, s>>>(st.cls, st.fvis, st.bvis, g.V, nchunks);
+    init_vis_kernel<<<blocks, SWEEP_BLOCK, 0, s>>>(st.cls, st.fvis, st.bvis, st.dbits, st.ctr, g.V,
+                                                  nchunks);
 }
 
 // ------------------------------------------------------------ pivots ---
@@ -266,7 +282,7 @@ __global__ void select_kernel(const uint32_t *__restrict__ Fr, const uint32_t *_
         if (!valid) continue;
         const uint32_t c = cls[v];
         if (c == DEAD || piv[c] != pivot_key(Fr, Br, v)) continue;
-        uint32_t i = atomicAdd(&ctr[C_NPIV], 1u);
+        uint32_t i = atomicAdd(&ctr[CT(C_NPIV)], 1u);
         pivlist[i] = v;
         qf[i] = v;
         atomicOr(&fvis[v >> 5], 1u << (v & 31));
@@ -277,19 +293,26 @@ __global__ void select_kernel(const uint32_t *__restrict__ Fr, const uint32_t *_
     }
 }
 
+__device__ __forceinline__ void zero_hub_counts(uint32_t *ctr)
+{
+    for (int i = 0; i < 2; i++) {
+        *reinterpret_cast<unsigned long long *>(ctr + CT(C_BIGF + i)) = 0ull;
+        *reinterpret_cast<unsigned long long *>(ctr + CT(C_BIGB + i)) = 0ull;
+    }
+}
+
 // Queue counts for level 0 and the two counters after it.
 __global__ void seed_counts_kernel(uint32_t *ctr, const uint32_t *pivlist, int seed_bw)
 {
-    uint32_t n = ctr[C_NPIV];
-    ctr[C_PIV0] = n ? pivlist[0] : 0u;
-    ctr[C_FWCNT + 0] = n;
-    ctr[C_FWCNT + 1] = 0;
-    ctr[C_FWCNT + 2] = 0;
-    ctr[C_BWCNT + 0] = seed_bw ? n : 0;
-    ctr[C_BWCNT + 1] = 0;
-    ctr[C_BWCNT + 2] = 0;
-    unsigned long long *b = reinterpret_cast<unsigned long long *>(ctr + C_BIGF);
-    b[0] = b[1] = b[2] = b[3] = 0;
+    uint32_t n = ctr[CT(C_NPIV)];
+    ctr[CT(C_PIV0)] = n ? pivlist[0] : 0u;
+    ctr[CT(C_FWCNT + 0)] = n;
+    ctr[CT(C_FWCNT + 1)] = 0;
+    ctr[CT(C_FWCNT + 2)] = 0;
+    ctr[CT(C_BWCNT + 0)] = seed_bw ? n : 0;
+    ctr[CT(C_BWCNT + 1)] = 0;
+    ctr[CT(C_BWCNT + 2)] = 0;
+    zero_hub_counts(ctr);
 }
 
 void k_select(hipStream_t s, const DevGraph &g, DevState &st, int seed_bw)
@@ -305,10 +328,9 @@ __global__ void seed_single_kernel(uint32_t *vis, uint32_t *q, uint32_t *ctr, ui
 {
     q[0] = src;
     vis[src >> 5] |= 1u << (src & 31);
-    for (int i = 0; i < 3; i++) ctr[C_FWCNT + i] = ctr[C_BWCNT + i] = 0;
-    ctr[(dir ? C_BWCNT : C_FWCNT) + 0] = 1;
-    unsigned long long *b = reinterpret_cast<unsigned long long *>(ctr + C_BIGF);
-    b[0] = b[1] = b[2] = b[3] = 0;
+    for (int i = 0; i < 3; i++) ctr[CT(C_FWCNT + i)] = ctr[CT(C_BWCNT + i)] = 0;
+    ctr[CT((dir ? C_BWCNT : C_FWCNT) + 0)] = 1;
+    zero_hub_counts(ctr);
 }
 
 void k_seed_single(hipStream_t s, DevState &st, uint32_t src, int dir)
@@ -333,7 +355,7 @@ __global__ void update_kernel(uint32_t *cls, uint32_t *lab, const uint64_t *__re
         if (c == DEAD) continue;
         uint32_t p = key_vertex(piv[c]);
         if (p < 1 || p > V) {          // no election for this class: cannot happen
-            atomicAdd(&ctr[C_ANOM], 1u);
+            atomicAdd(&ctr[CT(C_ANOM)], 1u);
             p = v;
         }
         const uint32_t f = bit_of(fvis, v), b = bit_of(bvis, v);
@@ -346,8 +368,8 @@ __global__ void update_kernel(uint32_t *cls, uint32_t *lab, const uint64_t *__re
             live++;
         }
     }
-    wave_add32(&ctr[C_LIVE], live);
-    wave_add32(&ctr[C_PIVOT], found);
+    wave_add32(&ctr[CT(C_LIVE)], live);
+    wave_add32(&ctr[CT(C_PIVOT)], found);
 }
 
 void k_update(hipStream_t s, const DevGraph &g, DevState &st)
@@ -360,7 +382,7 @@ void k_update(hipStream_t s, const DevGraph &g, DevState &st)
 __global__ void clear_piv_kernel(const uint32_t *__restrict__ pivlist, uint64_t *piv,
                                  const uint32_t *ctr)
 {
-    const uint32_t n = ctr[C_NPIV];
+    const uint32_t n = ctr[CT(C_NPIV)];
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         uint32_t p = pivlist[i];
         piv[3ull * p] = 0;
@@ -389,10 +411,10 @@ __global__ void reach_stats_kernel(const uint32_t *__restrict__ Fr, const uint32
         if (bit_of(fvis, v)) { fv++; fe += Fr[v + 1] - Fr[v]; }
         if (bit_of(bvis, v)) { bv++; be += Br[v + 1] - Br[v]; }
     }
-    wave_add64(&ctr[C_FWV], fv);
-    wave_add64(&ctr[C_FWE], fe);
-    wave_add64(&ctr[C_BWV], bv);
-    wave_add64(&ctr[C_BWE], be);
+    wave_add64(&ctr[CT(C_FWV)], fv);
+    wave_add64(&ctr[CT(C_FWE)], fe);
+    wave_add64(&ctr[CT(C_BWV)], bv);
+    wave_add64(&ctr[CT(C_BWE)], be);
 }
 
 void k_reach_stats(hipStream_t s, const DevGraph &g, DevState &st)
@@ -441,7 +463,7 @@ __global__ void minlabel_prop_kernel(const uint32_t *__restrict__ Fr,
             changed = 1;
         }
     }
-    wave_add32(&ctr[C_CHANGED], changed);
+    wave_add32(&ctr[CT(C_CHANGED)], changed);
 }
 
 void k_minlabel_prop(hipStream_t s, const DevGraph &g, DevState &st)
@@ -503,7 +525,7 @@ __global__ void color_prop_kernel(const uint32_t *__restrict__ Fr, const uint32_
             changed = 1;
         }
     }
-    wave_add32(&ctr[C_CHANGED], changed);
+    wave_add32(&ctr[CT(C_CHANGED)], changed);
 }
 
 void k_color_prop(hipStream_t s, const DevGraph &g, DevState &st)
@@ -522,7 +544,7 @@ __global__ void color_roots_kernel(uint32_t *cls, const uint32_t *__restrict__ a
         const uint32_t c = aux[v];
         cls[v] = c;
         if (c == v) {
-            uint32_t i = atomicAdd(&ctr[C_NPIV], 1u);
+            uint32_t i = atomicAdd(&ctr[CT(C_NPIV)], 1u);
             pivlist[i] = v;
             qf[i] = v;
             atomicOr(&fvis[v >> 5], 1u << (v & 31));
@@ -556,8 +578,8 @@ __global__ void color_finish_kernel(uint32_t *cls, uint32_t *lab, const uint32_t
             live++;
         }
     }
-    wave_add32(&ctr[C_LIVE], live);
-    wave_add32(&ctr[C_PIVOT], found);
+    wave_add32(&ctr[CT(C_LIVE)], live);
+    wave_add32(&ctr[CT(C_PIVOT)], found);
 }
 
 void k_color_finish(hipStream_t s, const DevGraph &g, DevState &st)
@@ -578,29 +600,43 @@ __global__ void canon_reset_kernel(uint32_t *aux, uint32_t *cnt, uint32_t V)
     }
 }
 
+// Lanes of a wave that share a label are combined before touching memory:
+// up to MATCH_ROUNDS times the lowest pending lane leads and every lane with
+// its label joins; what is left (a wave with many distinct labels) goes
+// lane by lane.  A giant SCC's members then cost one atomic per wave
+// instead of one per vertex.
+constexpr int MATCH_ROUNDS = 8;
+
 __global__ void canon_min_kernel(uint32_t *lab, uint32_t *aux, uint32_t *ctr, uint32_t V)
 {
     WAVE_LOOP(v, valid, V)
     {
         uint32_t r = valid ? lab[v] : DEAD;
         if (valid && (r < 1 || r > V)) {   // every vertex must hold a representative
-            atomicAdd(&ctr[C_ANOM], 1u);
+            atomicAdd(&ctr[CT(C_ANOM)], 1u);
             lab[v] = r = v;
         }
-        const uint32_t r0 = __shfl(r, 0);
-        const bool mine = valid && r == r0;
-        uint32_t m = mine ? v : DEAD;
-        for (int o = 32; o > 0; o >>= 1) {
-            uint32_t x = __shfl_xor(m, o);
-            m = x < m ? x : m;
+        bool pending = valid;
+        for (int it = 0; it < MATCH_ROUNDS; it++) {
+            const uint64_t pm = __ballot(pending);
+            if (!pm) break;
+            const int L = __ffsll((unsigned long long)pm) - 1;
+            const uint32_t rl = __shfl(r, L);
+            const bool mine = pending && r == rl;
+            uint32_t m = mine ? v : DEAD;
+            for (int o = 32; o > 0; o >>= 1) {
+                uint32_t x = __shfl_xor(m, o);
+                m = x < m ? x : m;
+            }
+            if ((int)lane() == L && m < aux[rl]) atomicMin(&aux[rl], m);
+            pending = pending && !mine;
         }
-        if (lane() == 0 && m != DEAD && m < aux[r0]) atomicMin(&aux[r0], m);
-        if (valid && !mine && v < aux[r]) atomicMin(&aux[r], v);
+        if (pending && v < aux[r]) atomicMin(&aux[r], v);
     }
 }
 
 __global__ void canon_apply_kernel(uint32_t *lab, const uint32_t *__restrict__ aux, uint32_t *cnt,
-                                   uint32_t *ctr, uint32_t V)
+                                   uint32_t *ctr, uint32_t V, int sizes)
 {
     uint32_t roots = 0;
     WAVE_LOOP(v, valid, V)
@@ -610,13 +646,20 @@ __global__ void canon_apply_kernel(uint32_t *lab, const uint32_t *__restrict__ a
             lab[v] = r;
             roots += r == v;
         }
-        const uint32_t r0 = __shfl(r, 0);
-        const bool mine = valid && r == r0;
-        const uint32_t n = (uint32_t)__popcll(__ballot(mine));
-        if (lane() == 0 && n) atomicAdd(&cnt[r0], n);
-        if (valid && !mine) atomicAdd(&cnt[r], 1u);
+        if (!sizes) continue;
+        bool pending = valid;
+        for (int it = 0; it < MATCH_ROUNDS; it++) {
+            const uint64_t pm = __ballot(pending);
+            if (!pm) break;
+            const int L = __ffsll((unsigned long long)pm) - 1;
+            const uint32_t rl = __shfl(r, L);
+            const uint64_t mm = __ballot(pending && r == rl);
+            if ((int)lane() == L) atomicAdd(&cnt[rl], (uint32_t)__popcll(mm));
+            pending = pending && r != rl;
+        }
+        if (pending) atomicAdd(&cnt[r], 1u);
     }
-    wave_add32(&ctr[C_NSCC], roots);
+    wave_add32(&ctr[CT(C_NSCC)], roots);
 }
 
 __global__ void canon_largest_kernel(const uint32_t *__restrict__ cnt, uint32_t *ctr, uint32_t V)
@@ -628,17 +671,17 @@ __global__ void canon_largest_kernel(const uint32_t *__restrict__ cnt, uint32_t 
         uint32_t x = __shfl_xor(m, o);
         m = x > m ? x : m;
     }
-    if (lane() == 0 && m) atomicMax(&ctr[C_LARGEST], m);
+    if (lane() == 0 && m) atomicMax(&ctr[CT(C_LARGEST)], m);
 }
 
-void k_canon(hipStream_t s, const DevGraph &g, DevState &st)
+void k_canon(hipStream_t s, const DevGraph &g, DevState &st, bool sizes)
 {
     const uint32_t b = sweep_blocks(g.V);
     // cls is all DEAD at this point; it holds the SCC sizes from here on
     canon_reset_kernel<<<b, SWEEP_BLOCK, 0, s>>>(st.aux, st.cls, g.V);
     canon_min_kernel<<<b, SWEEP_BLOCK, 0, s>>>(st.lab, st.aux, st.ctr, g.V);
-    canon_apply_kernel<<<b, SWEEP_BLOCK, 0, s>>>(st.lab, st.aux, st.cls, st.ctr, g.V);
-    canon_largest_kernel<<<b, SWEEP_BLOCK, 0, s>>>(st.cls, st.ctr, g.V);
+    canon_apply_kernel<<<b, SWEEP_BLOCK, 0, s>>>(st.lab, st.aux, st.cls, st.ctr, g.V, sizes ? 1 : 0);
+    if (sizes) canon_largest_kernel<<<b, SWEEP_BLOCK, 0, s>>>(st.cls, st.ctr, g.V);
 }
 
 // ------------------------------------------------ phase exports ---

@@ -27,7 +27,13 @@ namespace sccamd {
 constexpr uint32_t DEAD = 0xFFFFFFFFu;
 constexpr int SWEEP_BLOCK = 256;
 
-// counters block (uint32 words)
+// Counters block: counter i lives at word CT(i), one 128-byte line per
+// counter, so atomics on different counters never share a line (same-line
+// atomics from every workgroup serialise on one L2 channel).  64-bit
+// counters use words CT(i) and CT(i)+1.
+constexpr int CS = 32;
+__host__ __device__ constexpr int CT(int i) { return i * CS; }
+
 enum : int {
     C_FWCNT = 0,     // 3 rotating FW queue counts
     C_BWCNT = 3,     // 3 rotating BW queue counts
@@ -43,11 +49,12 @@ enum : int {
     C_FWE = 16,
     C_BWV = 18,
     C_BWE = 20,
-    C_BIGF = 22,     // 2 x 64-bit packed (items, chunks) counts of the FW hub list
-    C_BIGB = 26,     // same for BW
+    C_BIGF = 22,     // 2 x 64-bit packed (items, chunks) counts of the FW hub list (22, 23)
+    C_BIGB = 24,     // same for BW (24, 25)
     C_PIV0 = 30,     // first pivot of the latest election
     C_ANOM = 31,     // invariant violations caught by kernel guards (must stay 0)
-    C_COUNT = 32
+    C_NLIVE = 32,    // live vertices when the visited bitmaps were initialised
+    C_COUNT = 48
 };
 
 struct DevGraph {
@@ -58,6 +65,7 @@ struct DevGraph {
 struct DevState {
     uint32_t *cls = nullptr, *lab = nullptr, *aux = nullptr;
     uint32_t *fvis = nullptr, *bvis = nullptr;
+    uint32_t *dbits = nullptr;     // finished vertices (same layout as fvis/bvis)
     uint64_t *piv = nullptr;
     uint32_t *qf[2] = {nullptr, nullptr}, *qb[2] = {nullptr, nullptr};
     uint32_t *pivlist = nullptr;   // pivots of the current round
@@ -106,7 +114,7 @@ void k_color_init(hipStream_t s, const DevGraph &g, DevState &st);
 void k_color_prop(hipStream_t s, const DevGraph &g, DevState &st);
 void k_color_roots(hipStream_t s, const DevGraph &g, DevState &st);
 void k_color_finish(hipStream_t s, const DevGraph &g, DevState &st);
-void k_canon(hipStream_t s, const DevGraph &g, DevState &st);
+void k_canon(hipStream_t s, const DevGraph &g, DevState &st, bool sizes);
 void k_seed_single(hipStream_t s, DevState &st, uint32_t src, int dir);
 void k_export_visited(hipStream_t s, const DevGraph &g, DevState &st, int dir, uint8_t *out_dev);
 void k_export_removed(hipStream_t s, const DevGraph &g, DevState &st, uint8_t *out_dev);
@@ -115,6 +123,10 @@ void k_export_removed(hipStream_t s, const DevGraph &g, DevState &st, uint8_t *o
 // expand: SCC_EXPAND_VERTEX / VWARP / LB; warp = lanes per vertex for VWARP
 void k_expand_level(hipStream_t s, const LevelArgs &a, int expand, int warp, bool single_class,
                     uint32_t blocks_per_dir);
+// One bottom-up pass of direction d.dir: every live unvisited vertex scans
+// its parents (PR/PC = the opposite CSR) for a visited one of its class.
+void k_bottom_up(hipStream_t s, const DirArgs &d, const uint32_t *PR, const uint32_t *PC,
+                 const uint32_t *dbits, const uint32_t *cls, uint32_t V, int level, bool single_class);
 
 uint32_t sweep_blocks(uint32_t V);
 

@@ -116,6 +116,7 @@ typedef struct scc_ctx scc_ctx;
 #define SCC_EXPAND_VERTEX 1 /* one lane per frontier vertex (fwd/bwd, scc_kernels.cu:360-445) */
 #define SCC_EXPAND_VWARP  2 /* W lanes per vertex (fwd_warp/bwd_warp, scc_kernels.h:46-105) */
 #define SCC_EXPAND_LB     3 /* workgroup scan-based gather + hub splitting */
+#define SCC_EXPAND_DO     4 /* direction-optimising: LB top-down + bottom-up passes */
 
 typedef struct {
     char algo;        /* SCC_ALGO_* */
@@ -123,8 +124,9 @@ typedef struct {
     int trim2;        /* -q */
     int warp;         /* -w: 1,2,4,8,16,32,64 */
     int expand;       /* SCC_EXPAND_* */
-    int block;        /* workgroup size for sweep kernels, 0 = 256 */
+    int block;        /* reserved, 0 */
     int check_every;  /* BFS levels between host checks, 0 = auto */
+    int sizes;        /* 1: also compute stats.largest (one more sweep) */
 } scc_opts;
 
 void scc_opts_default(scc_opts *o);
@@ -146,7 +148,7 @@ typedef struct {
     uint32_t num_trim1;      /* "numberOf1Sccs" (scc_WCC.cu:948) */
     uint32_t num_trim2;      /* 2-SCCs removed by trim-2 */
     uint32_t num_pivot;      /* SCCs found by a pivot / colour-root search */
-    uint32_t largest;        /* size of the largest SCC */
+    uint32_t largest;        /* size of the largest SCC (0 unless opts.sizes) */
     uint32_t rounds;         /* outer rounds of the main loop */
     uint32_t levels;         /* BFS levels launched, both directions */
     uint32_t first_pivot;    /* vertex elected by the first-pivot rule */

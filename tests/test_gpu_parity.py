@@ -60,9 +60,50 @@ def test_random_graphs(ctx, seed):
     for algo in ALGOS:
         lab, st = _solve(ctx, g, algo=algo, warp=int(rng.choice(scc.WARP_SIZES)))
         _check(lab, st, want, n, f"seed {seed} V={V} E={E} -a {algo}")
+        lab, st = _solve(ctx, g, algo=algo, expand=scc.EXPAND_LB)
+        _check(lab, st, want, n, f"seed {seed} V={V} E={E} -a {algo} LB")
+        lab, st = _solve(ctx, g, algo=algo, expand=scc.EXPAND_DO)
+        _check(lab, st, want, n, f"seed {seed} V={V} E={E} -a {algo} DO")
 
 
-@pytest.mark.parametrize("expand,warp", [(scc.EXPAND_VERTEX, 1)] +
+@pytest.mark.parametrize("scale", [12, 16])
+def test_rmat_lb_hubs(ctx, scale):
+    """R-MAT hubs exceed the hub threshold, so the chunked hub path runs."""
+    g = scc.gen_rmat(scale, seed=scale)
+    assert np.diff(g.Fr.astype(np.int64)).max() >= 512
+    want, n = orc.tarjan(g)
+    for algo in ALGOS:
+        for ex in (scc.EXPAND_LB, scc.EXPAND_DO):
+            lab, st = _solve(ctx, g, algo=algo, expand=ex)
+            _check(lab, st, want, n, f"rmat{scale} -a {algo} expand {ex}")
+
+
+def test_reach_direction_optimised_switches(ctx):
+    """A dense ER graph from a hub: the DO schedule takes bottom-up passes
+    and still returns exactly the BFS reach sets, with and without a dead set."""
+    g = scc.gen_er(20000, 400000, seed=5)
+    ctx.upload(g)
+    src = int(np.argmax(np.diff(g.Fr.astype(np.int64))[1:])) + 1
+    dead = (np.arange(g.V + 1) % 11 == 4).astype(np.uint8)
+    dead[src] = 0
+    for d in (0, 1):
+        for dd in (None, dead):
+            got, levels, _ = ctx.phase_reach(src, d, scc.EXPAND_DO, 1, dead=dd)
+            assert np.array_equal(got, orc.reach(g, src, d, dead=dd)), (d, dd is None)
+
+
+def test_sizes_flag(ctx):
+    g = scc.gen_rmat(12, seed=2)
+    ctx.upload(g)
+    want, n = orc.tarjan(g)
+    big = int(np.bincount(want[1:].astype(np.int64)).max())
+    assert ctx.run(algo="g", sizes=True).largest == big
+    st = ctx.run(algo="g", sizes=False)
+    assert st.largest == 0 and st.num_sccs == n
+
+
+@pytest.mark.parametrize("expand,warp", [(scc.EXPAND_VERTEX, 1), (scc.EXPAND_LB, 1),
+                                         (scc.EXPAND_DO, 1)] +
                          [(scc.EXPAND_VWARP, w) for w in scc.WARP_SIZES])
 def test_reach_matches_bfs(ctx, expand, warp):
     g = scc.gen_rmat(14, seed=3)
