@@ -353,7 +353,7 @@ scc_status Engine::pivot_round(bool first, int expand, int W, scc_stats *stats, 
     mark(SCC_PHASE_PIVOT);
     init_vis();
     zero_ctr(C_NPIV, 1);
-    k_elect(stream, g, st);
+    k_elect(stream, g, st, first);
     k_select(stream, g, st, 1);
     mark(SCC_PHASE_REACH);
     const int first_mark = (int)nmarks - 1;
@@ -562,7 +562,7 @@ scc_status Engine::phase_trim2(const uint8_t *dead, uint32_t *pair)
     if ((s = fetch32(cls.data(), st.cls)) != SCC_OK) return s;
     if ((s = fetch32(pair, st.lab)) != SCC_OK) return s;
     for (uint32_t v = 0; v <= g.V; v++)
-        if (v == 0 || cls[v] != DEAD || (dead && dead[v])) pair[v] = 0;
+        pair[v] = (v == 0 || cls[v] != DEAD || (dead && dead[v])) ? 0u : (pair[v] & ~LAB_FINAL);
     return SCC_OK;
 }
 
@@ -571,7 +571,7 @@ scc_status Engine::phase_first_pivot(const uint8_t *dead, uint32_t *pivot)
     scc_status s;
     if ((s = phase_begin(dead)) != SCC_OK) return s;
     init_vis();
-    k_elect(stream, g, st);
+    k_elect(stream, g, st, true);
     k_select(stream, g, st, 1);
     if ((s = read_ctr()) != SCC_OK) return s;
     *pivot = 0;

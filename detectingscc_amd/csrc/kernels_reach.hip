@@ -286,9 +286,10 @@ static void launch_lb(hipStream_t s, const LevelArgs &a, bool single, uint32_t b
 //
 // Two tiers: each lane probes the first BU_SHORT parents of its vertex with
 // independent loads; vertices with longer parent lists and no hit yet are
-// listed in LDS and scanned by whole waves, 64 parents per step, with a
-// ballot to stop at the first hit.
+// listed in LDS and scanned by groups of BU_GROUP lanes, BU_GROUP parents
+// per step, with a ballot to stop each group at its first hit.
 constexpr uint32_t BU_SHORT = 4;
+constexpr uint32_t BU_GROUP = 8;
 
 template <bool SINGLE>
 __device__ __forceinline__ bool visited_parent(const uint32_t *vis, const uint32_t *dbits,
@@ -330,20 +331,23 @@ __global__ __launch_bounds__(256) void expand_bu_kernel(DirArgs D, const uint32_
         }
         lq_push(q, found, u);
         __syncthreads();
-        // long parent lists: one wave per vertex
+        // long parent lists: BU_GROUP lanes per vertex, 32 vertices per pass
+        // of the workgroup, each group stops at its first hit
         const uint32_t nlong = s_nlong;
-        for (uint32_t k = threadIdx.x >> 6; k < nlong; k += blockDim.x >> 6) {
+        const uint32_t sub = threadIdx.x % BU_GROUP, grp = threadIdx.x / BU_GROUP;
+        const uint32_t gshift = wlane() & ~(BU_GROUP - 1);
+        for (uint32_t k = grp; k < nlong; k += blockDim.x / BU_GROUP) {
             const uint32_t x = s_long[k];
             const uint32_t c = SINGLE ? 0u : cls[x];
             const uint32_t e = PR[x + 1];
             bool hit = false;
-            for (uint32_t j0 = PR[x] + BU_SHORT; j0 < e && !hit; j0 += 64) {
-                const uint32_t j = j0 + wlane();
+            for (uint32_t j0 = PR[x] + BU_SHORT; j0 < e && !hit; j0 += BU_GROUP) {
+                const uint32_t j = j0 + sub;
                 const bool h = j < e && visited_parent<SINGLE>(D.vis, dbits, cls, PC[j], c);
-                hit = __ballot(h) != 0;
+                hit = ((__ballot(h) >> gshift) & ((1ull << BU_GROUP) - 1)) != 0;
             }
             bool claimed = false;
-            if (hit && wlane() == 0)
+            if (hit && sub == 0)
                 claimed = !(atomicOr(&D.vis[x >> 5], 1u << (x & 31)) & (1u << (x & 31)));
             lq_push(q, claimed, x);
         }

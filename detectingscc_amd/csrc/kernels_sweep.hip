@@ -61,7 +61,9 @@ __device__ __forceinline__ void wave_add64(uint32_t *p32, uint64_t x)
 uint32_t sweep_blocks(uint32_t V)
 {
     uint64_t b = ((uint64_t)V + SWEEP_BLOCK) / SWEEP_BLOCK;
-    return (uint32_t)(b < 1 ? 1 : (b > 4096 ? 4096 : b));
+    // at most 1024 workgroups: each ends with one atomic per counter, and
+    // same-address atomics serialise
+    return (uint32_t)(b < 1 ? 1 : (b > 1024 ? 1024 : b));
 }
 
 // ------------------------------------------------------------ init ---
@@ -92,7 +94,7 @@ __global__ void init_state_kernel(uint32_t *cls, uint32_t *lab, uint64_t *piv, c
     }
     for (uint32_t v = tid + 1; v <= V; v += gridDim.x * blockDim.x) {
         cls[v] = (dead && dead[v]) ? DEAD : 0u;
-        lab[v] = v;
+        lab[v] = v | LAB_FINAL;
     }
 }
 
@@ -124,7 +126,7 @@ __global__ void trim1_kernel(const uint32_t *__restrict__ Fr, const uint32_t *__
                 if (cls[Fc[i]] == c) { keep = true; break; }
         }
         if (!keep) {
-            lab[v] = v;
+            lab[v] = v | LAB_FINAL;
             __hip_atomic_store(&cls[v], DEAD, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             removed++;
         }
@@ -187,8 +189,8 @@ __global__ void trim2_kernel(const uint32_t *__restrict__ Fr, const uint32_t *__
         if (hit) {
             uint32_t lo = v < k ? v : k, hi = v < k ? k : v;
             if (atomicCAS(&cls[lo], c, DEAD) == c) {
-                lab[lo] = lo;
-                lab[hi] = lo;
+                lab[lo] = lo | LAB_FINAL;
+                lab[hi] = lo | LAB_FINAL;
                 __hip_atomic_store(&cls[hi], DEAD, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 pairs++;
             }
@@ -264,10 +266,40 @@ __global__ void elect_kernel(const uint32_t *__restrict__ Fr, const uint32_t *__
     }
 }
 
-void k_elect(hipStream_t s, const DevGraph &g, DevState &st)
+// First pivot: every live vertex is in class 0, so each workgroup reduces
+// its maximum key first and issues one atomicMax.
+__global__ void elect_single_kernel(const uint32_t *__restrict__ Fr,
+                                    const uint32_t *__restrict__ Br,
+                                    const uint32_t *__restrict__ cls, unsigned long long *piv,
+                                    uint32_t V)
 {
-    elect_kernel<<<sweep_blocks(g.V), SWEEP_BLOCK, 0, s>>>(
-        g.Fr, g.Br, st.cls, reinterpret_cast<unsigned long long *>(st.piv), g.V);
+    __shared__ unsigned long long s_max[16];
+    unsigned long long m = 0;
+    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x + 1; v <= V; v += gridDim.x * blockDim.x)
+        if (cls[v] != DEAD) {
+            const unsigned long long k = pivot_key(Fr, Br, v);
+            m = k > m ? k : m;
+        }
+    for (int o = 32; o > 0; o >>= 1) {
+        unsigned long long x = __shfl_xor(m, o);
+        m = x > m ? x : m;
+    }
+    if (lane() == 0) s_max[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (uint32_t w = 1; w < (blockDim.x >> 6); w++) m = s_max[w] > m ? s_max[w] : m;
+        if (m > piv[0]) atomicMax(&piv[0], m);
+    }
+}
+
+void k_elect(hipStream_t s, const DevGraph &g, DevState &st, bool single_class)
+{
+    if (single_class)
+        elect_single_kernel<<<sweep_blocks(g.V), SWEEP_BLOCK, 0, s>>>(
+            g.Fr, g.Br, st.cls, reinterpret_cast<unsigned long long *>(st.piv), g.V);
+    else
+        elect_kernel<<<sweep_blocks(g.V), SWEEP_BLOCK, 0, s>>>(
+            g.Fr, g.Br, st.cls, reinterpret_cast<unsigned long long *>(st.piv), g.V);
 }
 
 // selectPivots (scc_kernels.cu:4-19): the winner of each class seeds the
@@ -612,11 +644,14 @@ __global__ void canon_min_kernel(uint32_t *lab, uint32_t *aux, uint32_t *ctr, ui
     WAVE_LOOP(v, valid, V)
     {
         uint32_t r = valid ? lab[v] : DEAD;
-        if (valid && (r < 1 || r > V)) {   // every vertex must hold a representative
+        const bool final_lab = valid && (r & LAB_FINAL);
+        const uint32_t rr = r & ~LAB_FINAL;
+        if (valid && (rr < 1 || rr > V || (final_lab && rr > v))) {
+            // every vertex must hold a representative (a final one is <= v)
             atomicAdd(&ctr[CT(C_ANOM)], 1u);
-            lab[v] = r = v;
+            lab[v] = r = v | LAB_FINAL;
         }
-        bool pending = valid;
+        bool pending = valid && !(r & LAB_FINAL);
         for (int it = 0; it < MATCH_ROUNDS; it++) {
             const uint64_t pm = __ballot(pending);
             if (!pm) break;
@@ -641,8 +676,10 @@ __global__ void canon_apply_kernel(uint32_t *lab, const uint32_t *__restrict__ a
     uint32_t roots = 0;
     WAVE_LOOP(v, valid, V)
     {
-        const uint32_t r = valid ? aux[lab[v]] : DEAD;
+        uint32_t r = DEAD;
         if (valid) {
+            const uint32_t l = lab[v];
+            r = (l & LAB_FINAL) ? (l & ~LAB_FINAL) : aux[l];
             lab[v] = r;
             roots += r == v;
         }

@@ -25,6 +25,9 @@
 namespace sccamd {
 
 constexpr uint32_t DEAD = 0xFFFFFFFFu;
+// lab[v] flag: the label is already the canonical minimum of v's SCC (trim-1
+// singletons, trim-2 pairs), so the final min-per-SCC pass skips it.
+constexpr uint32_t LAB_FINAL = 0x80000000u;
 constexpr int SWEEP_BLOCK = 256;
 
 // Counters block: counter i lives at word CT(i), one 128-byte line per
@@ -102,7 +105,8 @@ void k_init_state(hipStream_t s, const DevGraph &g, DevState &st, const uint8_t 
 void k_trim1(hipStream_t s, const DevGraph &g, DevState &st);
 void k_trim2(hipStream_t s, const DevGraph &g, DevState &st);
 void k_init_vis(hipStream_t s, const DevGraph &g, DevState &st);
-void k_elect(hipStream_t s, const DevGraph &g, DevState &st);
+// single_class: every live vertex is in class 0 (the first pivot)
+void k_elect(hipStream_t s, const DevGraph &g, DevState &st, bool single_class);
 void k_select(hipStream_t s, const DevGraph &g, DevState &st, int seed_bw);
 void k_update(hipStream_t s, const DevGraph &g, DevState &st);
 void k_clear_piv(hipStream_t s, DevState &st, uint32_t max_pivots);
