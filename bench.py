@@ -52,6 +52,16 @@ def reach_bytes(nv, ne):
     return 12 * nv + 8 * ne
 
 
+def profiled_traffic(workload):
+    """HBM bytes of one first-pivot FW+BW search measured with rocprofv3
+    (FETCH_SIZE + WRITE_SIZE, calibrated on a plain copy), if recorded."""
+    p = os.path.join(ROOT, "profiles", f"reach_traffic_{workload}.json")
+    if not os.path.exists(p):
+        return None
+    with open(p) as f:
+        return json.load(f).get("hbm_bytes_per_search")
+
+
 def cpu_baseline(sample_scale: int, threads: int):
     import detectingscc_amd as scc
     from oracle import pyoracle as orc
@@ -84,12 +94,20 @@ def main():
     ap.add_argument("--expand", type=int, default=4, help="0 auto, 1 vertex, 2 vwarp, 3 LB, 4 DO")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-scale", type=int, default=20)
+    ap.add_argument("--copy-calibrate", type=int, default=0, metavar="BYTES",
+                    help="only run a plain device copy of BYTES (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus above 1 is not supported"}))
         sys.exit(2)
 
     import detectingscc_amd as scc
+    if args.copy_calibrate:
+        with scc.Context(0) as ctx:
+            ms = ctx.copy_bench(args.copy_calibrate, 5)
+        print(json.dumps({"copy_bytes": args.copy_calibrate, "ms_per_copy": ms,
+                          "GBps_read_plus_write": 2 * args.copy_calibrate / ms / 1e6}), flush=True)
+        return
     t0 = time.perf_counter()
     g = build_graph(args.workload)
     print(f"[bench] built {args.workload}: V={g.V} E={g.E} in {time.perf_counter() - t0:.1f}s",
@@ -130,7 +148,7 @@ def main():
                    "warp": args.warp, "expand": args.expand, "num_sccs": st.num_sccs,
                    "largest_scc": st.largest, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": profiled_traffic(args.workload),
                      "kernel": "expand (first-pivot FW+BW)", "search_ms": fr_ms,
                      "algorithmic_bytes": alg,
                      "fw": [st.first_fw_vertices, st.first_fw_arcs],

@@ -295,6 +295,13 @@ class Context:
         st = self.run(**kw)
         return self.labels(), st
 
+    def copy_bench(self, nbytes: int, iters: int = 5) -> float:
+        """Mean ms of a plain device copy of nbytes (counter calibration)."""
+        ms = ctypes.c_double(0)
+        _check(lib().scc_copy_bench(self._ctx, ctypes.c_uint64(nbytes), ctypes.c_int(iters),
+                                    ctypes.byref(ms)))
+        return ms.value
+
     # -- single phases (kernel-level checks) --
     def _dead(self, dead):
         if dead is None:

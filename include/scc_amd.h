@@ -206,6 +206,12 @@ scc_status scc_phase_min_label(scc_ctx *ctx, const uint8_t *dead, uint32_t *out)
 /* max reachable id (colorPropagation, scc_kernels.cu:539-564); 0 for dead */
 scc_status scc_phase_max_color(scc_ctx *ctx, const uint8_t *dead, uint32_t *out);
 
+/* ------------------------------------------------------- measurement --- */
+/* A plain device copy of `bytes` (read + write), `iters` times, on the
+ * context's stream; *ms_per_copy = mean duration.  Calibrates the HBM byte
+ * counters used to profile the kernels (no reference counterpart). */
+scc_status scc_copy_bench(scc_ctx *ctx, uint64_t bytes, int iters, double *ms_per_copy);
+
 /* --------------------------------------------------- drop-in wrappers --- */
 /*
  * The five drivers of include/scc.h:49-53 with the same parameters:
