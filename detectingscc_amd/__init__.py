@@ -346,6 +346,12 @@ class Context:
         _check(lib().scc_phase_min_label(self._ctx, _p8(d), _p32(out)))
         return out
 
+    def phase_local(self) -> np.ndarray:
+        """Local-SCC labels of detectSCC for the uploaded graph + partition."""
+        out = np.zeros(self.graph.V + 1, np.uint32)
+        _check(lib().scc_phase_local(self._ctx, _p32(out)))
+        return out
+
     def phase_max_color(self, dead=None) -> np.ndarray:
         d = self._dead(dead)
         out = np.zeros(self.graph.V + 1, np.uint32)

@@ -138,6 +138,14 @@ scc_status scc_phase_min_label(scc_ctx *ctx, const uint8_t *dead, uint32_t *out)
     return ctx->eng.phase_min_label(dead, out);
 }
 
+scc_status scc_phase_local(scc_ctx *ctx, uint32_t *out)
+{
+    clear_error();
+    CTX_CHECK(ctx);
+    if (!out) return fail(SCC_ERR_INVALID, "null output");
+    return ctx->eng.phase_local(out);
+}
+
 scc_status scc_phase_max_color(scc_ctx *ctx, const uint8_t *dead, uint32_t *out)
 {
     clear_error();

@@ -1,5 +1,6 @@
 // engine.h -- the device-resident SCC engine behind the C ABI.
 #pragma once
+#include <memory>
 #include <vector>
 
 #include "scc_amd.h"
@@ -18,7 +19,10 @@ class Engine {
     scc_status upload(uint32_t V, uint32_t E, const uint32_t *Fr, const uint32_t *Fc,
                       const uint32_t *Br, const uint32_t *Bc);
     scc_status upload_partition(const uint32_t *part);
+    scc_status upload_device(uint32_t V, uint32_t E, const uint32_t *dFr, const uint32_t *dFc,
+                             const uint32_t *dBr, const uint32_t *dBc, hipStream_t from);
     scc_status run(const scc_opts &o, scc_stats *out);
+    scc_status phase_local(uint32_t *out);
     scc_status labels(uint32_t *lab);
 
     scc_status phase_trim1(const uint8_t *dead, uint8_t *removed);
@@ -51,7 +55,12 @@ class Engine {
     scc_status pivot_round(bool first, int expand, int W, scc_stats *stats, uint32_t *live);
     scc_status min_label();
     scc_status color_loop(int expand, int W, scc_stats *stats);
-    scc_status run_detect(const scc_opts &o, scc_stats *out);
+    scc_status detect_phases(const scc_opts &o, int expand, int W, scc_stats *stats, bool local_only);
+    struct DevBuf {
+        void *p = nullptr;
+        size_t cap = 0;
+    };
+    template <class T> scc_status scratch(DevBuf &b, size_t n, T **out);
     scc_status phase_begin(const uint8_t *dead);
     scc_status fetch8(uint8_t *out);
     scc_status fetch32(uint32_t *out, const uint32_t *src);
@@ -64,6 +73,9 @@ class Engine {
     std::vector<int> mark_phase;
     size_t nmarks = 0;
     int first_reach_marks[2] = {-1, -1};
+    uint32_t num_parts = 0;
+    DevBuf buf[12];                    // detectSCC scratch (lazily sized)
+    std::unique_ptr<Engine> sub;       // engine for the quotient graph
 };
 
 } // namespace sccamd

@@ -14,8 +14,11 @@
 #include <string>
 #include <vector>
 
+#include <algorithm>
+
 #include "engine.h"
 #include "host_util.h"
+#include "merge.h"
 
 namespace sccamd {
 
@@ -47,6 +50,11 @@ void Engine::release()
         f(st.bigoff_f[i]); f(st.bigoff_b[i]);
     }
     f(st.pivlist); f(st.part); f(st.ctr); f(d_tmp8);
+    for (auto &b : buf) {
+        f(b.p);
+        b = DevBuf();
+    }
+    sub.reset();
     if (h_ctr) (void)hipHostFree(h_ctr);
     for (auto e : events) (void)hipEventDestroy(e);
     events.clear();
@@ -174,11 +182,64 @@ scc_status Engine::upload(uint32_t V, uint32_t E, const uint32_t *Fr, const uint
     return SCC_OK;
 }
 
+// Partition ids are remapped to dense 0..K-1 (order kept) so that the local
+// classes 3*id stay inside the pivot table.
 scc_status Engine::upload_partition(const uint32_t *part)
 {
     if (!has_graph) return fail(SCC_ERR_NOTREADY, "no graph uploaded");
-    HIP_TRY(hipMemcpy(st.part, part, sizeof(uint32_t) * ((size_t)g.V + 2), hipMemcpyHostToDevice));
+    std::vector<uint32_t> ids(part + 1, part + 1 + g.V);
+    std::sort(ids.begin(), ids.end());
+    ids.erase(std::unique(ids.begin(), ids.end()), ids.end());
+    std::vector<uint32_t> dense((size_t)g.V + 2, 0);
+    parallel_chunks(g.V, [&](uint64_t b, uint64_t e, int) {
+        for (uint64_t v = b + 1; v <= e; v++)
+            dense[v] = (uint32_t)(std::lower_bound(ids.begin(), ids.end(), part[v]) - ids.begin());
+    });
+    HIP_TRY(hipMemcpy(st.part, dense.data(), sizeof(uint32_t) * ((size_t)g.V + 2), hipMemcpyHostToDevice));
+    num_parts = (uint32_t)ids.size();
     has_part = true;
+    return SCC_OK;
+}
+
+// Adopt a graph that is already in device memory (copied device to device).
+// Used for the quotient graph, whose CSR is built by kernels and valid by
+// construction.
+scc_status Engine::upload_device(uint32_t V, uint32_t E, const uint32_t *dFr, const uint32_t *dFc,
+                                 const uint32_t *dBr, const uint32_t *dBc, hipStream_t from)
+{
+    if (V >= (1u << 30)) return fail(SCC_ERR_INVALID, "V=%u exceeds 2^30-1", V);
+    scc_status s;
+    if ((s = reserve(V, E)) != SCC_OK) return s;
+    g.V = V;
+    g.E = E;
+    HIP_TRY(hipStreamSynchronize(from));
+    HIP_TRY(hipMemcpyAsync(g.Fr, dFr, sizeof(uint32_t) * ((size_t)V + 2), hipMemcpyDeviceToDevice, stream));
+    HIP_TRY(hipMemcpyAsync(g.Br, dBr, sizeof(uint32_t) * ((size_t)V + 2), hipMemcpyDeviceToDevice, stream));
+    if (E) {
+        HIP_TRY(hipMemcpyAsync(g.Fc, dFc, sizeof(uint32_t) * E, hipMemcpyDeviceToDevice, stream));
+        HIP_TRY(hipMemcpyAsync(g.Bc, dBc, sizeof(uint32_t) * E, hipMemcpyDeviceToDevice, stream));
+    }
+    has_graph = true;
+    has_part = false;
+    ran = false;
+    return SCC_OK;
+}
+
+template <class T> scc_status Engine::scratch(DevBuf &b, size_t n, T **out)
+{
+    const size_t bytes = sizeof(T) * (n ? n : 1);
+    if (b.cap < bytes) {
+        if (b.p) (void)hipFree(b.p);
+        b.p = nullptr;
+        b.cap = 0;
+        hipError_t e = hipMalloc(&b.p, bytes);
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            return fail(SCC_ERR_NOMEM, "Error: Not enough memory on GPU (%zu scratch bytes)", bytes);
+        }
+        b.cap = bytes;
+    }
+    *out = static_cast<T *>(b.p);
     return SCC_OK;
 }
 
@@ -425,7 +486,8 @@ scc_status Engine::run(const scc_opts &o, scc_stats *out)
     const char A = o.algo;
     if (A != 'g' && A != 'h' && A != 'x' && A != 'y' && A != 'd')
         return fail(SCC_ERR_INVALID, "unknown algorithm '%c'", A);
-    if (A == 'd') return run_detect(o, out);
+    if (A == 'd' && !has_part)
+        return fail(SCC_ERR_NOTREADY, "detectSCC needs a partition vector (scc_partition_upload)");
     int W = o.warp;
     if (W != 1 && W != 2 && W != 4 && W != 8 && W != 16 && W != 32 && W != 64)
         return fail(SCC_ERR_INVALID, "warp size %d not in {1,2,4,8,16,32,64}", W);
@@ -450,7 +512,9 @@ scc_status Engine::run(const scc_opts &o, scc_stats *out)
     scc_status s;
     uint32_t live = g.V;
     const bool hong = (A == 'g' || A == 'x');
-    if (hong) {
+    if (A == 'd') {
+        if ((s = detect_phases(o, loop_expand, W, &stats, false)) != SCC_OK) return s;
+    } else if (hong) {
         if (o.trim1 && (s = trim1(true)) != SCC_OK) return s;
         if ((s = pivot_round(true, first_expand, W, &stats, &live)) != SCC_OK) return s;
         if (o.trim1 && (s = trim1(true)) != SCC_OK) return s;
@@ -473,8 +537,10 @@ scc_status Engine::run(const scc_opts &o, scc_stats *out)
         if ((s = pivot_round(true, first_expand, W, &stats, &live)) != SCC_OK) return s;
         if ((s = color_loop(loop_expand, W, &stats)) != SCC_OK) return s;
     }
-    mark(SCC_PHASE_LABEL);
-    k_canon(stream, g, st, o.sizes != 0);
+    if (A != 'd') {   // detectSCC labels come out of the merge already canonical
+        mark(SCC_PHASE_LABEL);
+        k_canon(stream, g, st, o.sizes != 0);
+    }
     mark(-1);
     if ((s = read_ctr()) != SCC_OK) return s;
     if (h_ctr[CT(C_ANOM)])
@@ -500,9 +566,85 @@ scc_status Engine::run(const scc_opts &o, scc_stats *out)
     return SCC_OK;
 }
 
-scc_status Engine::run_detect(const scc_opts &, scc_stats *)
+// detectSCC (scc_WCC.cu:1000-1617): global trims (1122-1173), local SCCs
+// per (partition, class) (1175-1269), then the merge as a quotient-graph
+// SCC computed by a second engine on the same device.  With local_only the
+// canonical local labels are left in lab.
+scc_status Engine::detect_phases(const scc_opts &o, int expand, int W, scc_stats *stats, bool local_only)
 {
-    return fail(SCC_ERR_UNSUPPORTED, "detectSCC is not available in this build");
+    if (!has_part) return fail(SCC_ERR_NOTREADY, "detectSCC needs a partition vector");
+    scc_status s;
+    if (o.trim1 && (s = trim1(true)) != SCC_OK) return s;
+    if (o.trim2) {
+        mark(SCC_PHASE_TRIM2);
+        k_trim2(stream, g, st);
+    }
+    if (o.trim1 && (s = trim1(true)) != SCC_OK) return s;
+    mark(SCC_PHASE_LOCAL);
+    const size_t nv = (size_t)g.V + 1;
+    uint32_t *tbits, *isrep, *qid, *rep, *cout, *cin, *Qr, *QBr, *tiles;
+    if ((s = scratch(buf[0], st.nwords, &tbits)) != SCC_OK) return s;
+    k_local_init(stream, g, st, tbits);
+    uint32_t live = 0;
+    for (;;) {
+        if ((s = pivot_round(false, expand, W, stats, &live)) != SCC_OK) return s;
+        if (!live) break;
+        if (++stats->rounds > limit()) return no_convergence("local rounds");
+    }
+    mark(SCC_PHASE_LABEL);
+    k_canon(stream, g, st, false);
+    if (local_only) return SCC_OK;
+    mark(SCC_PHASE_MERGE);
+    if ((s = scratch(buf[1], nv, &isrep)) != SCC_OK || (s = scratch(buf[2], nv, &qid)) != SCC_OK ||
+        (s = scratch(buf[3], nv, &rep)) != SCC_OK || (s = scratch(buf[4], nv + 1, &cout)) != SCC_OK ||
+        (s = scratch(buf[5], nv + 1, &cin)) != SCC_OK || (s = scratch(buf[6], nv + 1, &Qr)) != SCC_OK ||
+        (s = scratch(buf[7], nv + 1, &QBr)) != SCC_OK ||
+        (s = scratch(buf[8], scan_tile_count(nv + 1), &tiles)) != SCC_OK)
+        return s;
+    k_q_mark(stream, g, st, tbits, isrep);
+    k_exscan(stream, isrep, qid, nv, tiles);
+    uint32_t last[2];
+    HIP_TRY(hipMemcpyAsync(&last[0], qid + g.V, 4, hipMemcpyDeviceToHost, stream));
+    HIP_TRY(hipMemcpyAsync(&last[1], isrep + g.V, 4, hipMemcpyDeviceToHost, stream));
+    HIP_TRY(hipStreamSynchronize(stream));
+    const uint32_t QV = last[0] + last[1];
+    const uint32_t *qlab = nullptr;
+    if (QV) {
+        k_fill_u32(stream, cout, 0, (uint64_t)QV + 2);
+        k_fill_u32(stream, cin, 0, (uint64_t)QV + 2);
+        k_q_arcs(stream, 0, g, st, tbits, qid, cout, cin, nullptr, nullptr);
+        k_exscan(stream, cout, Qr, (uint64_t)QV + 2, tiles);
+        k_exscan(stream, cin, QBr, (uint64_t)QV + 2, tiles);
+        uint32_t QE = 0;
+        HIP_TRY(hipMemcpyAsync(&QE, Qr + QV + 1, 4, hipMemcpyDeviceToHost, stream));
+        HIP_TRY(hipStreamSynchronize(stream));
+        if (QE) {
+            uint32_t *Qc, *QBc;
+            if ((s = scratch(buf[9], QE, &Qc)) != SCC_OK || (s = scratch(buf[10], QE, &QBc)) != SCC_OK)
+                return s;
+            HIP_TRY(hipMemcpyAsync(cout, Qr, sizeof(uint32_t) * ((size_t)QV + 2), hipMemcpyDeviceToDevice, stream));
+            HIP_TRY(hipMemcpyAsync(cin, QBr, sizeof(uint32_t) * ((size_t)QV + 2), hipMemcpyDeviceToDevice, stream));
+            k_q_arcs(stream, 1, g, st, tbits, qid, cout, cin, Qc, QBc);
+            k_q_rep(stream, g, isrep, qid, rep);
+            if (!sub) {
+                sub.reset(new Engine());
+                if ((s = sub->init(device)) != SCC_OK) return s;
+            }
+            if ((s = sub->upload_device(QV, QE, Qr, Qc, QBr, QBc, stream)) != SCC_OK) return s;
+            scc_opts so;
+            scc_opts_default(&so);
+            so.algo = 'g';
+            so.expand = SCC_EXPAND_DO;
+            so.sizes = 0;
+            scc_stats sst;
+            if ((s = sub->run(so, &sst)) != SCC_OK) return s;
+            qlab = sub->st.lab;
+            stats->levels += sst.levels;
+        }
+    }
+    zero_ctr(C_NSCC, 1);
+    k_q_map(stream, g, st, tbits, qid, qlab, rep);
+    return SCC_OK;
 }
 
 scc_status Engine::labels(uint32_t *lab)
@@ -537,6 +679,22 @@ scc_status Engine::fetch32(uint32_t *out, const uint32_t *src)
 {
     HIP_TRY(hipMemcpyAsync(out, src, sizeof(uint32_t) * ((size_t)g.V + 1), hipMemcpyDeviceToHost, stream));
     HIP_TRY(hipStreamSynchronize(stream));
+    return SCC_OK;
+}
+
+scc_status Engine::phase_local(uint32_t *out)
+{
+    if (!has_part) return fail(SCC_ERR_NOTREADY, "no partition uploaded");
+    scc_status s;
+    if ((s = phase_begin(nullptr)) != SCC_OK) return s;
+    nmarks = 0;
+    scc_opts o;
+    scc_opts_default(&o);
+    scc_stats tmp;
+    std::memset(&tmp, 0, sizeof tmp);
+    if ((s = detect_phases(o, SCC_EXPAND_DO, 1, &tmp, true)) != SCC_OK) return s;
+    if ((s = fetch32(out, st.lab)) != SCC_OK) return s;
+    out[0] = 0;
     return SCC_OK;
 }
 
