@@ -690,6 +690,10 @@ scc_status Engine::phase_local(uint32_t *out)
     nmarks = 0;
     scc_opts o;
     scc_opts_default(&o);
+    // trim-2 would finish a 2-cycle whose ends lie in different partitions as
+    // one pair: a global SCC but two local ones.  Trim-1 removes only
+    // vertices that are singletons locally as well.
+    o.trim2 = 0;
     scc_stats tmp;
     std::memset(&tmp, 0, sizeof tmp);
     if ((s = detect_phases(o, SCC_EXPAND_DO, 1, &tmp, true)) != SCC_OK) return s;

@@ -205,9 +205,10 @@ scc_status scc_phase_reach(scc_ctx *ctx, uint32_t src, int dir, int expand, int 
 scc_status scc_phase_min_label(scc_ctx *ctx, const uint8_t *dead, uint32_t *out);
 /* max reachable id (colorPropagation, scc_kernels.cu:539-564); 0 for dead */
 scc_status scc_phase_max_color(scc_ctx *ctx, const uint8_t *dead, uint32_t *out);
-/* local SCCs of detectSCC (scc_WCC.cu:1122-1269): global trims, then SCCs of
- * the union of partition-induced subgraphs; canonical labels (V+1) of the
- * uploaded graph and partition */
+/* local SCCs of detectSCC (scc_WCC.cu:1122-1269): global trim-1, then SCCs
+ * of the union of partition-induced subgraphs; canonical labels (V+1) of the
+ * uploaded graph and partition (trim-2 is left out: a 2-cycle across two
+ * partitions is one global SCC but two local ones) */
 scc_status scc_phase_local(scc_ctx *ctx, uint32_t *out);
 
 /* ------------------------------------------------------- measurement --- */
