@@ -64,6 +64,30 @@ class _Stats(ctypes.Structure):
                 ("ms_phase", ctypes.c_double * len(PHASES))]
 
 
+class _BatchHost(ctypes.Structure):
+    _fields_ = [("num_graphs", ctypes.c_uint32), ("V", _u32p), ("E", _u32p),
+                ("row_off", _u64p), ("col_off", _u64p),
+                ("Fr", _u32p), ("Fc", _u32p), ("Br", _u32p), ("Bc", _u32p), ("kind", _u8p)]
+
+
+class _BatchOpts(ctypes.Structure):
+    _fields_ = [("algo", ctypes.c_char), ("trim1", ctypes.c_int), ("trim2", ctypes.c_int),
+                ("lanes", ctypes.c_int), ("block", ctypes.c_int), ("autotuned", ctypes.c_int)]
+
+
+class _BatchStats(ctypes.Structure):
+    _fields_ = [("graphs", ctypes.c_uint32), ("ms", ctypes.c_double), ("graphs_per_s", ctypes.c_double),
+                ("p50_ms", ctypes.c_double), ("p99_ms", ctypes.c_double),
+                ("vertices", ctypes.c_uint64), ("arcs", ctypes.c_uint64), ("launches", ctypes.c_uint32)]
+
+
+class _ServerStats(ctypes.Structure):
+    _fields_ = [("submitted", ctypes.c_uint64), ("completed", ctypes.c_uint64),
+                ("rejected", ctypes.c_uint64), ("failed", ctypes.c_uint64), ("batches", ctypes.c_uint64),
+                ("p50_ms", ctypes.c_double), ("p99_ms", ctypes.c_double), ("mean_ms", ctypes.c_double),
+                ("graphs_per_s", ctypes.c_double), ("peak_bytes", ctypes.c_uint64)]
+
+
 _LIB = None
 
 
@@ -77,6 +101,14 @@ def lib() -> ctypes.CDLL:
         for name in ["scc_status_string", "scc_last_error", "scc_version"]:
             getattr(L, name).restype = ctypes.c_char_p
         L.scc_status_string.argtypes = [ctypes.c_int]
+        vp = ctypes.c_void_p
+        L.scc_gen_batch_mix.argtypes = [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_double,
+                                        ctypes.c_double, ctypes.c_uint64, ctypes.POINTER(_BatchHost)]
+        L.scc_batch_autotune.argtypes = [vp, ctypes.c_int, ctypes.c_char_p, ctypes.c_size_t]
+        L.scc_server_start.argtypes = [vp, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32,
+                                       ctypes.POINTER(_BatchOpts)]
+        L.scc_server_submit.argtypes = [vp, ctypes.POINTER(_HostGraph), ctypes.POINTER(ctypes.c_uint64)]
+        L.scc_server_poll.argtypes = [vp, ctypes.c_uint64, _u32p, _u32p, ctypes.POINTER(ctypes.c_double)]
         _LIB = L
     return _LIB
 
@@ -242,6 +274,110 @@ def make_opts(algo: str = "g", trim1: bool = True, trim2: bool = True, warp: int
     return o
 
 
+@dataclass
+class Batch:
+    """Many graphs in one host layout: graph i's rows start at row_off[i]
+    (V[i]+2 entries, local ids 1..V[i]), its columns at col_off[i]."""
+    V: np.ndarray
+    E: np.ndarray
+    row_off: np.ndarray
+    col_off: np.ndarray
+    Fr: np.ndarray
+    Fc: np.ndarray
+    Br: np.ndarray
+    Bc: np.ndarray
+    kind: np.ndarray | None = None
+
+    @property
+    def num_graphs(self) -> int:
+        return int(self.V.size)
+
+    def graph(self, i: int) -> Graph:
+        r, c = int(self.row_off[i]), int(self.col_off[i])
+        V, E = int(self.V[i]), int(self.E[i])
+        return Graph(V, E, self.Fr[r:r + V + 2].copy(), self.Fc[c:c + E].copy(),
+                     self.Br[r:r + V + 2].copy(), self.Bc[c:c + E].copy())
+
+    def _host(self) -> _BatchHost:
+        h = _BatchHost()
+        h.num_graphs = self.num_graphs
+        h.V, h.E = _p32(self.V), _p32(self.E)
+        h.row_off = self.row_off.ctypes.data_as(_u64p)
+        h.col_off = self.col_off.ctypes.data_as(_u64p)
+        h.Fr, h.Fc, h.Br, h.Bc = _p32(self.Fr), _p32(self.Fc), _p32(self.Br), _p32(self.Bc)
+        h.kind = None
+        return h
+
+
+def batch_from_graphs(graphs: list[Graph]) -> Batch:
+    V = np.array([g.V for g in graphs], np.uint32)
+    E = np.array([g.E for g in graphs], np.uint32)
+    ro = np.zeros(len(graphs) + 1, np.uint64)
+    co = np.zeros(len(graphs) + 1, np.uint64)
+    ro[1:] = np.cumsum(V.astype(np.uint64) + 2)
+    co[1:] = np.cumsum(E.astype(np.uint64))
+    cat = lambda xs: np.ascontiguousarray(np.concatenate(xs) if xs else np.zeros(0), dtype=np.uint32)
+    return Batch(V, E, ro, co, cat([g.Fr for g in graphs]), cat([g.Fc for g in graphs]),
+                 cat([g.Br for g in graphs]), cat([g.Bc for g in graphs]))
+
+
+def gen_batch_mix(num_graphs: int, lgmin: int = 8, lgmax: int = 16, dmin: float = 2.0,
+                  dmax: float = 16.0, seed: int = 4096) -> Batch:
+    """The batched mix: V log-uniform in [2^lgmin, 2^lgmax], mean degree
+    uniform in [dmin, dmax], 40% ER / 40% R-MAT / 20% planted cycles + ER."""
+    h = _BatchHost()
+    _check(lib().scc_gen_batch_mix(num_graphs, lgmin, lgmax, dmin, dmax, seed, ctypes.byref(h)))
+    G = int(h.num_graphs)
+    def arr(p, n, dt):
+        return np.ctypeslib.as_array(p, shape=(max(n, 1),))[:n].astype(dt, copy=True)
+    ro = arr(h.row_off, G + 1, np.uint64)
+    co = arr(h.col_off, G + 1, np.uint64)
+    b = Batch(arr(h.V, G, np.uint32), arr(h.E, G, np.uint32), ro, co,
+              arr(h.Fr, int(ro[-1]), np.uint32), arr(h.Fc, int(co[-1]), np.uint32),
+              arr(h.Br, int(ro[-1]), np.uint32), arr(h.Bc, int(co[-1]), np.uint32),
+              arr(h.kind, G, np.uint8) if h.kind else None)
+    lib().scc_batch_host_free(ctypes.byref(h))
+    return b
+
+
+@dataclass
+class BatchStats:
+    graphs: int
+    ms: float
+    graphs_per_s: float
+    p50_ms: float
+    p99_ms: float
+    vertices: int
+    arcs: int
+    launches: int
+
+
+@dataclass
+class ServerStats:
+    submitted: int
+    completed: int
+    rejected: int
+    failed: int
+    batches: int
+    p50_ms: float
+    p99_ms: float
+    mean_ms: float
+    graphs_per_s: float
+    peak_bytes: int
+
+
+def _batch_stats(s: _BatchStats) -> BatchStats:
+    return BatchStats(s.graphs, s.ms, s.graphs_per_s, s.p50_ms, s.p99_ms, s.vertices, s.arcs, s.launches)
+
+
+def make_batch_opts(algo: str = "g", trim1: bool = True, trim2: bool = True, lanes: int = 4,
+                    block: int = 256, autotuned: bool = False) -> _BatchOpts:
+    o = _BatchOpts()
+    o.algo = algo.encode()[:1]
+    o.trim1, o.trim2, o.lanes, o.block, o.autotuned = int(trim1), int(trim2), int(lanes), int(block), int(autotuned)
+    return o
+
+
 class Context:
     """A device context: resident graph + state on one HIP device."""
 
@@ -249,6 +385,8 @@ class Context:
         self._ctx = ctypes.c_void_p()
         _check(lib().scc_ctx_create(ctypes.c_int(device), ctypes.byref(self._ctx)))
         self.graph: Graph | None = None
+        self._batch: Batch | None = None
+        self._served: dict[int, int] = {}
 
     def close(self) -> None:
         if self._ctx:
@@ -301,6 +439,75 @@ class Context:
         _check(lib().scc_copy_bench(self._ctx, ctypes.c_uint64(nbytes), ctypes.c_int(iters),
                                     ctypes.byref(ms)))
         return ms.value
+
+    # -- batches --
+    def batch_upload(self, b: Batch) -> None:
+        self._batch = b
+        _check(lib().scc_batch_upload(self._ctx, ctypes.byref(b._host())))
+
+    def batch_run(self, algo: str = "g", trim1: bool = True, trim2: bool = True, lanes: int = 4,
+                  block: int = 256, autotuned: bool = False) -> BatchStats:
+        o = make_batch_opts(algo, trim1, trim2, lanes, block, autotuned)
+        s = _BatchStats()
+        _check(lib().scc_batch_run(self._ctx, ctypes.byref(o), ctypes.byref(s)))
+        return _batch_stats(s)
+
+    def batch_run_serial(self, **kw) -> BatchStats:
+        kw.setdefault("sizes", False)
+        o = make_opts(**kw)
+        s = _BatchStats()
+        _check(lib().scc_batch_run_serial(self._ctx, ctypes.byref(o), ctypes.byref(s)))
+        return _batch_stats(s)
+
+    def batch_autotune(self, sample: int = 16) -> str:
+        buf = ctypes.create_string_buffer(8192)
+        _check(lib().scc_batch_autotune(self._ctx, sample, buf, len(buf)))
+        return buf.value.decode()
+
+    def batch_labels(self) -> tuple[list[np.ndarray], np.ndarray]:
+        b = self._batch
+        lab = np.zeros(int(b.row_off[-1]), np.uint32)
+        nscc = np.zeros(b.num_graphs, np.uint32)
+        _check(lib().scc_batch_labels(self._ctx, _p32(lab), _p32(nscc)))
+        return [lab[int(b.row_off[i]):int(b.row_off[i]) + int(b.V[i]) + 1] for i in range(b.num_graphs)], nscc
+
+    # -- continuous-batching server --
+    def server_start(self, budget_bytes: int, max_batch: int = 256, slots: int = 2, algo: str = "g",
+                     trim1: bool = True, trim2: bool = True, lanes: int = 4, block: int = 256) -> None:
+        o = make_batch_opts(algo, trim1, trim2, lanes, block, False)
+        _check(lib().scc_server_start(self._ctx, budget_bytes, max_batch, slots, ctypes.byref(o)))
+        self._served = {}
+
+    def server_submit(self, g: Graph) -> int:
+        hg = _HostGraph(g.V, g.E, _p32(g.Fr), _p32(g.Fc), _p32(g.Br), _p32(g.Bc))
+        rid = ctypes.c_uint64(0)
+        _check(lib().scc_server_submit(self._ctx, ctypes.byref(hg), ctypes.byref(rid)))
+        self._served[rid.value] = g.V
+        return rid.value
+
+    def server_poll(self, rid: int):
+        """None while pending; (labels, nscc, latency_ms) once done."""
+        lab = np.zeros(self._served[rid] + 1, np.uint32)
+        n = ctypes.c_uint32(0)
+        ms = ctypes.c_double(0)
+        st = lib().scc_server_poll(self._ctx, rid, _p32(lab), ctypes.byref(n), ctypes.byref(ms))
+        if st == 8:
+            return None
+        del self._served[rid]
+        _check(st)
+        return lab, n.value, ms.value
+
+    def server_wait_all(self) -> None:
+        _check(lib().scc_server_wait_all(self._ctx))
+
+    def server_stats(self) -> ServerStats:
+        s = _ServerStats()
+        _check(lib().scc_server_stats_get(self._ctx, ctypes.byref(s)))
+        return ServerStats(s.submitted, s.completed, s.rejected, s.failed, s.batches, s.p50_ms, s.p99_ms,
+                           s.mean_ms, s.graphs_per_s, s.peak_bytes)
+
+    def server_stop(self) -> None:
+        _check(lib().scc_server_stop(self._ctx))
 
     # -- single phases (kernel-level checks) --
     def _dead(self, dead):

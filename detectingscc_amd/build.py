@@ -28,8 +28,8 @@ ARCH = os.environ.get("SCC_OFFLOAD_ARCH", "gfx950")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 CXX = os.environ.get("CXX", "g++")
 
-HIP_SOURCES = ["kernels_sweep.hip", "kernels_reach.hip", "kernels_merge.hip", "engine.hip",
-               "capi.hip", "measure.hip"]
+HIP_SOURCES = ["kernels_sweep.hip", "kernels_reach.hip", "kernels_merge.hip", "kernels_batch.hip",
+               "engine.hip", "batch_engine.hip", "server.hip", "capi.hip", "measure.hip"]
 HOST_SOURCES = ["graph_io.cpp", "generators.cpp"]
 
 HIP_FLAGS = [

@@ -7,10 +7,15 @@
 #include <mutex>
 #include <new>
 
+#include "batch_engine.h"
 #include "engine.h"
 #include "host_util.h"
+#include "server.h"
 
 using namespace sccamd;
+
+// the server's slot threads stop before the engines they share the device with
+scc_ctx::~scc_ctx() { server.reset(); }
 
 #define CTX_CHECK(ctx)                                                                      \
     do {                                                                                    \

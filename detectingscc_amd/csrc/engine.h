@@ -80,6 +80,18 @@ class Engine {
 
 } // namespace sccamd
 
+namespace sccamd {
+class BatchEngine;
+class Server;
+}
+
+// A context: the single-graph engine, plus the batched form and its
+// one-graph-per-launch baseline engine when used.  Destroyed in capi.hip,
+// which sees the complete types.
 struct scc_ctx {
     sccamd::Engine eng;
+    std::unique_ptr<sccamd::BatchEngine> batch;
+    std::unique_ptr<sccamd::Engine> serial;
+    std::unique_ptr<sccamd::Server> server;
+    ~scc_ctx();
 };

@@ -24,7 +24,7 @@ std::vector<uint32_t> seeded_permutation(uint32_t n, uint64_t seed)
 }
 
 void er_arcs(uint32_t V, uint64_t draws, uint64_t seed, std::vector<uint32_t> &s,
-             std::vector<uint32_t> &d)
+             std::vector<uint32_t> &d, int threads)
 {
     size_t base = s.size();
     std::vector<uint32_t> u(draws), v(draws);
@@ -33,7 +33,7 @@ void er_arcs(uint32_t V, uint64_t draws, uint64_t seed, std::vector<uint32_t> &s
             u[k] = 1 + (uint32_t)below(rnd(seed, 1, 2 * k), V);
             v[k] = 1 + (uint32_t)below(rnd(seed, 1, 2 * k + 1), V);
         }
-    });
+    }, threads);
     s.reserve(base + draws);
     d.reserve(base + draws);
     for (uint64_t k = 0; k < draws; k++)
@@ -71,7 +71,7 @@ void planted_cycles(uint32_t V, uint32_t cycles, uint32_t cmin, uint32_t cmax, u
 // R-MAT graphs of any vertex count.
 void rmat_arcs(uint32_t scale, uint32_t vcap, uint64_t E, double a, double b, double c,
                uint64_t seed, const std::vector<uint32_t> *perm, std::vector<uint32_t> &s,
-               std::vector<uint32_t> &d)
+               std::vector<uint32_t> &d, int threads)
 {
     const uint32_t ta = (uint32_t)std::lround(a * 65536.0);
     const uint32_t tab = (uint32_t)std::lround((a + b) * 65536.0);
@@ -104,7 +104,40 @@ void rmat_arcs(uint32_t scale, uint32_t vcap, uint64_t E, double a, double b, do
                 break;
             }
         }
-    });
+    }, threads);
+}
+
+// Graph `index` of the batched mix: V log-uniform in
+// [2^lgmin, 2^lgmax], mean out-degree uniform in [dmin, dmax]; 40% ER,
+// 40% R-MAT (a=.57 b=.19 c=.19, ids above V redrawn, seeded permutation),
+// 20% planted directed cycles of length 2..512 carrying about half of the
+// arcs plus ER arcs for the rest.  Single-threaded: callers run many
+// graphs in parallel.
+MixGraph mix_graph(uint64_t seed, uint32_t index, uint32_t lgmin, uint32_t lgmax, double dmin, double dmax)
+{
+    MixGraph m;
+    const uint64_t gs = mix64(seed * 0x9E3779B97F4A7C15ull + index + 1);
+    const double lg = lgmin + unit(rnd(gs, 40, 0)) * (double)(lgmax - lgmin);
+    m.V = (uint32_t)std::llround(std::pow(2.0, lg));
+    if (m.V < 2) m.V = 2;
+    const double deg = dmin + unit(rnd(gs, 40, 1)) * (dmax - dmin);
+    const uint64_t E = (uint64_t)std::llround(deg * m.V);
+    const double t = unit(rnd(gs, 40, 2));
+    m.kind = t < 0.4 ? 0 : (t < 0.8 ? 1 : 2);
+    if (m.kind == 0) {
+        er_arcs(m.V, E, gs, m.s, m.d, 1);
+    } else if (m.kind == 1) {
+        uint32_t scale = 1;
+        while ((1ull << scale) < m.V) scale++;
+        std::vector<uint32_t> perm = seeded_permutation(m.V, gs);
+        rmat_arcs(scale, m.V, E, 0.57, 0.19, 0.19, gs, &perm, m.s, m.d, 1);
+    } else {
+        const uint32_t cycles = (uint32_t)std::max<uint64_t>(1, E / (2 * 257));
+        planted_cycles(m.V, cycles, 2, 512, gs, m.s, m.d);
+        const uint64_t have = m.s.size();
+        if (have < E) er_arcs(m.V, E - have, gs ^ 0x5bd1e995ull, m.s, m.d, 1);
+    }
+    return m;
 }
 
 void grid_arcs(uint32_t side, double p, uint64_t seed, std::vector<uint32_t> &s,
@@ -182,6 +215,79 @@ scc_status scc_gen_grid(uint32_t side, double p, uint64_t seed, scc_host_graph *
     std::vector<uint32_t> s, d;
     grid_arcs(side, p, seed, s, d);
     return finish(side * side, s, d, out);
+}
+
+void scc_batch_host_free(scc_batch_host *b)
+{
+    if (!b) return;
+    std::free(b->V);
+    std::free(b->E);
+    std::free(b->row_off);
+    std::free(b->col_off);
+    std::free(b->Fr);
+    std::free(b->Fc);
+    std::free(b->Br);
+    std::free(b->Bc);
+    std::free(b->kind);
+    std::memset(b, 0, sizeof *b);
+}
+
+scc_status scc_gen_batch_mix(uint32_t num_graphs, uint32_t lgmin, uint32_t lgmax, double dmin,
+                             double dmax, uint64_t seed, scc_batch_host *out)
+{
+    clear_error();
+    if (!out || !num_graphs || lgmin < 1 || lgmax < lgmin || lgmax > 24 || dmin < 0 || dmax < dmin)
+        return fail(SCC_ERR_INVALID, "bad batch-mix arguments");
+    const uint32_t G = num_graphs;
+    // pass 1: sizes (cheap, deterministic) so the arrays can be laid out
+    std::vector<MixGraph> graphs(G);
+    std::atomic<uint32_t> next{0};
+    std::vector<std::thread> th;
+    const int nt = host_threads();
+    for (int t = 0; t < nt; t++)
+        th.emplace_back([&] {
+            for (uint32_t i; (i = next.fetch_add(1)) < G;) graphs[i] = mix_graph(seed, i, lgmin, lgmax, dmin, dmax);
+        });
+    for (auto &x : th) x.join();
+    scc_batch_host b;
+    std::memset(&b, 0, sizeof b);
+    b.num_graphs = G;
+    b.V = xalloc<uint32_t>(G);
+    b.E = xalloc<uint32_t>(G);
+    b.kind = xalloc<uint8_t>(G);
+    b.row_off = xalloc<uint64_t>((size_t)G + 1);
+    b.col_off = xalloc<uint64_t>((size_t)G + 1);
+    b.row_off[0] = b.col_off[0] = 0;
+    for (uint32_t i = 0; i < G; i++) {
+        b.V[i] = graphs[i].V;
+        b.E[i] = (uint32_t)graphs[i].s.size();
+        b.kind[i] = graphs[i].kind;
+        b.row_off[i + 1] = b.row_off[i] + graphs[i].V + 2;
+        b.col_off[i + 1] = b.col_off[i] + b.E[i];
+    }
+    b.Fr = xalloc<uint32_t>(b.row_off[G]);
+    b.Br = xalloc<uint32_t>(b.row_off[G]);
+    b.Fc = xalloc<uint32_t>(b.col_off[G]);
+    b.Bc = xalloc<uint32_t>(b.col_off[G]);
+    if (!b.Fr || !b.Br || !b.Fc || !b.Bc) {
+        scc_batch_host_free(&b);
+        return fail(SCC_ERR_NOMEM, "host allocation of the batch failed");
+    }
+    next = 0;
+    th.clear();
+    for (int t = 0; t < nt; t++)
+        th.emplace_back([&] {
+            for (uint32_t i; (i = next.fetch_add(1)) < G;) {
+                MixGraph &m = graphs[i];
+                csr_build_serial(m.V, m.s.size(), m.s.data(), m.d.data(), b.Fr + b.row_off[i],
+                                 b.Fc + b.col_off[i], b.Br + b.row_off[i], b.Bc + b.col_off[i]);
+                std::vector<uint32_t>().swap(m.s);
+                std::vector<uint32_t>().swap(m.d);
+            }
+        });
+    for (auto &x : th) x.join();
+    *out = b;
+    return SCC_OK;
 }
 
 scc_status scc_gen_communities(uint32_t V, uint32_t K, double deg, double inter, uint64_t seed,

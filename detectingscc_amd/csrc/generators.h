@@ -8,13 +8,23 @@ namespace sccamd {
 
 std::vector<uint32_t> seeded_permutation(uint32_t n, uint64_t seed);
 void er_arcs(uint32_t V, uint64_t draws, uint64_t seed, std::vector<uint32_t> &s,
-             std::vector<uint32_t> &d);
+             std::vector<uint32_t> &d, int threads = 0);
 void planted_cycles(uint32_t V, uint32_t cycles, uint32_t cmin, uint32_t cmax, uint64_t seed,
                     std::vector<uint32_t> &s, std::vector<uint32_t> &d);
 void rmat_arcs(uint32_t scale, uint32_t vcap, uint64_t E, double a, double b, double c,
                uint64_t seed, const std::vector<uint32_t> *perm, std::vector<uint32_t> &s,
-               std::vector<uint32_t> &d);
+               std::vector<uint32_t> &d, int threads = 0);
 void grid_arcs(uint32_t side, double p, uint64_t seed, std::vector<uint32_t> &s,
                std::vector<uint32_t> &d);
+void csr_build_serial(uint32_t V, uint64_t E, const uint32_t *src, const uint32_t *dst, uint32_t *Fr,
+                      uint32_t *Fc, uint32_t *Br, uint32_t *Bc);
+
+// One graph of the batched mix: kind 0 ER, 1 R-MAT, 2 planted cycles + ER.
+struct MixGraph {
+    uint32_t V;
+    uint8_t kind;
+    std::vector<uint32_t> s, d;
+};
+MixGraph mix_graph(uint64_t seed, uint32_t index, uint32_t lgmin, uint32_t lgmax, double dmin, double dmax);
 
 } // namespace sccamd

@@ -9,6 +9,7 @@
 #include <cstring>
 #include <map>
 
+#include "generators.h"
 #include "host_util.h"
 
 namespace sccamd {
@@ -135,6 +136,37 @@ static void stable_sort_pairs(uint64_t n, uint32_t K, const uint32_t *key, const
     for (uint64_t k = 1; k <= K; k++) rows[k] += rows[k - 1];
     std::free(tk);
     std::free(tv);
+}
+
+// Single-threaded CSR + reverse CSR in the reference layout, for callers
+// that already parallelise over many small graphs.  Counting sort by dst,
+// then a stable counting sort by src gives rows sorted by (src, dst); a
+// stable counting sort of that order by dst gives ascending in-lists.
+void csr_build_serial(uint32_t V, uint64_t E, const uint32_t *src, const uint32_t *dst, uint32_t *Fr,
+                      uint32_t *Fc, uint32_t *Br, uint32_t *Bc)
+{
+    std::vector<uint32_t> cnt((size_t)V + 2), s1(E), d1(E);
+    auto count_rows = [&](const uint32_t *key, uint32_t *rows) {
+        std::fill(cnt.begin(), cnt.end(), 0u);
+        for (uint64_t i = 0; i < E; i++) cnt[key[i] + 1]++;
+        for (uint32_t k = 1; k <= V + 1; k++) cnt[k] += cnt[k - 1];
+        if (rows) std::memcpy(rows, cnt.data(), sizeof(uint32_t) * ((size_t)V + 2));
+    };
+    count_rows(dst, nullptr);
+    for (uint64_t i = 0; i < E; i++) {
+        const uint32_t p = cnt[dst[i]]++;
+        s1[p] = src[i];
+        d1[p] = dst[i];
+    }
+    count_rows(s1.data(), Fr);
+    std::vector<uint32_t> s2(E);
+    for (uint64_t i = 0; i < E; i++) {
+        const uint32_t p = cnt[s1[i]]++;
+        s2[p] = s1[i];
+        Fc[p] = d1[i];
+    }
+    count_rows(Fc, Br);
+    for (uint64_t i = 0; i < E; i++) Bc[cnt[Fc[i]]++] = s2[i];
 }
 
 } // namespace sccamd

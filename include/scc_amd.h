@@ -211,6 +211,88 @@ scc_status scc_phase_max_color(scc_ctx *ctx, const uint8_t *dead, uint32_t *out)
  * partitions is one global SCC but two local ones) */
 scc_status scc_phase_local(scc_ctx *ctx, uint32_t *out);
 
+/* ----------------------------------------------------------- batches --- */
+/*
+ * Many independent graphs solved together (no reference counterpart: the
+ * reference solves one graph per process).  Host layout: graph i's rows
+ * Fr/Br (V[i]+2 entries, reference layout, local ids 1..V[i]) start at
+ * row_off[i]; its columns Fc/Bc (E[i] entries) start at col_off[i].
+ */
+typedef struct {
+    uint32_t num_graphs;
+    uint32_t *V, *E;
+    uint64_t *row_off, *col_off;  /* num_graphs+1 entries each */
+    uint32_t *Fr, *Fc, *Br, *Bc;
+    uint8_t *kind;                /* generator kind (0 ER, 1 R-MAT, 2 planted) or NULL */
+} scc_batch_host;
+
+void scc_batch_host_free(scc_batch_host *b);
+
+/* The batched mix: V log-uniform in [2^lgmin, 2^lgmax], mean degree uniform
+ * in [dmin, dmax], 40% ER / 40% R-MAT / 20% planted cycles (2..512) + ER. */
+scc_status scc_gen_batch_mix(uint32_t num_graphs, uint32_t lgmin, uint32_t lgmax, double dmin,
+                             double dmax, uint64_t seed, scc_batch_host *out);
+
+typedef struct {
+    char algo;       /* 'g' Hong phase order or 'h' Slota colouring */
+    int trim1, trim2;
+    int lanes;       /* lanes per frontier vertex, 1..64 */
+    int block;       /* workgroup size: 256, 512 or 1024 */
+    int autotuned;   /* 1: per size/degree bucket choices of scc_batch_autotune */
+} scc_batch_opts;
+
+typedef struct {
+    uint32_t graphs;
+    double ms;              /* batch start -> last graph finished */
+    double graphs_per_s;
+    double p50_ms, p99_ms;  /* per-graph time from batch start to its labels */
+    uint64_t vertices, arcs;
+    uint32_t launches;      /* kernel launches issued for the batch */
+} scc_batch_stats;
+
+/* Copy a batch to the device (stays resident in the context). */
+scc_status scc_batch_upload(scc_ctx *ctx, const scc_batch_host *b);
+/* All graphs in persistent launches (one per workgroup size in use). */
+scc_status scc_batch_run(scc_ctx *ctx, const scc_batch_opts *opts, scc_batch_stats *stats);
+/* Baseline: one graph at a time through the single-graph engine with its
+ * host-synchronised phase loop (the reference's style of driving). */
+scc_status scc_batch_run_serial(scc_ctx *ctx, const scc_opts *opts, scc_batch_stats *stats);
+/* Pick lanes, workgroup size and trim/colouring strategy per size and
+ * degree bucket by timing candidates on up to `sample` graphs per bucket;
+ * a text report of the choices goes to `report` (may be NULL). */
+scc_status scc_batch_autotune(scc_ctx *ctx, int sample, char *report, size_t report_len);
+/* Labels (row_off layout: V[i]+1 entries at row_off[i]) and SCC counts. */
+scc_status scc_batch_labels(scc_ctx *ctx, uint32_t *lab, uint32_t *nscc);
+
+/* ----------------------------------------------------------- serving --- */
+/*
+ * Continuous-batching request server (no reference counterpart).  Requests
+ * (graphs, copied at submit) are admitted first-come first-served while the
+ * device bytes of admitted requests stay within `hbm_budget`; each of
+ * `slots` worker threads packs up to `max_batch` admitted requests into one
+ * persistent batched launch on its own stream, so batches overlap on the
+ * GPU and requests that arrive meanwhile join the next free slot.  Latency
+ * = submit -> labels on the host.  A request larger than the whole budget is
+ * rejected with SCC_ERR_BUDGET.
+ */
+typedef struct {
+    uint64_t submitted, completed, rejected, failed, batches;
+    double p50_ms, p99_ms, mean_ms;   /* over completed requests */
+    double graphs_per_s;              /* completed / (last completion - first arrival) */
+    uint64_t peak_bytes;              /* largest admitted footprint */
+} scc_server_stats;
+
+scc_status scc_server_start(scc_ctx *ctx, uint64_t hbm_budget, uint32_t max_batch, uint32_t slots,
+                            const scc_batch_opts *opts);
+scc_status scc_server_submit(scc_ctx *ctx, const scc_host_graph *g, uint64_t *request_id);
+/* SCC_ERR_NOTREADY until done; then labels (V+1, may be NULL), SCC count
+ * and latency, and the request is released */
+scc_status scc_server_poll(scc_ctx *ctx, uint64_t request_id, uint32_t *lab, uint32_t *nscc,
+                           double *latency_ms);
+scc_status scc_server_wait_all(scc_ctx *ctx);
+scc_status scc_server_stats_get(scc_ctx *ctx, scc_server_stats *stats);
+scc_status scc_server_stop(scc_ctx *ctx);
+
 /* ------------------------------------------------------- measurement --- */
 /* A plain device copy of `bytes` (read + write), `iters` times, on the
  * context's stream; *ms_per_copy = mean duration.  Calibrates the HBM byte
