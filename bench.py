@@ -13,6 +13,14 @@ Printed as ONE JSON line:
                library's stream), against 8 TB/s HBM
   cpu_baseline the CPU restatement of vHong (oracle port), all host cores,
                on a bounded sample (R-MAT scale 20)
+  batched      the 4096-graph mix (V log-uniform in [2^8, 2^16], mean degree
+               uniform in [2, 16], 40% ER / 40% R-MAT / 20% planted cycles):
+               autotuned persistent launches vs one graph per launch through
+               the single-graph engine, graphs/s and per-graph p50/p99
+  serving      Poisson arrivals of lognormal requests (median 2^14 V, sigma
+               1.5, capped at 2^22, mean degree 8, ER and R-MAT) into the
+               continuous-batching server under an HBM budget: graphs/s and
+               p50/p99 latency (submit -> labels on the host)
 """
 from __future__ import annotations
 
@@ -21,6 +29,8 @@ import json
 import os
 import sys
 import time
+
+import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
@@ -83,6 +93,76 @@ def cpu_baseline(sample_scale: int, threads: int):
                       f"median of {len(runs)} runs"}
 
 
+def bench_batched(graphs=4096, steps=3, sample=1024):
+    import detectingscc_amd as scc
+    t0 = time.perf_counter()
+    b = scc.gen_batch_mix(graphs, 8, 16, 2.0, 16.0, seed=4096)
+    log(f"built the {graphs}-graph mix: {int(b.V.sum())} V, {int(b.E.sum())} arcs "
+        f"in {time.perf_counter() - t0:.1f}s")
+    with scc.Context(0) as ctx:
+        ctx.batch_upload(b)
+        report = ctx.batch_autotune(sample=sample)
+        ctx.batch_run(autotuned=True)                        # warm-up
+        runs = [ctx.batch_run(autotuned=True) for _ in range(steps)]
+        best = min(runs, key=lambda r: r.ms)
+        labs, nscc = ctx.batch_labels()
+        serial = ctx.batch_run_serial(expand=4)              # one graph per launch, DO expansion
+        slabs, snscc = ctx.batch_labels()
+    same = bool(np.array_equal(nscc, snscc)) and all(np.array_equal(x, y) for x, y in zip(labs, slabs))
+    log(f"batched {best.ms:.2f} ms ({best.graphs_per_s:.0f} graphs/s), one-graph-per-launch "
+        f"{serial.ms:.0f} ms ({serial.graphs_per_s:.0f} graphs/s), labels identical: {same}")
+    return {
+        "workload": f"{graphs} graphs, V log-uniform [2^8, 2^16], degree [2, 16], 40% ER / 40% R-MAT / "
+                    "20% planted cycles 2..512, seed 4096",
+        "vertices": int(b.V.sum()), "arcs": int(b.E.sum()),
+        "graphs_per_s": best.graphs_per_s, "ms": best.ms, "p50_ms": best.p50_ms, "p99_ms": best.p99_ms,
+        "launches": best.launches, "steps": steps,
+        "serial_graphs_per_s": serial.graphs_per_s, "serial_ms": serial.ms, "serial_p99_ms": serial.p99_ms,
+        "speedup_vs_one_graph_per_launch": best.graphs_per_s / serial.graphs_per_s,
+        "labels_identical": same,
+        "autotune": report.strip().splitlines(),
+    }
+
+
+def bench_serving(requests=400, rate=1000.0, budget_gib=32, slots=3):
+    import detectingscc_amd as scc
+    t0 = time.perf_counter()
+    b = scc.gen_serving_mix(requests, 16384, 1.5, 1 << 22, 8.0, seed=8)
+    graphs = [b.graph(i) for i in range(requests)]
+    log(f"built {requests} requests: {int(b.V.sum())} V, {int(b.E.sum())} arcs, largest {int(b.V.max())} V "
+        f"in {time.perf_counter() - t0:.1f}s")
+    gaps = np.random.default_rng(1).exponential(1.0 / rate, size=requests)
+    with scc.Context(0) as c:
+        c.server_start(budget_bytes=budget_gib << 30, max_batch=256, slots=slots, solo_vertices=1 << 16,
+                       lanes=16, block=1024)
+        t = time.perf_counter()
+        due = t
+        ids = []
+        for g, gap in zip(graphs, gaps):
+            due += gap
+            while time.perf_counter() < due:
+                pass
+            ids.append(c.server_submit(g))
+        c.server_wait_all()
+        wall = time.perf_counter() - t
+        st = c.server_stats()
+        for rid in ids:
+            c.server_poll(rid)
+        c.server_stop()
+    log(f"served {st.completed} requests in {wall:.2f}s: p50 {st.p50_ms:.2f} ms, p99 {st.p99_ms:.2f} ms")
+    return {
+        "workload": f"{requests} requests, Poisson {rate:g}/s, V lognormal median 16384 sigma 1.5 cap 2^22, "
+                    "degree 8, 50% ER / 50% R-MAT, seed 8",
+        "arrival_rate_per_s": rate, "hbm_budget_bytes": budget_gib << 30, "slots": slots,
+        "graphs_per_s": st.graphs_per_s, "p50_ms": st.p50_ms, "p99_ms": st.p99_ms, "mean_ms": st.mean_ms,
+        "completed": st.completed, "batches": st.batches, "solo": st.solo, "peak_admitted_bytes": st.peak_bytes,
+    }
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -94,6 +174,9 @@ def main():
     ap.add_argument("--expand", type=int, default=4, help="0 auto, 1 vertex, 2 vwarp, 3 LB, 4 DO")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-scale", type=int, default=20)
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched-mix measurement")
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving measurement")
+    ap.add_argument("--serve-rate", type=float, default=1000.0)
     ap.add_argument("--copy-calibrate", type=int, default=0, metavar="BYTES",
                     help="only run a plain device copy of BYTES (HBM counter calibration)")
     args = ap.parse_args()
@@ -132,7 +215,7 @@ def main():
     achieved = alg / (fr_ms * 1e-3) / 1e9 if fr_ms > 0 else 0.0
     phase_ms = {k: round(sum(s.ms_phase[k] for s in stats) / len(stats), 4) for k in scc.PHASES}
     out = {
-        "metric": "SCC edges/s (device-resident CSR -> canonical labels)",
+        "metric": "SCC edges/s & %HBM peak; batched graphs/s and p99 vs one-graph-per-launch",
         "value": g.E * args.steps / wall,
         "unit": "edges/s",
         "n_gpus": 1,
@@ -157,6 +240,9 @@ def main():
         "device_ms_per_step": sum(s.ms_scc for s in stats) / len(stats),
     }
     ctx.close()
+    del g
+    out["batched"] = None if args.no_batch else bench_batched()
+    out["serving"] = None if args.no_serve else bench_serving(rate=args.serve_rate)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args.cpu_scale, 0)
     else:

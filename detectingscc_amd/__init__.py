@@ -84,6 +84,7 @@ class _BatchStats(ctypes.Structure):
 class _ServerStats(ctypes.Structure):
     _fields_ = [("submitted", ctypes.c_uint64), ("completed", ctypes.c_uint64),
                 ("rejected", ctypes.c_uint64), ("failed", ctypes.c_uint64), ("batches", ctypes.c_uint64),
+                ("solo", ctypes.c_uint64),
                 ("p50_ms", ctypes.c_double), ("p99_ms", ctypes.c_double), ("mean_ms", ctypes.c_double),
                 ("graphs_per_s", ctypes.c_double), ("peak_bytes", ctypes.c_uint64)]
 
@@ -106,7 +107,9 @@ def lib() -> ctypes.CDLL:
                                         ctypes.c_double, ctypes.c_uint64, ctypes.POINTER(_BatchHost)]
         L.scc_batch_autotune.argtypes = [vp, ctypes.c_int, ctypes.c_char_p, ctypes.c_size_t]
         L.scc_server_start.argtypes = [vp, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32,
-                                       ctypes.POINTER(_BatchOpts)]
+                                       ctypes.c_uint32, ctypes.POINTER(_BatchOpts)]
+        L.scc_gen_serving_mix.argtypes = [ctypes.c_uint32, ctypes.c_double, ctypes.c_double, ctypes.c_uint32,
+                                          ctypes.c_double, ctypes.c_uint64, ctypes.POINTER(_BatchHost)]
         L.scc_server_submit.argtypes = [vp, ctypes.POINTER(_HostGraph), ctypes.POINTER(ctypes.c_uint64)]
         L.scc_server_poll.argtypes = [vp, ctypes.c_uint64, _u32p, _u32p, ctypes.POINTER(ctypes.c_double)]
         _LIB = L
@@ -321,12 +324,7 @@ def batch_from_graphs(graphs: list[Graph]) -> Batch:
                  cat([g.Br for g in graphs]), cat([g.Bc for g in graphs]))
 
 
-def gen_batch_mix(num_graphs: int, lgmin: int = 8, lgmax: int = 16, dmin: float = 2.0,
-                  dmax: float = 16.0, seed: int = 4096) -> Batch:
-    """The batched mix: V log-uniform in [2^lgmin, 2^lgmax], mean degree
-    uniform in [dmin, dmax], 40% ER / 40% R-MAT / 20% planted cycles + ER."""
-    h = _BatchHost()
-    _check(lib().scc_gen_batch_mix(num_graphs, lgmin, lgmax, dmin, dmax, seed, ctypes.byref(h)))
+def _take_batch(h: _BatchHost) -> Batch:
     G = int(h.num_graphs)
     def arr(p, n, dt):
         return np.ctypeslib.as_array(p, shape=(max(n, 1),))[:n].astype(dt, copy=True)
@@ -338,6 +336,24 @@ def gen_batch_mix(num_graphs: int, lgmin: int = 8, lgmax: int = 16, dmin: float 
               arr(h.kind, G, np.uint8) if h.kind else None)
     lib().scc_batch_host_free(ctypes.byref(h))
     return b
+
+
+def gen_batch_mix(num_graphs: int, lgmin: int = 8, lgmax: int = 16, dmin: float = 2.0,
+                  dmax: float = 16.0, seed: int = 4096) -> Batch:
+    """The batched mix: V log-uniform in [2^lgmin, 2^lgmax], mean degree
+    uniform in [dmin, dmax], 40% ER / 40% R-MAT / 20% planted cycles + ER."""
+    h = _BatchHost()
+    _check(lib().scc_gen_batch_mix(num_graphs, lgmin, lgmax, dmin, dmax, seed, ctypes.byref(h)))
+    return _take_batch(h)
+
+
+def gen_serving_mix(num_graphs: int, median_v: float = 16384, sigma: float = 1.5, vcap: int = 1 << 22,
+                    degree: float = 8.0, seed: int = 8) -> Batch:
+    """Serving requests: V lognormal (median, sigma) clamped to [2, vcap],
+    mean degree `degree`, half ER and half R-MAT."""
+    h = _BatchHost()
+    _check(lib().scc_gen_serving_mix(num_graphs, median_v, sigma, vcap, degree, seed, ctypes.byref(h)))
+    return _take_batch(h)
 
 
 @dataclass
@@ -359,6 +375,7 @@ class ServerStats:
     rejected: int
     failed: int
     batches: int
+    solo: int
     p50_ms: float
     p99_ms: float
     mean_ms: float
@@ -472,10 +489,11 @@ class Context:
         return [lab[int(b.row_off[i]):int(b.row_off[i]) + int(b.V[i]) + 1] for i in range(b.num_graphs)], nscc
 
     # -- continuous-batching server --
-    def server_start(self, budget_bytes: int, max_batch: int = 256, slots: int = 2, algo: str = "g",
-                     trim1: bool = True, trim2: bool = True, lanes: int = 4, block: int = 256) -> None:
+    def server_start(self, budget_bytes: int, max_batch: int = 256, slots: int = 2, solo_vertices: int = 0,
+                     algo: str = "g", trim1: bool = True, trim2: bool = True, lanes: int = 4,
+                     block: int = 256) -> None:
         o = make_batch_opts(algo, trim1, trim2, lanes, block, False)
-        _check(lib().scc_server_start(self._ctx, budget_bytes, max_batch, slots, ctypes.byref(o)))
+        _check(lib().scc_server_start(self._ctx, budget_bytes, max_batch, slots, solo_vertices, ctypes.byref(o)))
         self._served = {}
 
     def server_submit(self, g: Graph) -> int:
@@ -503,7 +521,7 @@ class Context:
     def server_stats(self) -> ServerStats:
         s = _ServerStats()
         _check(lib().scc_server_stats_get(self._ctx, ctypes.byref(s)))
-        return ServerStats(s.submitted, s.completed, s.rejected, s.failed, s.batches, s.p50_ms, s.p99_ms,
+        return ServerStats(s.submitted, s.completed, s.rejected, s.failed, s.batches, s.solo, s.p50_ms, s.p99_ms,
                            s.mean_ms, s.graphs_per_s, s.peak_bytes)
 
     def server_stop(self) -> None:

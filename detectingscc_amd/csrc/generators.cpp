@@ -2,8 +2,11 @@
 // batched mix and the serving stream.  Every random choice is a pure
 // function rnd(seed, stream, index), so results do not depend on the
 // number of host threads.
+#include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstring>
+#include <thread>
 #include <unordered_set>
 
 #include "generators.h"
@@ -140,6 +143,29 @@ MixGraph mix_graph(uint64_t seed, uint32_t index, uint32_t lgmin, uint32_t lgmax
     return m;
 }
 
+MixGraph serving_graph(uint64_t seed, uint32_t index, double median_v, double sigma, uint32_t vcap,
+                       double degree)
+{
+    MixGraph m;
+    const uint64_t gs = mix64(seed * 0xD1B54A32D192ED03ull + index + 1);
+    // lognormal V: standard normal by Box-Muller from two uniforms in (0, 1]
+    const double u1 = 1.0 - unit(rnd(gs, 41, 0)), u2 = unit(rnd(gs, 41, 1));
+    const double z = std::sqrt(-2.0 * std::log(u1)) * std::cos(6.283185307179586 * u2);
+    const double v = (double)std::llround(median_v * std::exp(sigma * z));
+    m.V = (uint32_t)std::min<double>(vcap, std::max<double>(2.0, v));
+    const uint64_t E = (uint64_t)std::llround(degree * m.V);
+    m.kind = unit(rnd(gs, 41, 2)) < 0.5 ? 0 : 1;
+    if (m.kind == 0) {
+        er_arcs(m.V, E, gs, m.s, m.d, 1);
+    } else {
+        uint32_t scale = 1;
+        while ((1ull << scale) < m.V) scale++;
+        std::vector<uint32_t> perm = seeded_permutation(m.V, gs);
+        rmat_arcs(scale, m.V, E, 0.57, 0.19, 0.19, gs, &perm, m.s, m.d, 1);
+    }
+    return m;
+}
+
 void grid_arcs(uint32_t side, double p, uint64_t seed, std::vector<uint32_t> &s,
                std::vector<uint32_t> &d)
 {
@@ -232,21 +258,19 @@ void scc_batch_host_free(scc_batch_host *b)
     std::memset(b, 0, sizeof *b);
 }
 
-scc_status scc_gen_batch_mix(uint32_t num_graphs, uint32_t lgmin, uint32_t lgmax, double dmin,
-                             double dmax, uint64_t seed, scc_batch_host *out)
+} // extern "C"
+
+// Lays out G generated graphs (make(i) -> MixGraph) as one host batch: graphs
+// are generated in parallel, then each is sorted into its CSR slice.
+template <class Make> static scc_status assemble_batch(uint32_t G, Make make, scc_batch_host *out)
 {
-    clear_error();
-    if (!out || !num_graphs || lgmin < 1 || lgmax < lgmin || lgmax > 24 || dmin < 0 || dmax < dmin)
-        return fail(SCC_ERR_INVALID, "bad batch-mix arguments");
-    const uint32_t G = num_graphs;
-    // pass 1: sizes (cheap, deterministic) so the arrays can be laid out
     std::vector<MixGraph> graphs(G);
     std::atomic<uint32_t> next{0};
     std::vector<std::thread> th;
     const int nt = host_threads();
     for (int t = 0; t < nt; t++)
         th.emplace_back([&] {
-            for (uint32_t i; (i = next.fetch_add(1)) < G;) graphs[i] = mix_graph(seed, i, lgmin, lgmax, dmin, dmax);
+            for (uint32_t i; (i = next.fetch_add(1)) < G;) graphs[i] = make(i);
         });
     for (auto &x : th) x.join();
     scc_batch_host b;
@@ -257,8 +281,16 @@ scc_status scc_gen_batch_mix(uint32_t num_graphs, uint32_t lgmin, uint32_t lgmax
     b.kind = xalloc<uint8_t>(G);
     b.row_off = xalloc<uint64_t>((size_t)G + 1);
     b.col_off = xalloc<uint64_t>((size_t)G + 1);
+    if (!b.V || !b.E || !b.kind || !b.row_off || !b.col_off) {
+        scc_batch_host_free(&b);
+        return fail(SCC_ERR_NOMEM, "host allocation of the batch failed");
+    }
     b.row_off[0] = b.col_off[0] = 0;
     for (uint32_t i = 0; i < G; i++) {
+        if (graphs[i].s.size() > 0xFFFFFFFFull) {
+            scc_batch_host_free(&b);
+            return fail(SCC_ERR_INVALID, "graph %u has too many arcs", i);
+        }
         b.V[i] = graphs[i].V;
         b.E[i] = (uint32_t)graphs[i].s.size();
         b.kind[i] = graphs[i].kind;
@@ -288,6 +320,28 @@ scc_status scc_gen_batch_mix(uint32_t num_graphs, uint32_t lgmin, uint32_t lgmax
     for (auto &x : th) x.join();
     *out = b;
     return SCC_OK;
+}
+
+extern "C" {
+
+scc_status scc_gen_batch_mix(uint32_t num_graphs, uint32_t lgmin, uint32_t lgmax, double dmin,
+                             double dmax, uint64_t seed, scc_batch_host *out)
+{
+    clear_error();
+    if (!out || !num_graphs || lgmin < 1 || lgmax < lgmin || lgmax > 24 || dmin < 0 || dmax < dmin)
+        return fail(SCC_ERR_INVALID, "bad batch-mix arguments");
+    return assemble_batch(
+        num_graphs, [&](uint32_t i) { return mix_graph(seed, i, lgmin, lgmax, dmin, dmax); }, out);
+}
+
+scc_status scc_gen_serving_mix(uint32_t num_graphs, double median_v, double sigma, uint32_t vcap,
+                               double degree, uint64_t seed, scc_batch_host *out)
+{
+    clear_error();
+    if (!out || !num_graphs || median_v < 2 || sigma < 0 || vcap < 2 || vcap >= (1u << 30) || degree < 0)
+        return fail(SCC_ERR_INVALID, "bad serving-mix arguments");
+    return assemble_batch(
+        num_graphs, [&](uint32_t i) { return serving_graph(seed, i, median_v, sigma, vcap, degree); }, out);
 }
 
 scc_status scc_gen_communities(uint32_t V, uint32_t K, double deg, double inter, uint64_t seed,

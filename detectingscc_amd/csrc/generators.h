@@ -26,5 +26,9 @@ struct MixGraph {
     std::vector<uint32_t> s, d;
 };
 MixGraph mix_graph(uint64_t seed, uint32_t index, uint32_t lgmin, uint32_t lgmax, double dmin, double dmax);
+// One request of the serving stream: V lognormal (median, sigma) clamped
+// to [2, vcap], mean degree `degree`, kind 0 ER or 1 R-MAT (even odds).
+MixGraph serving_graph(uint64_t seed, uint32_t index, double median_v, double sigma, uint32_t vcap,
+                       double degree);
 
 } // namespace sccamd

@@ -28,9 +28,18 @@ uint64_t Server::footprint(uint32_t V, uint32_t E)
     return S * 64 + S / 4 + 8ull * E + 48;
 }
 
+uint64_t Server::solo_footprint(uint32_t V, uint32_t E)
+{
+    // the device-wide engine: rows, state, pivot keys, two queue pairs per
+    // direction, hub lists and offsets, three bitmaps and byte flags per
+    // vertex (under 128 B), 8 B per arc, counters
+    return 128ull * ((uint64_t)V + 2) + 8ull * E + 4096;
+}
+
 Server::~Server() { stop(); }
 
-scc_status Server::start(uint64_t budget_bytes, uint32_t mb, uint32_t slots, const scc_batch_opts &o)
+scc_status Server::start(uint64_t budget_bytes, uint32_t mb, uint32_t slots, uint32_t solo_v,
+                         const scc_batch_opts &o)
 {
     if (running) return fail(SCC_ERR_INVALID, "server already running");
     if (!budget_bytes || !mb || !slots || slots > 8)
@@ -42,10 +51,17 @@ scc_status Server::start(uint64_t budget_bytes, uint32_t mb, uint32_t slots, con
         return fail(SCC_ERR_INVALID, "lanes %d not a power of two in [1, 64]", o.lanes);
     if (o.algo != 'g' && o.algo != 'h') return fail(SCC_ERR_INVALID, "batch algorithm must be 'g' or 'h'");
     opts = o;
+    scc_opts_default(&solo_opts);
+    solo_opts.algo = o.algo;
+    solo_opts.trim1 = o.trim1;
+    solo_opts.trim2 = o.trim2;
+    solo_opts.expand = o.algo == 'g' ? SCC_EXPAND_DO : SCC_EXPAND_LB;
+    solo_opts.sizes = 0;
+    solo_vertices = solo_v ? solo_v : (1u << 16);
     budget = budget_bytes;
     max_batch = mb;
     inflight = peak = 0;
-    submitted = completed = rejected = batches = failed = 0;
+    submitted = completed = rejected = batches = failed = solo_runs = 0;
     latencies.clear();
     busy_ms = 0;
     stopping = false;
@@ -60,7 +76,8 @@ scc_status Server::submit(const scc_host_graph *g, uint64_t *id)
     if (!g || !g->Fr || !g->Br || (g->E && (!g->Fc || !g->Bc)))
         return fail(SCC_ERR_INVALID, "incomplete graph");
     if (g->V >= (1u << 30)) return fail(SCC_ERR_INVALID, "V too large for a batch slot");
-    const uint64_t bytes = footprint(g->V, g->E);
+    const bool solo = g->V >= solo_vertices;
+    const uint64_t bytes = solo ? solo_footprint(g->V, g->E) : footprint(g->V, g->E);
     if (bytes > budget) {
         std::lock_guard<std::mutex> lk(mu);
         rejected++;
@@ -71,6 +88,7 @@ scc_status Server::submit(const scc_host_graph *g, uint64_t *id)
     r->V = g->V;
     r->E = g->E;
     r->bytes = bytes;
+    r->solo = solo;
     r->Fr.assign(g->Fr, g->Fr + (size_t)g->V + 2);
     r->Br.assign(g->Br, g->Br + (size_t)g->V + 2);
     r->Fc.assign(g->Fc, g->Fc + g->E);
@@ -106,12 +124,13 @@ void Server::worker(int slot)
     (void)slot;
     hipStream_t stream = nullptr;
     std::unique_ptr<BatchEngine> be(new BatchEngine());
+    std::unique_ptr<Engine> solo_eng;
     std::string init_err;
     if (hipSetDevice(dev) != hipSuccess || hipStreamCreateWithFlags(&stream, hipStreamNonBlocking) != hipSuccess)
         init_err = "cannot create a stream for a server slot";
     std::vector<Req *> batch;
     scc_batch_host h;
-    std::vector<uint32_t> hV, hE, Fr, Br, Fc, Bc, nscc;
+    std::vector<uint32_t> hV, hE, Fr, Br, Fc, Bc, nscc, lab;
     std::vector<uint64_t> ro, co;
     for (;;) {
         uint64_t taken = 0;
@@ -121,55 +140,73 @@ void Server::worker(int slot)
             // on stop the queue is drained first
             cv_work.wait(lk, [&] { return (stopping && pending.empty()) || can_admit_locked(); });
             if (pending.empty()) break;
+            // a large request runs alone on the device-wide engine; small ones
+            // are packed up to the next large one
             while (!pending.empty() && batch.size() < max_batch) {
                 Req *r = reqs.at(pending.front()).get();
                 if (inflight + taken + r->bytes > budget) break;
+                if (r->solo && !batch.empty()) break;
                 taken += r->bytes;
                 batch.push_back(r);
                 pending.pop_front();
+                if (r->solo) break;
             }
             inflight += taken;
             peak = std::max(peak, inflight);
         }
-        // pack the batch in the reference layout, one graph after another
         const uint32_t G = (uint32_t)batch.size();
-        hV.resize(G);
-        hE.resize(G);
         ro.assign(G + 1, 0);
-        co.assign(G + 1, 0);
-        for (uint32_t i = 0; i < G; i++) {
-            hV[i] = batch[i]->V;
-            hE[i] = batch[i]->E;
-            ro[i + 1] = ro[i] + batch[i]->V + 2;
-            co[i + 1] = co[i] + batch[i]->E;
-        }
-        Fr.resize(ro[G]);
-        Br.resize(ro[G]);
-        Fc.resize(co[G]);
-        Bc.resize(co[G]);
-        for (uint32_t i = 0; i < G; i++) {
-            std::copy(batch[i]->Fr.begin(), batch[i]->Fr.end(), Fr.begin() + ro[i]);
-            std::copy(batch[i]->Br.begin(), batch[i]->Br.end(), Br.begin() + ro[i]);
-            std::copy(batch[i]->Fc.begin(), batch[i]->Fc.end(), Fc.begin() + co[i]);
-            std::copy(batch[i]->Bc.begin(), batch[i]->Bc.end(), Bc.begin() + co[i]);
-        }
-        std::memset(&h, 0, sizeof h);
-        h.num_graphs = G;
-        h.V = hV.data();
-        h.E = hE.data();
-        h.row_off = ro.data();
-        h.col_off = co.data();
-        h.Fr = Fr.data();
-        h.Br = Br.data();
-        h.Fc = Fc.data();
-        h.Bc = Bc.data();
-        std::vector<uint32_t> lab(ro[G]);
+        for (uint32_t i = 0; i < G; i++) ro[i + 1] = ro[i] + batch[i]->V + 2;
+        lab.resize(ro[G]);
         nscc.resize(G);
-        scc_batch_stats bst;
         scc_status st = init_err.empty() ? SCC_OK : fail(SCC_ERR_DEVICE, "%s", init_err.c_str());
-        if (st == SCC_OK) st = be->upload(dev, stream, &h);
-        if (st == SCC_OK) st = be->run(opts, &bst);
-        if (st == SCC_OK) st = be->labels(lab.data(), nscc.data());
+        if (st == SCC_OK && batch[0]->solo) {
+            Req *r = batch[0];
+            if (!solo_eng) {
+                solo_eng.reset(new Engine());
+                st = solo_eng->init(dev);
+            }
+            if (st == SCC_OK)
+                st = solo_eng->upload(r->V, r->E, r->Fr.data(), r->Fc.data(), r->Br.data(), r->Bc.data());
+            scc_stats sst;
+            if (st == SCC_OK) st = solo_eng->run(solo_opts, &sst);
+            if (st == SCC_OK) st = solo_eng->labels(lab.data());
+            nscc[0] = sst.num_sccs;
+        } else if (st == SCC_OK) {
+            // pack the batch in the reference layout, one graph after another
+            hV.resize(G);
+            hE.resize(G);
+            co.assign(G + 1, 0);
+            for (uint32_t i = 0; i < G; i++) {
+                hV[i] = batch[i]->V;
+                hE[i] = batch[i]->E;
+                co[i + 1] = co[i] + batch[i]->E;
+            }
+            Fr.resize(ro[G]);
+            Br.resize(ro[G]);
+            Fc.resize(co[G]);
+            Bc.resize(co[G]);
+            for (uint32_t i = 0; i < G; i++) {
+                std::copy(batch[i]->Fr.begin(), batch[i]->Fr.end(), Fr.begin() + ro[i]);
+                std::copy(batch[i]->Br.begin(), batch[i]->Br.end(), Br.begin() + ro[i]);
+                std::copy(batch[i]->Fc.begin(), batch[i]->Fc.end(), Fc.begin() + co[i]);
+                std::copy(batch[i]->Bc.begin(), batch[i]->Bc.end(), Bc.begin() + co[i]);
+            }
+            std::memset(&h, 0, sizeof h);
+            h.num_graphs = G;
+            h.V = hV.data();
+            h.E = hE.data();
+            h.row_off = ro.data();
+            h.col_off = co.data();
+            h.Fr = Fr.data();
+            h.Br = Br.data();
+            h.Fc = Fc.data();
+            h.Bc = Bc.data();
+            scc_batch_stats bst;
+            st = be->upload(dev, stream, &h);
+            if (st == SCC_OK) st = be->run(opts, &bst);
+            if (st == SCC_OK) st = be->labels(lab.data(), nscc.data());
+        }
         const std::string err = st == SCC_OK ? std::string() : std::string(scc_last_error());
         const auto now = clock::now();
         {
@@ -179,14 +216,10 @@ void Server::worker(int slot)
                 r->done = true;
                 r->done_at = now;
                 r->status = st;
-                r->Fr.clear();
-                r->Fr.shrink_to_fit();
-                r->Br.clear();
-                r->Br.shrink_to_fit();
-                r->Fc.clear();
-                r->Fc.shrink_to_fit();
-                r->Bc.clear();
-                r->Bc.shrink_to_fit();
+                std::vector<uint32_t>().swap(r->Fr);
+                std::vector<uint32_t>().swap(r->Br);
+                std::vector<uint32_t>().swap(r->Fc);
+                std::vector<uint32_t>().swap(r->Bc);
                 if (st == SCC_OK) {
                     r->lab.assign(lab.begin() + ro[i], lab.begin() + ro[i] + r->V + 1);
                     r->nscc = nscc[i];
@@ -198,14 +231,15 @@ void Server::worker(int slot)
                 }
             }
             last_done = now;
-            if (st == SCC_OK) busy_ms += bst.ms;
             batches++;
+            if (G == 1 && batch[0]->solo) solo_runs++;
             inflight -= taken;
         }
         cv_done.notify_all();
         cv_work.notify_all();
     }
     be.reset();
+    solo_eng.reset();
     if (stream) (void)hipStreamDestroy(stream);
 }
 
@@ -246,6 +280,7 @@ scc_status Server::stats(scc_server_stats *s)
     s->rejected = rejected;
     s->failed = failed;
     s->batches = batches;
+    s->solo = solo_runs;
     s->peak_bytes = peak;
     if (!latencies.empty()) {
         std::vector<double> v = latencies;
@@ -288,12 +323,12 @@ using namespace sccamd;
 extern "C" {
 
 scc_status scc_server_start(scc_ctx *ctx, uint64_t hbm_budget, uint32_t max_batch, uint32_t slots,
-                            const scc_batch_opts *opts)
+                            uint32_t solo_vertices, const scc_batch_opts *opts)
 {
     clear_error();
     if (!ctx || !opts) return fail(SCC_ERR_INVALID, "null argument");
     if (!ctx->server) ctx->server.reset(new Server(ctx->eng.device));
-    return ctx->server->start(hbm_budget, max_batch, slots, *opts);
+    return ctx->server->start(hbm_budget, max_batch, slots, solo_vertices, *opts);
 }
 
 scc_status scc_server_submit(scc_ctx *ctx, const scc_host_graph *g, uint64_t *request_id)

@@ -232,6 +232,10 @@ void scc_batch_host_free(scc_batch_host *b);
  * in [dmin, dmax], 40% ER / 40% R-MAT / 20% planted cycles (2..512) + ER. */
 scc_status scc_gen_batch_mix(uint32_t num_graphs, uint32_t lgmin, uint32_t lgmax, double dmin,
                              double dmax, uint64_t seed, scc_batch_host *out);
+/* The serving stream: V lognormal with the given median and sigma, clamped
+ * to [2, vcap], mean out-degree `degree`, 50% ER / 50% R-MAT. */
+scc_status scc_gen_serving_mix(uint32_t num_graphs, double median_v, double sigma, uint32_t vcap,
+                               double degree, uint64_t seed, scc_batch_host *out);
 
 typedef struct {
     char algo;       /* 'g' Hong phase order or 'h' Slota colouring */
@@ -258,8 +262,9 @@ scc_status scc_batch_run(scc_ctx *ctx, const scc_batch_opts *opts, scc_batch_sta
  * host-synchronised phase loop (the reference's style of driving). */
 scc_status scc_batch_run_serial(scc_ctx *ctx, const scc_opts *opts, scc_batch_stats *stats);
 /* Pick lanes, workgroup size and trim/colouring strategy per size and
- * degree bucket by timing candidates on up to `sample` graphs per bucket;
- * a text report of the choices goes to `report` (may be NULL). */
+ * degree bucket by timing each candidate on up to `sample` graphs of the
+ * bucket in one launch; a text report of the choices goes to `report`
+ * (may be NULL).  Labels of the uploaded batch are overwritten. */
 scc_status scc_batch_autotune(scc_ctx *ctx, int sample, char *report, size_t report_len);
 /* Labels (row_off layout: V[i]+1 entries at row_off[i]) and SCC counts. */
 scc_status scc_batch_labels(scc_ctx *ctx, uint32_t *lab, uint32_t *nscc);
@@ -271,19 +276,22 @@ scc_status scc_batch_labels(scc_ctx *ctx, uint32_t *lab, uint32_t *nscc);
  * device bytes of admitted requests stay within `hbm_budget`; each of
  * `slots` worker threads packs up to `max_batch` admitted requests into one
  * persistent batched launch on its own stream, so batches overlap on the
- * GPU and requests that arrive meanwhile join the next free slot.  Latency
- * = submit -> labels on the host.  A request larger than the whole budget is
- * rejected with SCC_ERR_BUDGET.
+ * GPU and requests that arrive meanwhile join the next free slot.  A
+ * request with V >= `solo_vertices` (0: 65536) runs alone on that slot's
+ * device-wide engine instead of inside one workgroup.  Latency = submit ->
+ * labels on the host.  A request larger than the whole budget is rejected
+ * with SCC_ERR_BUDGET.
  */
 typedef struct {
     uint64_t submitted, completed, rejected, failed, batches;
+    uint64_t solo;                    /* requests run on the device-wide engine */
     double p50_ms, p99_ms, mean_ms;   /* over completed requests */
     double graphs_per_s;              /* completed / (last completion - first arrival) */
     uint64_t peak_bytes;              /* largest admitted footprint */
 } scc_server_stats;
 
 scc_status scc_server_start(scc_ctx *ctx, uint64_t hbm_budget, uint32_t max_batch, uint32_t slots,
-                            const scc_batch_opts *opts);
+                            uint32_t solo_vertices, const scc_batch_opts *opts);
 scc_status scc_server_submit(scc_ctx *ctx, const scc_host_graph *g, uint64_t *request_id);
 /* SCC_ERR_NOTREADY until done; then labels (V+1, may be NULL), SCC count
  * and latency, and the request is released */

@@ -124,12 +124,13 @@ def test_batch_repeatable(ctx, mix):
 
 def test_server_each_alone_equals_mix():
     """Requests arriving at random times, batched with whatever else is
-    queued, get the labels the same graph gets when solved alone."""
+    queued (or run alone on the device-wide engine when V >= 3000), get the
+    labels the same graph gets when solved alone."""
     b = scc.gen_batch_mix(160, 6, 12, 1.5, 12.0, seed=99)
     graphs = [b.graph(i) for i in range(b.num_graphs)] + _edge_graphs()[:8]
     rng = np.random.default_rng(3)
     with scc.Context(0) as c, scc.Context(0) as alone:
-        c.server_start(budget_bytes=64 << 20, max_batch=32, slots=2, lanes=4, block=256)
+        c.server_start(budget_bytes=64 << 20, max_batch=32, slots=2, solo_vertices=3000, lanes=4, block=256)
         ids = []
         for g in graphs:
             ids.append(c.server_submit(g))
@@ -146,6 +147,7 @@ def test_server_each_alone_equals_mix():
             assert n == wn == st.num_sccs and ms > 0
         s = c.server_stats()
         assert s.completed == len(graphs) and s.failed == 0 and s.batches >= 2
+        assert s.solo == sum(g.V >= 3000 for g in graphs) > 0
         assert s.peak_bytes <= 64 << 20 and s.p99_ms >= s.p50_ms > 0
         c.server_stop()
 

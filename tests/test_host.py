@@ -226,3 +226,53 @@ def test_cli_missing_file():
 def test_cli_detect_needs_partition(fixture_path):
     r = _cli("-a", "d", "-p", "1", "-q", "1", "-w", "1", "-f", fixture_path("sample0.gr"))
     assert r.returncode == 1 and "Error Reading metis graph file" in r.stdout
+
+
+def _check_batch_layout(b):
+    assert b.row_off[0] == 0 and b.col_off[0] == 0
+    for i in range(b.num_graphs):
+        g = b.graph(i)
+        assert g.Fr[0] == 0 and g.Fr[1] == 0 and g.Fr[-1] == g.E == b.E[i]
+        assert np.all(np.diff(g.Fr.astype(np.int64)) >= 0) and np.all(np.diff(g.Br.astype(np.int64)) >= 0)
+        if g.E:
+            assert g.Fc.min() >= 1 and g.Fc.max() <= g.V and g.Bc.min() >= 1 and g.Bc.max() <= g.V
+        # the reverse CSR holds exactly the forward arcs
+        src = np.repeat(np.arange(g.V + 1), np.diff(g.Fr.astype(np.int64))[:g.V + 1])
+        dst = np.repeat(np.arange(g.V + 1), np.diff(g.Br.astype(np.int64))[:g.V + 1])
+        fw = sorted(zip(src.tolist(), g.Fc.tolist()))
+        bw = sorted(zip(g.Bc.tolist(), dst.tolist()))
+        assert fw == bw
+
+
+def test_batch_mix_layout_and_seed():
+    b = scc.gen_batch_mix(40, 6, 10, 2.0, 12.0, seed=9)
+    again = scc.gen_batch_mix(40, 6, 10, 2.0, 12.0, seed=9)
+    assert np.array_equal(b.Fc, again.Fc) and np.array_equal(b.V, again.V)
+    assert b.V.min() >= 2 ** 6 - 1 and b.V.max() <= 2 ** 10 + 1
+    assert set(np.unique(b.kind)) <= {0, 1, 2}
+    _check_batch_layout(b)
+    # per-graph arrays match a rebuild from the same arcs
+    g = b.graph(7)
+    src = np.repeat(np.arange(g.V + 1, dtype=np.uint32), np.diff(g.Fr.astype(np.int64))[:g.V + 1])
+    h = scc.from_arcs(g.V, src, g.Fc)
+    assert np.array_equal(h.Fr, g.Fr) and np.array_equal(h.Fc, g.Fc) and np.array_equal(h.Bc, g.Bc)
+
+
+def test_serving_mix_distribution():
+    b = scc.gen_serving_mix(300, median_v=4096, sigma=1.0, vcap=1 << 15, degree=6.0, seed=2)
+    assert b.V.max() <= 1 << 15 and b.V.min() >= 2
+    lg = np.log(b.V.astype(np.float64))
+    assert abs(np.median(lg) - np.log(4096)) < 0.3          # lognormal median
+    assert 0.7 < np.std(lg[b.V < 1 << 15]) < 1.2            # sigma, cap trimmed
+    assert abs(float(b.E.sum()) / float(b.V.sum()) - 6.0) < 0.2
+    assert 100 < int((b.kind == 1).sum()) < 200             # ER / R-MAT even odds
+    _check_batch_layout(scc.gen_serving_mix(12, median_v=300, sigma=1.5, vcap=5000, degree=8.0, seed=3))
+
+
+def test_batch_from_graphs_roundtrip():
+    gs = [scc.gen_er(50 + 7 * i, 200, cycles=2, seed=i) for i in range(6)] + [scc.from_arcs(1, [], [])]
+    b = scc.batch_from_graphs(gs)
+    for i, g in enumerate(gs):
+        h = b.graph(i)
+        for name in ("Fr", "Fc", "Br", "Bc"):
+            assert np.array_equal(getattr(h, name), getattr(g, name))
