@@ -42,6 +42,9 @@ WORKLOADS = {
     "rmat24": "R-MAT scale 24, 16,777,216 V, 268,435,456 arcs, a=.57 b=.19 c=.19, seed 24",
     "er16k": "ER 16,384 V, 131,072 draws + 64 planted cycles (2..256), seed 1",
     "grid4096": "grid 4096x4096, arc probability 0.7, seed 4096",
+    # measured alone (one JSON line each, value = graphs/s)
+    "batch4096": "the 4096-graph mix, batched persistent form vs one graph per launch",
+    "serve": "Poisson request stream into the continuous-batching server",
 }
 
 
@@ -190,6 +193,16 @@ def main():
             ms = ctx.copy_bench(args.copy_calibrate, 5)
         print(json.dumps({"copy_bytes": args.copy_calibrate, "ms_per_copy": ms,
                           "GBps_read_plus_write": 2 * args.copy_calibrate / ms / 1e6}), flush=True)
+        return
+    if args.workload in ("batch4096", "serve"):
+        part = bench_batched(steps=args.steps) if args.workload == "batch4096" else bench_serving(rate=args.serve_rate)
+        print(json.dumps({
+            "metric": "batched graphs/s and p99 vs one-graph-per-launch" if args.workload == "batch4096"
+                      else "served graphs/s and p50/p99 latency under an HBM budget",
+            "value": part["graphs_per_s"], "unit": "graphs/s", "n_gpus": 1, "steps": args.steps,
+            "warmup": 1, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "int32",
+            "data": "synthetic", "config": {"workload": part.pop("workload"), "parallelism": "single GPU"},
+            "detail": part}), flush=True)
         return
     t0 = time.perf_counter()
     g = build_graph(args.workload)

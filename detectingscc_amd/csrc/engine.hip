@@ -310,6 +310,14 @@ void Engine::init_vis()
 // back to top-down when it falls under 1/kToTopDown of the live vertices.
 static constexpr int64_t kToBottomUp = 16;
 static constexpr int64_t kToTopDown = 48;
+// While every active direction is top-down and its frontier would have to
+// grow kSpecGrowth-fold to reach the bottom-up switch point, the host queues
+// kSpecLevels levels before it reads the counters again (high-diameter
+// graphs run thousands of small levels; a host round trip per level would
+// cost more than the level).  A frontier that empties inside the batch
+// turns the remaining launches into no-ops.
+static constexpr int kSpecLevels = 16;
+static constexpr int64_t kSpecGrowth = 8;
 
 // FW and/or BW search from the seeded queues until both frontiers are empty.
 scc_status Engine::reach(bool fw, bool bw, bool single, int expand, int W, uint32_t *levels_out)
@@ -321,10 +329,13 @@ scc_status Engine::reach(bool fw, bool bw, bool single, int expand, int W, uint3
     int64_t visited[2] = {0, 0};
     bool counted_seeds = false;
     if (!fw && !bw) return SCC_OK;
-    // DO mode decides every level, so it reads the counters every level
+    // DO mode decides after every level unless the frontiers are far from
+    // the switch point (then kSpecLevels levels are queued per decision)
     const int K = dopt ? 1 : (check_every > 0 ? check_every : 2);
     const uint32_t blocks = bpd(td_expand == SCC_EXPAND_VWARP ? W : 1);
     uint32_t level = 0;
+    int queued = 0, batch = 1;
+    uint32_t spec_blocks = blocks;
     for (;;) {
         const int in = level & 1, out = in ^ 1;
         DirArgs D[2];
@@ -347,37 +358,43 @@ scc_status Engine::reach(bool fw, bool bw, bool single, int expand, int W, uint3
         a.level = (int)level;
         for (int d = 0; d < 2; d++)
             if (active[d] && !bottom_up[d]) a.d[a.ndirs++] = D[d];
-        if (a.ndirs) k_expand_level(stream, a, td_expand, W, single, blocks);
+        if (a.ndirs) k_expand_level(stream, a, td_expand, W, single, batch > 1 ? spec_blocks : blocks);
         for (int d = 0; d < 2; d++)
             if (active[d] && bottom_up[d])
                 k_bottom_up(stream, D[d], d ? g.Fr : g.Br, d ? g.Fc : g.Bc, st.dbits, st.cls, g.V,
                             (int)level, single);
         level++;
-        if (level % K == 0) {
-            scc_status s = read_ctr();
-            if (s != SCC_OK) return s;
-            const int64_t nlive = h_ctr[CT(C_NLIVE)];
-            bool any = false;
-            for (int d = 0; d < 2; d++) {
-                if (!active[d]) continue;
-                const int64_t nf = h_ctr[CT((d ? C_BWCNT : C_FWCNT) + level % 3)];
-                if (!nf) {
-                    if (dopt) active[d] = false;
-                    continue;
-                }
-                any = true;
-                if (!dopt) continue;
-                if (!counted_seeds) visited[d] += h_ctr[CT(C_NPIV)];
-                visited[d] += nf;
-                const int64_t remaining = nlive - visited[d];
-                if (!bottom_up[d] && nf * kToBottomUp > remaining) bottom_up[d] = true;
-                else if (bottom_up[d] && nf * kToTopDown < nlive) bottom_up[d] = false;
+        if (dopt ? ++queued < batch : level % K != 0) continue;
+        queued = 0;
+        scc_status s = read_ctr();
+        if (s != SCC_OK) return s;
+        const int64_t nlive = h_ctr[CT(C_NLIVE)];
+        bool any = false, spec = dopt;
+        int64_t nf_max = 0;
+        for (int d = 0; d < 2; d++) {
+            if (!active[d]) continue;
+            const int64_t nf = h_ctr[CT((d ? C_BWCNT : C_FWCNT) + level % 3)];
+            if (!nf) {
+                if (dopt) active[d] = false;
+                continue;
             }
-            counted_seeds = true;
-            if (!any) break;
-            // every level claims at least one vertex, so V+1 levels bound a search
-            if (level > limit()) return no_convergence("reachability");
+            any = true;
+            if (!dopt) continue;
+            if (!counted_seeds) visited[d] += h_ctr[CT(C_NPIV)];
+            visited[d] += nf;
+            const int64_t remaining = nlive - visited[d];
+            if (!bottom_up[d] && nf * kToBottomUp > remaining) bottom_up[d] = true;
+            else if (bottom_up[d] && nf * kToTopDown < nlive) bottom_up[d] = false;
+            if (bottom_up[d] || nf * kToBottomUp * kSpecGrowth > remaining) spec = false;
+            nf_max = std::max(nf_max, nf);
         }
+        counted_seeds = true;
+        if (!any) break;
+        // every level claims at least one vertex, so V+1 levels bound a search
+        if (level > limit()) return no_convergence("reachability");
+        batch = spec ? kSpecLevels : 1;
+        // queued levels get a grid sized for a few times the current frontier
+        spec_blocks = (uint32_t)std::min<int64_t>(blocks, std::max<int64_t>(32, (nf_max * 4 + 255) / 256));
     }
     if (levels_out) *levels_out += level;
     return SCC_OK;
