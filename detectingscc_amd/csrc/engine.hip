@@ -46,8 +46,10 @@ void Engine::release()
     f(g.Fr); f(g.Fc); f(g.Br); f(g.Bc);
     f(st.cls); f(st.lab); f(st.aux); f(st.fvis); f(st.bvis); f(st.dbits); f(st.piv);
     for (int i = 0; i < 2; i++) {
-        f(st.qf[i]); f(st.qb[i]); f(st.bigf[i]); f(st.bigb[i]);
-        f(st.bigoff_f[i]); f(st.bigoff_b[i]);
+        f(st.qf[i]); f(st.qb[i]);
+    }
+    for (int i = 0; i < 3; i++) {
+        f(st.bigf[i]); f(st.bigb[i]); f(st.bigoff_f[i]); f(st.bigoff_b[i]);
     }
     f(st.pivlist); f(st.part); f(st.ctr); f(d_tmp8);
     for (auto &b : buf) {
@@ -100,8 +102,10 @@ scc_status Engine::reserve(uint32_t V, uint32_t E)
     f(g.Fr); f(g.Fc); f(g.Br); f(g.Bc);
     f(st.cls); f(st.lab); f(st.aux); f(st.fvis); f(st.bvis); f(st.dbits); f(st.piv);
     for (int i = 0; i < 2; i++) {
-        f(st.qf[i]); f(st.qb[i]); f(st.bigf[i]); f(st.bigb[i]);
-        f(st.bigoff_f[i]); f(st.bigoff_b[i]);
+        f(st.qf[i]); f(st.qb[i]);
+    }
+    for (int i = 0; i < 3; i++) {
+        f(st.bigf[i]); f(st.bigb[i]); f(st.bigoff_f[i]); f(st.bigoff_b[i]);
     }
     f(st.pivlist); f(st.part); f(d_tmp8);
     uint32_t *ctr = st.ctr;
@@ -122,8 +126,12 @@ scc_status Engine::reserve(uint32_t V, uint32_t E)
     A(dalloc(&st.piv, 3 * nv + 3));
     for (int i = 0; i < 2; i++) {
         A(dalloc(&st.qf[i], nv)); A(dalloc(&st.qb[i], nv));
-        A(dalloc(&st.bigf[i], nv)); A(dalloc(&st.bigb[i], nv));
-        A(dalloc(&st.bigoff_f[i], nv)); A(dalloc(&st.bigoff_b[i], nv));
+    }
+    // a hub list holds vertices of degree >= LB_BIG only
+    const size_t nhub = std::min<size_t>(nv, (size_t)nE / LB_BIG + 1);
+    for (int i = 0; i < 3; i++) {
+        A(dalloc(&st.bigf[i], nhub)); A(dalloc(&st.bigb[i], nhub));
+        A(dalloc(&st.bigoff_f[i], nhub)); A(dalloc(&st.bigoff_b[i], nhub));
     }
     A(dalloc(&st.pivlist, nv)); A(dalloc(&st.part, nv + 1)); A(dalloc(&d_tmp8, nv));
     if (e != hipSuccess) {
@@ -310,14 +318,16 @@ void Engine::init_vis()
 // back to top-down when it falls under 1/kToTopDown of the live vertices.
 static constexpr int64_t kToBottomUp = 16;
 static constexpr int64_t kToTopDown = 48;
-// While every active direction is top-down and its frontier would have to
-// grow kSpecGrowth-fold to reach the bottom-up switch point, the host queues
+// While every active direction is top-down, has no hubs pending, grew at
+// most kSpecRise-fold since the previous check and would have to grow
+// kSpecGrowth-fold more to reach the bottom-up switch point, the host queues
 // kSpecLevels levels before it reads the counters again (high-diameter
 // graphs run thousands of small levels; a host round trip per level would
 // cost more than the level).  A frontier that empties inside the batch
 // turns the remaining launches into no-ops.
 static constexpr int kSpecLevels = 16;
 static constexpr int64_t kSpecGrowth = 8;
+static constexpr int64_t kSpecRise = 2;
 
 // FW and/or BW search from the seeded queues until both frontiers are empty.
 scc_status Engine::reach(bool fw, bool bw, bool single, int expand, int W, uint32_t *levels_out)
@@ -327,6 +337,7 @@ scc_status Engine::reach(bool fw, bool bw, bool single, int expand, int W, uint3
     bool active[2] = {fw, bw};
     bool bottom_up[2] = {false, false};
     int64_t visited[2] = {0, 0};
+    int64_t nf_prev[2] = {0, 0};
     bool counted_seeds = false;
     if (!fw && !bw) return SCC_OK;
     // DO mode decides after every level unless the frontiers are far from
@@ -347,10 +358,14 @@ scc_status Engine::reach(bool fw, bool bw, bool single, int expand, int W, uint3
             D[d].qin = d ? st.qb[in] : st.qf[in];
             D[d].qout = d ? st.qb[out] : st.qf[out];
             D[d].cnt = st.ctr + CT(d ? C_BWCNT : C_FWCNT);
-            D[d].big = d ? st.bigb[in] : st.bigf[in];
-            D[d].bigoff = d ? st.bigoff_b[in] : st.bigoff_f[in];
-            D[d].bigc = reinterpret_cast<unsigned long long *>(st.ctr + CT(hub + in));
-            D[d].bigc_next = reinterpret_cast<unsigned long long *>(st.ctr + CT(hub + out));
+            const int cur = level % 3, prev = (level + 2) % 3, next = (level + 1) % 3;
+            D[d].big = d ? st.bigb[cur] : st.bigf[cur];
+            D[d].bigoff = d ? st.bigoff_b[cur] : st.bigoff_f[cur];
+            D[d].bigc = reinterpret_cast<unsigned long long *>(st.ctr + CT(hub + cur));
+            D[d].pbig = d ? st.bigb[prev] : st.bigf[prev];
+            D[d].pbigoff = d ? st.bigoff_b[prev] : st.bigoff_f[prev];
+            D[d].pbigc = reinterpret_cast<unsigned long long *>(st.ctr + CT(hub + prev));
+            D[d].bigc_zero = reinterpret_cast<unsigned long long *>(st.ctr + CT(hub + next));
         }
         LevelArgs a;
         std::memset(&a, 0, sizeof a);
@@ -374,7 +389,11 @@ scc_status Engine::reach(bool fw, bool bw, bool single, int expand, int W, uint3
         for (int d = 0; d < 2; d++) {
             if (!active[d]) continue;
             const int64_t nf = h_ctr[CT((d ? C_BWCNT : C_FWCNT) + level % 3)];
-            if (!nf) {
+            // hubs of the last top-down level still to expand (a bottom-up
+            // pass needs no list: it finds them as visited parents)
+            const uint64_t hubs = bottom_up[d] ? 0 : *reinterpret_cast<const unsigned long long *>(
+                h_ctr + CT((d ? C_BIGB : C_BIGF) + (level + 2) % 3)) >> 32;
+            if (!nf && !hubs) {
                 if (dopt) active[d] = false;
                 continue;
             }
@@ -385,7 +404,10 @@ scc_status Engine::reach(bool fw, bool bw, bool single, int expand, int W, uint3
             const int64_t remaining = nlive - visited[d];
             if (!bottom_up[d] && nf * kToBottomUp > remaining) bottom_up[d] = true;
             else if (bottom_up[d] && nf * kToTopDown < nlive) bottom_up[d] = false;
-            if (bottom_up[d] || nf * kToBottomUp * kSpecGrowth > remaining) spec = false;
+            if (bottom_up[d] || hubs || !nf_prev[d] || nf > nf_prev[d] * kSpecRise ||
+                nf * kToBottomUp * kSpecGrowth > remaining)
+                spec = false;
+            nf_prev[d] = nf;
             nf_max = std::max(nf_max, nf);
         }
         counted_seeds = true;

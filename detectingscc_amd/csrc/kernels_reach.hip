@@ -11,7 +11,7 @@
 // the class compare is skipped (the probe is one bitmap bit).
 //
 // Atomics on one address serialise (~10 ns each on MI355X), so the fast
-// kernels (LB, hub, bottom-up) stage pushes in LDS and reserve queue space
+// kernels (LB, bottom-up) stage pushes in LDS and reserve queue space
 // with one global atomic per ~1,800 pushes; the reference-mapping kernels
 // (vertex, virtual warp) aggregate per wave.
 #include "scc_dev.h"
@@ -123,11 +123,13 @@ __global__ __launch_bounds__(256) void expand_vw_kernel(LevelArgs a, uint32_t bp
 // Load-balanced expansion (SCC_EXPAND_LB).  A 256-lane workgroup takes 256
 // frontier vertices, scans their degrees in LDS and walks the concatenated
 // adjacency slots with all lanes, so consecutive lanes read consecutive
-// column entries whatever the degree mix.  Vertices of degree >= LB_BIG are
-// deferred to a hub list and split into LB_CHUNK-arc chunks spread over
-// every workgroup by expand_hub_kernel, launched right after.
+// column entries whatever the degree mix.  A vertex of degree >= LB_BIG
+// (a hub) is instead appended to this level's hub list; the next level's
+// launch first splits the listed hubs into LB_CHUNK-arc chunks spread over
+// every workgroup, then takes its own tiles.  A hub's neighbours thus join
+// the frontier one level late, which leaves every reachable set unchanged
+// and makes a level a single launch.
 constexpr uint32_t LB_TILE = 256;
-constexpr uint32_t LB_BIG = 512;
 constexpr uint32_t LB_CHUNK = 2048;
 
 // Exclusive scan of x over the 256-lane workgroup; returns the prefix and
@@ -164,15 +166,43 @@ __global__ __launch_bounds__(256) void expand_lb_kernel(LevelArgs a, uint32_t bp
     const DirArgs &D = a.d[dir];
     const int L = a.level;
     const uint32_t n = D.cnt[CT(L % 3)];
+    const unsigned long long packed = *D.pbigc;
+    const uint32_t items = (uint32_t)(packed >> 32), chunks = (uint32_t)packed;
     uint32_t *cnt_out = &D.cnt[CT((L + 1) % 3)];
     if (bid == 0 && threadIdx.x == 0) {
         D.cnt[CT((L + 2) % 3)] = 0;
-        *D.bigc_next = 0ull;         // the previous level's hub list is consumed
+        *D.bigc_zero = 0ull;
     }
-    if (n == 0) return;
+    if (n == 0 && items == 0) return;
     DECLARE_LDS_QUEUE(q);
     const uint32_t *__restrict__ R = D.R;
     const uint32_t *__restrict__ C = D.C;
+    // hubs of the previous level: chunk g belongs to the item whose chunk
+    // offset is the last one <= g (offsets grow with the item index because
+    // both come from one 64-bit atomicAdd)
+    for (uint32_t gch = bid; gch < chunks; gch += bpd) {
+        uint32_t lo = 0, hi = items - 1;
+        while (lo < hi) {
+            const uint32_t mid = (lo + hi + 1) >> 1;
+            if (D.pbigoff[mid] <= gch) lo = mid;
+            else hi = mid - 1;
+        }
+        const uint32_t v = D.pbig[lo];
+        const uint32_t c = SINGLE ? 0u : a.cls[v];
+        const uint32_t b = R[v] + (gch - D.pbigoff[lo]) * LB_CHUNK;
+        const uint32_t e = min(R[v + 1], b + LB_CHUNK);
+        for (uint32_t j0 = b; j0 < e; j0 += 256) {
+            const uint32_t j = j0 + threadIdx.x;
+            bool claimed = false;
+            uint32_t w = 0;
+            if (j < e) {
+                w = C[j];
+                claimed = try_claim<SINGLE>(D.vis, a.cls, w, c);
+            }
+            lq_push(q, claimed, w);
+            lq_flush(q, cnt_out, D.qout, false);
+        }
+    }
     for (uint32_t base = bid * LB_TILE; base < n; base += bpd * LB_TILE) {
         const uint32_t i = base + threadIdx.x;
         uint32_t v = 0, beg = 0, deg = 0, c = 0;
@@ -182,8 +212,8 @@ __global__ __launch_bounds__(256) void expand_lb_kernel(LevelArgs a, uint32_t bp
             deg = R[v + 1] - beg;
             if (!SINGLE) c = a.cls[v];
             if (deg >= LB_BIG) {
-                const unsigned long long chunks = (deg + LB_CHUNK - 1) / LB_CHUNK;
-                const unsigned long long old = atomicAdd(D.bigc, (1ull << 32) | chunks);
+                const unsigned long long nch = (deg + LB_CHUNK - 1) / LB_CHUNK;
+                const unsigned long long old = atomicAdd(D.bigc, (1ull << 32) | nch);
                 const uint32_t item = (uint32_t)(old >> 32);
                 D.big[item] = v;
                 D.bigoff[item] = (uint32_t)old;
@@ -220,58 +250,11 @@ __global__ __launch_bounds__(256) void expand_lb_kernel(LevelArgs a, uint32_t bp
     lq_flush(q, cnt_out, D.qout, true);
 }
 
-// Chunks of hub adjacency lists: chunk g belongs to the hub item whose chunk
-// offset is the last one <= g (offsets grow with the item index because
-// both come from one 64-bit atomicAdd).
-template <bool SINGLE>
-__global__ __launch_bounds__(256) void expand_hub_kernel(LevelArgs a, uint32_t bpd)
-{
-    const int dir = (a.ndirs == 2 && blockIdx.x >= bpd) ? 1 : 0;
-    const uint32_t bid = blockIdx.x - (uint32_t)dir * bpd;
-    const DirArgs &D = a.d[dir];
-    const unsigned long long packed = *D.bigc;
-    const uint32_t items = (uint32_t)(packed >> 32), chunks = (uint32_t)packed;
-    if (!items || bid >= chunks) return;
-    DECLARE_LDS_QUEUE(q);
-    uint32_t *cnt_out = &D.cnt[CT((a.level + 1) % 3)];
-    const uint32_t *__restrict__ R = D.R;
-    const uint32_t *__restrict__ C = D.C;
-    for (uint32_t gch = bid; gch < chunks; gch += bpd) {
-        uint32_t lo = 0, hi = items - 1;
-        while (lo < hi) {
-            const uint32_t mid = (lo + hi + 1) >> 1;
-            if (D.bigoff[mid] <= gch) lo = mid;
-            else hi = mid - 1;
-        }
-        const uint32_t v = D.big[lo];
-        const uint32_t c = SINGLE ? 0u : a.cls[v];
-        const uint32_t b = R[v] + (gch - D.bigoff[lo]) * LB_CHUNK;
-        const uint32_t e = min(R[v + 1], b + LB_CHUNK);
-        for (uint32_t j0 = b; j0 < e; j0 += 256) {
-            const uint32_t j = j0 + threadIdx.x;
-            bool claimed = false;
-            uint32_t w = 0;
-            if (j < e) {
-                w = C[j];
-                claimed = try_claim<SINGLE>(D.vis, a.cls, w, c);
-            }
-            lq_push(q, claimed, w);
-            lq_flush(q, cnt_out, D.qout, false);
-        }
-    }
-    lq_flush(q, cnt_out, D.qout, true);
-}
-
 static void launch_lb(hipStream_t s, const LevelArgs &a, bool single, uint32_t bpd)
 {
     const uint32_t grid = bpd * (uint32_t)a.ndirs;
-    if (single) {
-        expand_lb_kernel<true><<<grid, 256, 0, s>>>(a, bpd);
-        expand_hub_kernel<true><<<grid, 256, 0, s>>>(a, bpd);
-    } else {
-        expand_lb_kernel<false><<<grid, 256, 0, s>>>(a, bpd);
-        expand_hub_kernel<false><<<grid, 256, 0, s>>>(a, bpd);
-    }
+    if (single) expand_lb_kernel<true><<<grid, 256, 0, s>>>(a, bpd);
+    else expand_lb_kernel<false><<<grid, 256, 0, s>>>(a, bpd);
 }
 
 // ---------------------------------------------------------------------
@@ -310,7 +293,7 @@ __global__ __launch_bounds__(256) void expand_bu_kernel(DirArgs D, const uint32_
     uint32_t *cnt_out = &D.cnt[CT((L + 1) % 3)];
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         D.cnt[CT((L + 2) % 3)] = 0;
-        *D.bigc_next = 0ull;
+        *D.bigc_zero = 0ull;
     }
     if (threadIdx.x == 0) s_nlong = 0;
     DECLARE_LDS_QUEUE(q);

@@ -327,7 +327,7 @@ __global__ void select_kernel(const uint32_t *__restrict__ Fr, const uint32_t *_
 
 __device__ __forceinline__ void zero_hub_counts(uint32_t *ctr)
 {
-    for (int i = 0; i < 2; i++) {
+    for (int i = 0; i < 3; i++) {
         *reinterpret_cast<unsigned long long *>(ctr + CT(C_BIGF + i)) = 0ull;
         *reinterpret_cast<unsigned long long *>(ctr + CT(C_BIGB + i)) = 0ull;
     }

@@ -29,6 +29,7 @@ constexpr uint32_t DEAD = 0xFFFFFFFFu;
 // singletons, trim-2 pairs), so the final min-per-SCC pass skips it.
 constexpr uint32_t LAB_FINAL = 0x80000000u;
 constexpr int SWEEP_BLOCK = 256;
+constexpr uint32_t LB_BIG = 512;   // degree from which a frontier vertex is a hub (LB mode)
 
 // Counters block: counter i lives at word CT(i), one 128-byte line per
 // counter, so atomics on different counters never share a line (same-line
@@ -52,8 +53,8 @@ enum : int {
     C_FWE = 16,
     C_BWV = 18,
     C_BWE = 20,
-    C_BIGF = 22,     // 2 x 64-bit packed (items, chunks) counts of the FW hub list (22, 23)
-    C_BIGB = 24,     // same for BW (24, 25)
+    C_BIGF = 22,     // 3 rotating 64-bit packed (items, chunks) counts of the FW hub lists (22..24)
+    C_BIGB = 25,     // same for BW (25..27)
     C_PIV0 = 30,     // first pivot of the latest election
     C_ANOM = 31,     // invariant violations caught by kernel guards (must stay 0)
     C_NLIVE = 32,    // live vertices when the visited bitmaps were initialised
@@ -72,8 +73,8 @@ struct DevState {
     uint64_t *piv = nullptr;
     uint32_t *qf[2] = {nullptr, nullptr}, *qb[2] = {nullptr, nullptr};
     uint32_t *pivlist = nullptr;   // pivots of the current round
-    uint32_t *bigf[2] = {nullptr, nullptr}, *bigb[2] = {nullptr, nullptr};  // hub lists
-    uint32_t *bigoff_f[2] = {nullptr, nullptr}, *bigoff_b[2] = {nullptr, nullptr};
+    uint32_t *bigf[3] = {}, *bigb[3] = {};           // hub lists, rotating like the counts
+    uint32_t *bigoff_f[3] = {}, *bigoff_b[3] = {};
     uint32_t *part = nullptr;      // partition ids (detectSCC)
     uint32_t *ctr = nullptr;       // C_COUNT words
     uint32_t nwords = 0;           // bitmap words
@@ -86,10 +87,16 @@ struct DirArgs {
     const uint32_t *qin;
     uint32_t *qout;
     uint32_t *cnt;               // the 3 rotating counts
-    uint32_t *big;               // hub list of this level (LB mode)
+    // LB mode defers each hub (degree >= LB_BIG) of level L to a list whose
+    // chunks the level L+1 kernel expands before its own tiles, so a level
+    // is one launch; three lists rotate like the counts
+    uint32_t *big;               // list this level appends to
     uint32_t *bigoff;            // chunk offset of each hub
-    unsigned long long *bigc;    // packed (items << 32 | chunks) of this level
-    unsigned long long *bigc_next;
+    unsigned long long *bigc;    // its packed (items << 32 | chunks)
+    const uint32_t *pbig;        // previous level's list, expanded by this level
+    const uint32_t *pbigoff;
+    const unsigned long long *pbigc;
+    unsigned long long *bigc_zero;   // the list the next level appends to, cleared here
 };
 
 struct LevelArgs {
