@@ -69,6 +69,8 @@ class Engine {
     uint32_t *h_ctr = nullptr;        // pinned mirror of st.ctr
     uint8_t *d_tmp8 = nullptr;        // V+1 bytes for phase inputs/outputs
     int check_every = 0;
+    bool search_stats = false;        // count |R| and |E_R| of the first search
+    uint32_t trim_epoch = 0;          // last epoch stamped by a trim-1 list pass
     std::vector<hipEvent_t> events;
     std::vector<int> mark_phase;
     size_t nmarks = 0;

@@ -44,14 +44,14 @@ void Engine::release()
         if (p) (void)hipFree(p);
     };
     f(g.Fr); f(g.Fc); f(g.Br); f(g.Bc);
-    f(st.cls); f(st.lab); f(st.aux); f(st.fvis); f(st.bvis); f(st.dbits); f(st.piv);
+    f(st.cls); f(st.lab); f(st.aux); f(st.fvis); f(st.bvis); f(st.piv);
     for (int i = 0; i < 2; i++) {
         f(st.qf[i]); f(st.qb[i]);
     }
     for (int i = 0; i < 3; i++) {
         f(st.bigf[i]); f(st.bigb[i]); f(st.bigoff_f[i]); f(st.bigoff_b[i]);
     }
-    f(st.pivlist); f(st.part); f(st.ctr); f(d_tmp8);
+    f(st.pivlist); f(st.stamp); f(st.part); f(st.ctr); f(d_tmp8);
     for (auto &b : buf) {
         f(b.p);
         b = DevBuf();
@@ -100,14 +100,14 @@ scc_status Engine::reserve(uint32_t V, uint32_t E)
         if (p) (void)hipFree(p);
     };
     f(g.Fr); f(g.Fc); f(g.Br); f(g.Bc);
-    f(st.cls); f(st.lab); f(st.aux); f(st.fvis); f(st.bvis); f(st.dbits); f(st.piv);
+    f(st.cls); f(st.lab); f(st.aux); f(st.fvis); f(st.bvis); f(st.piv);
     for (int i = 0; i < 2; i++) {
         f(st.qf[i]); f(st.qb[i]);
     }
     for (int i = 0; i < 3; i++) {
         f(st.bigf[i]); f(st.bigb[i]); f(st.bigoff_f[i]); f(st.bigoff_b[i]);
     }
-    f(st.pivlist); f(st.part); f(d_tmp8);
+    f(st.pivlist); f(st.stamp); f(st.part); f(d_tmp8);
     uint32_t *ctr = st.ctr;
     g = DevGraph();
     st = DevState();
@@ -122,7 +122,7 @@ scc_status Engine::reserve(uint32_t V, uint32_t E)
     A(dalloc(&g.Fr, nv + 1)); A(dalloc(&g.Br, nv + 1));
     A(dalloc(&g.Fc, nE)); A(dalloc(&g.Bc, nE));
     A(dalloc(&st.cls, nv)); A(dalloc(&st.lab, nv)); A(dalloc(&st.aux, nv));
-    A(dalloc(&st.fvis, nwords)); A(dalloc(&st.bvis, nwords)); A(dalloc(&st.dbits, nwords));
+    A(dalloc(&st.fvis, 2 * nwords)); A(dalloc(&st.bvis, 2 * nwords));
     A(dalloc(&st.piv, 3 * nv + 3));
     for (int i = 0; i < 2; i++) {
         A(dalloc(&st.qf[i], nv)); A(dalloc(&st.qb[i], nv));
@@ -133,7 +133,7 @@ scc_status Engine::reserve(uint32_t V, uint32_t E)
         A(dalloc(&st.bigf[i], nhub)); A(dalloc(&st.bigb[i], nhub));
         A(dalloc(&st.bigoff_f[i], nhub)); A(dalloc(&st.bigoff_b[i], nhub));
     }
-    A(dalloc(&st.pivlist, nv)); A(dalloc(&st.part, nv + 1)); A(dalloc(&d_tmp8, nv));
+    A(dalloc(&st.pivlist, nv)); A(dalloc(&st.stamp, nv)); A(dalloc(&st.part, nv + 1)); A(dalloc(&d_tmp8, nv));
     if (e != hipSuccess) {
         (void)hipGetLastError();
         return fail(e == hipErrorOutOfMemory ? SCC_ERR_NOMEM : SCC_ERR_DEVICE,
@@ -142,6 +142,8 @@ scc_status Engine::reserve(uint32_t V, uint32_t E)
     st.nwords = (uint32_t)nwords;
     capV = nV;
     capE = nE;
+    k_fill_u32(stream, st.stamp, 0u, nv);   // no epoch is 0
+    trim_epoch = 0;
     return SCC_OK;
 }
 
@@ -374,10 +376,18 @@ scc_status Engine::reach(bool fw, bool bw, bool single, int expand, int W, uint3
         for (int d = 0; d < 2; d++)
             if (active[d] && !bottom_up[d]) a.d[a.ndirs++] = D[d];
         if (a.ndirs) k_expand_level(stream, a, td_expand, W, single, batch > 1 ? spec_blocks : blocks);
+        BottomUpArgs b;
+        std::memset(&b, 0, sizeof b);
+        b.cls = st.cls;
+        b.V = g.V;
+        b.level = (int)level;
         for (int d = 0; d < 2; d++)
-            if (active[d] && bottom_up[d])
-                k_bottom_up(stream, D[d], d ? g.Fr : g.Br, d ? g.Fc : g.Bc, st.dbits, st.cls, g.V,
-                            (int)level, single);
+            if (active[d] && bottom_up[d]) {
+                b.PR[b.ndirs] = d ? g.Fr : g.Br;
+                b.PC[b.ndirs] = d ? g.Fc : g.Bc;
+                b.d[b.ndirs++] = D[d];
+            }
+        if (b.ndirs) k_bottom_up(stream, b, single);
         level++;
         if (dopt ? ++queued < batch : level % K != 0) continue;
         queued = 0;
@@ -422,20 +432,29 @@ scc_status Engine::reach(bool fw, bool bw, bool single, int expand, int W, uint3
     return SCC_OK;
 }
 
-// trim-1 to its fixpoint (scc_WCC.cu:680-687), or one pass.
+// trim-1 to its fixpoint (scc_WCC.cu:680-687), or one pass.  After the
+// first sweep each pass re-checks the neighbours of the previous pass's
+// removals (listed alternately in the FW queue buffers, which are free
+// outside a search) until a pass removes nothing.
 scc_status Engine::trim1(bool fixpoint)
 {
     mark(SCC_PHASE_TRIM1);
-    uint32_t prev;
+    zero_ctr(C_TLIST, 3);
+    k_trim1(stream, g, st, st.qf[0]);
     scc_status s = read_ctr();
     if (s != SCC_OK) return s;
-    uint64_t pass = 0;
-    do {
-        prev = h_ctr[CT(C_TRIM1)];
-        k_trim1(stream, g, st);
+    for (uint32_t pass = 1; fixpoint; pass++) {
+        const uint32_t n = h_ctr[CT(C_TLIST + pass % 3)];
+        if (!n) break;
+        // every pass but the last removes a vertex
+        if (pass > limit()) return no_convergence("trim-1");
+        if (++trim_epoch == 0) {   // wrapped: no stale stamp may match a new epoch
+            k_fill_u32(stream, st.stamp, 0u, (uint64_t)g.V + 1);
+            trim_epoch = 1;
+        }
+        k_trim1_list(stream, g, st, st.qf[(pass + 1) & 1], st.qf[pass & 1], trim_epoch, pass, n);
         if ((s = read_ctr()) != SCC_OK) return s;
-        if (++pass > limit()) return no_convergence("trim-1");
-    } while (fixpoint && h_ctr[CT(C_TRIM1)] != prev);
+    }
     return SCC_OK;
 }
 
@@ -463,7 +482,7 @@ scc_status Engine::pivot_round(bool first, int expand, int W, scc_stats *stats, 
     if (first) {
         first_reach_marks[0] = first_mark;
         first_reach_marks[1] = (int)nmarks - 1;
-        k_reach_stats(stream, g, st);
+        if (search_stats) k_reach_stats(stream, g, st);
     }
     zero_ctr(C_LIVE, 1);
     k_update(stream, g, st);
@@ -540,6 +559,7 @@ scc_status Engine::run(const scc_opts &o, scc_stats *out)
     const int first_expand = expand;
     const int loop_expand = (expand == SCC_EXPAND_VWARP) ? SCC_EXPAND_VERTEX : expand;
     check_every = o.check_every;
+    search_stats = o.sizes != 0;
 
     scc_stats stats;
     std::memset(&stats, 0, sizeof stats);

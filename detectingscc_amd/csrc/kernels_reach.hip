@@ -14,6 +14,7 @@
 // kernels (LB, bottom-up) stage pushes in LDS and reserve queue space
 // with one global atomic per ~1,800 pushes; the reference-mapping kernels
 // (vertex, virtual warp) aggregate per wave.
+#include "lds_queue.h"
 #include "scc_dev.h"
 #include "scc_amd.h"
 
@@ -25,9 +26,9 @@ template <bool SINGLE>
 __device__ __forceinline__ bool try_claim(uint32_t *vis, const uint32_t *cls, uint32_t w, uint32_t c)
 {
     const uint32_t bit = 1u << (w & 31);
-    if (vis[w >> 5] & bit) return false;
+    if (vis[vword(w)] & bit) return false;
     if (!SINGLE && cls[w] != c) return false;
-    return !(atomicOr(&vis[w >> 5], bit) & bit);
+    return !(atomicOr(&vis[vword(w)], bit) & bit);
 }
 
 // Append w for every active lane with `claimed`; one atomic per wave.
@@ -42,52 +43,6 @@ __device__ __forceinline__ void push_wave(bool claimed, uint32_t w, uint32_t *qo
     base = __shfl(base, leader);
     if (claimed) qout[base + (uint32_t)__popcll(m & ((1ull << wlane()) - 1))] = w;
 }
-
-// ------------------------------------------------- LDS-staged queue ---
-constexpr uint32_t QBUF = 2048;
-
-struct LdsQueue {
-    uint32_t *buf;   // QBUF entries
-    uint32_t *n;     // entries held
-    uint32_t *base;  // broadcast of the reserved global offset
-};
-
-// Wave-aggregated append to the workgroup's LDS buffer.  The caller
-// guarantees room for one push per lane since the last flush check.
-__device__ __forceinline__ void lq_push(const LdsQueue &q, bool p, uint32_t v)
-{
-    const uint64_t act = __ballot(1);
-    const uint64_t m = __ballot(p);
-    if (!m) return;
-    const int leader = __ffsll((unsigned long long)act) - 1;
-    uint32_t b = 0;
-    if ((int)wlane() == leader) b = atomicAdd(q.n, (uint32_t)__popcll(m));
-    b = __shfl(b, leader);
-    if (p) q.buf[b + (uint32_t)__popcll(m & ((1ull << wlane()) - 1))] = v;
-}
-
-// Flush when `force` or when another round of blockDim pushes might not
-// fit.  Must be reached by every thread of the workgroup.
-__device__ __forceinline__ void lq_flush(const LdsQueue &q, uint32_t *gcnt, uint32_t *gout, bool force)
-{
-    __syncthreads();
-    const uint32_t n = *q.n;
-    if (n == 0 || (!force && n + blockDim.x <= QBUF)) return;
-    if (threadIdx.x == 0) *q.base = atomicAdd(gcnt, n);
-    __syncthreads();
-    const uint32_t b = *q.base;
-    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) gout[b + i] = q.buf[i];
-    __syncthreads();
-    if (threadIdx.x == 0) *q.n = 0;
-    __syncthreads();
-}
-
-#define DECLARE_LDS_QUEUE(q)                                                                 \
-    __shared__ uint32_t q##_buf[QBUF];                                                       \
-    __shared__ uint32_t q##_n, q##_base;                                                     \
-    if (threadIdx.x == 0) q##_n = 0;                                                         \
-    __syncthreads();                                                                         \
-    const LdsQueue q{q##_buf, &q##_n, &q##_base}
 
 // W lanes per frontier vertex; W = 1 is the vertex-per-thread mapping of
 // fwd/bwd, W > 1 the virtual warp of fwd_warp<W> (lanes stride the list).
@@ -274,33 +229,41 @@ static void launch_lb(hipStream_t s, const LevelArgs &a, bool single, uint32_t b
 constexpr uint32_t BU_SHORT = 4;
 constexpr uint32_t BU_GROUP = 8;
 
+// One 64-bit load gives p's visited-or-finished and finished bits.
 template <bool SINGLE>
-__device__ __forceinline__ bool visited_parent(const uint32_t *vis, const uint32_t *dbits,
-                                               const uint32_t *cls, uint32_t p, uint32_t c)
+__device__ __forceinline__ bool visited_parent(const uint32_t *vis, const uint32_t *cls, uint32_t p, uint32_t c)
 {
-    const uint32_t wi = p >> 5, pb = 1u << (p & 31);
-    return (vis[wi] & ~dbits[wi] & pb) && (SINGLE || cls[p] == c);
+    const uint64_t w = reinterpret_cast<const uint64_t *>(vis)[p >> 5];
+    const uint32_t pb = 1u << (p & 31);
+    return ((uint32_t)w & ~(uint32_t)(w >> 32) & pb) && (SINGLE || cls[p] == c);
 }
 
+// Both directions of a level share one launch: the first `bpd` workgroups
+// run direction a.d[0], the rest a.d[1].
 template <bool SINGLE>
-__global__ __launch_bounds__(256) void expand_bu_kernel(DirArgs D, const uint32_t *__restrict__ PR,
-                                                        const uint32_t *__restrict__ PC,
-                                                        const uint32_t *__restrict__ dbits,
-                                                        const uint32_t *cls, uint32_t V, int L)
+__global__ __launch_bounds__(256) void expand_bu_kernel(BottomUpArgs a, uint32_t bpd)
 {
     __shared__ uint32_t s_long[256];
     __shared__ uint32_t s_nlong;
+    const int dir = (a.ndirs == 2 && blockIdx.x >= bpd) ? 1 : 0;
+    const uint32_t bid = blockIdx.x - (uint32_t)dir * bpd;
+    const DirArgs &D = a.d[dir];
+    const uint32_t *__restrict__ PR = a.PR[dir];
+    const uint32_t *__restrict__ PC = a.PC[dir];
+    const uint32_t *cls = a.cls;
+    const uint32_t V = a.V;
+    const int L = a.level;
     uint32_t *cnt_out = &D.cnt[CT((L + 1) % 3)];
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
+    if (bid == 0 && threadIdx.x == 0) {
         D.cnt[CT((L + 2) % 3)] = 0;
         *D.bigc_zero = 0ull;
     }
     if (threadIdx.x == 0) s_nlong = 0;
     DECLARE_LDS_QUEUE(q);
-    for (uint32_t base = blockIdx.x * 256; base < V; base += gridDim.x * 256) {
+    for (uint32_t base = bid * 256; base < V; base += bpd * 256) {
         const uint32_t u = base + threadIdx.x + 1;
         bool found = false;
-        if (u <= V && !(D.vis[u >> 5] & (1u << (u & 31)))) {
+        if (u <= V && !(D.vis[vword(u)] & (1u << (u & 31)))) {
             const uint32_t c = SINGLE ? 0u : cls[u];
             const uint32_t b = PR[u], e = PR[u + 1];
             uint32_t p[BU_SHORT];
@@ -308,9 +271,9 @@ __global__ __launch_bounds__(256) void expand_bu_kernel(DirArgs D, const uint32_
             for (uint32_t t = 0; t < BU_SHORT; t++) p[t] = b + t < e ? PC[b + t] : 0u;
 #pragma unroll
             for (uint32_t t = 0; t < BU_SHORT; t++)
-                found |= b + t < e && visited_parent<SINGLE>(D.vis, dbits, cls, p[t], c);
+                found |= b + t < e && visited_parent<SINGLE>(D.vis, cls, p[t], c);
             if (!found && e - b > BU_SHORT) s_long[atomicAdd(&s_nlong, 1u)] = u;
-            if (found) found = !(atomicOr(&D.vis[u >> 5], 1u << (u & 31)) & (1u << (u & 31)));
+            if (found) found = !(atomicOr(&D.vis[vword(u)], 1u << (u & 31)) & (1u << (u & 31)));
         }
         lq_push(q, found, u);
         __syncthreads();
@@ -326,12 +289,12 @@ __global__ __launch_bounds__(256) void expand_bu_kernel(DirArgs D, const uint32_
             bool hit = false;
             for (uint32_t j0 = PR[x] + BU_SHORT; j0 < e && !hit; j0 += BU_GROUP) {
                 const uint32_t j = j0 + sub;
-                const bool h = j < e && visited_parent<SINGLE>(D.vis, dbits, cls, PC[j], c);
+                const bool h = j < e && visited_parent<SINGLE>(D.vis, cls, PC[j], c);
                 hit = ((__ballot(h) >> gshift) & ((1ull << BU_GROUP) - 1)) != 0;
             }
             bool claimed = false;
             if (hit && sub == 0)
-                claimed = !(atomicOr(&D.vis[x >> 5], 1u << (x & 31)) & (1u << (x & 31)));
+                claimed = !(atomicOr(&D.vis[vword(x)], 1u << (x & 31)) & (1u << (x & 31)));
             lq_push(q, claimed, x);
         }
         __syncthreads();
@@ -341,15 +304,15 @@ __global__ __launch_bounds__(256) void expand_bu_kernel(DirArgs D, const uint32_
     lq_flush(q, cnt_out, D.qout, true);
 }
 
-void k_bottom_up(hipStream_t s, const DirArgs &d, const uint32_t *PR, const uint32_t *PC,
-                 const uint32_t *dbits, const uint32_t *cls, uint32_t V, int level, bool single_class)
+void k_bottom_up(hipStream_t s, const BottomUpArgs &a, bool single_class)
 {
-    uint64_t b = ((uint64_t)V + 256) / 256;
-    const uint32_t grid = (uint32_t)(b > 2048 ? 2048 : b);
+    uint64_t b = ((uint64_t)a.V + 256) / 256;
+    const uint32_t bpd = (uint32_t)(b > 2048 ? 2048 : b);
+    const uint32_t grid = bpd * (uint32_t)a.ndirs;
     if (single_class)
-        expand_bu_kernel<true><<<grid, 256, 0, s>>>(d, PR, PC, dbits, cls, V, level);
+        expand_bu_kernel<true><<<grid, 256, 0, s>>>(a, bpd);
     else
-        expand_bu_kernel<false><<<grid, 256, 0, s>>>(d, PR, PC, dbits, cls, V, level);
+        expand_bu_kernel<false><<<grid, 256, 0, s>>>(a, bpd);
 }
 
 template <int W>

@@ -4,6 +4,7 @@
 // consecutive vertices per step, so ballots and cross-lane reductions always
 // see a full wave.  Class and state updates that other lanes may race on use
 // atomics (the reference's byte read-modify-writes, scc.h:28-42, are not).
+#include "lds_queue.h"
 #include "scc_dev.h"
 
 namespace sccamd {
@@ -26,9 +27,10 @@ __device__ __forceinline__ uint64_t pivot_key(const uint32_t *__restrict__ Fr,
 
 __device__ __forceinline__ uint32_t key_vertex(uint64_t k) { return 0xFFFFFFFFu - (uint32_t)k; }
 
-__device__ __forceinline__ bool bit_of(const uint32_t *bm, uint32_t v)
+// visited (or finished) bit of v in fvis / bvis
+__device__ __forceinline__ bool bit_of(const uint32_t *vis, uint32_t v)
 {
-    return (bm[v >> 5] >> (v & 31)) & 1u;
+    return (vis[vword(v)] >> (v & 31)) & 1u;
 }
 
 // Sum over the whole workgroup, added to *p by one atomic.  Every thread of
@@ -107,37 +109,132 @@ void k_init_state(hipStream_t s, const DevGraph &g, DevState &st, const uint8_t 
 // scc_kernels.cu:187-233: a live vertex with no live same-class
 // in-neighbour, or no live same-class out-neighbour, is a singleton SCC.
 
-__global__ void trim1_kernel(const uint32_t *__restrict__ Fr, const uint32_t *__restrict__ Fc,
-                             const uint32_t *__restrict__ Br, const uint32_t *__restrict__ Bc,
-                             uint32_t *cls, uint32_t *lab, uint32_t V, uint32_t *ctr)
+//
+// The first pass sweeps every vertex; each later pass re-checks only the
+// live neighbours of the vertices the previous pass removed (a vertex can
+// only become removable when a neighbour goes), so the fixpoint costs one
+// sweep plus work proportional to the removals.  A candidate is tested once
+// per pass (the first lane to stamp it with the pass's epoch), and removal
+// is a CAS on the class word, so each removal is listed once.  Pass k reads list count C_TLIST + k%3, appends
+// to C_TLIST + (k+1)%3 and clears C_TLIST + (k+2)%3.
+
+// v (live, class c) keeps a live same-class in-neighbour and out-neighbour
+__device__ __forceinline__ bool trim1_keep(const uint32_t *__restrict__ Fr, const uint32_t *__restrict__ Fc,
+                                           const uint32_t *__restrict__ Br, const uint32_t *__restrict__ Bc,
+                                           const uint32_t *cls, uint32_t v, uint32_t c)
 {
+    bool keep = false;
+    for (uint32_t i = Br[v], e = Br[v + 1]; i < e; i++)
+        if (cls[Bc[i]] == c) { keep = true; break; }
+    if (!keep) return false;
+    for (uint32_t i = Fr[v], e = Fr[v + 1]; i < e; i++)
+        if (cls[Fc[i]] == c) return true;
+    return false;
+}
+
+__device__ __forceinline__ bool trim1_remove(uint32_t *cls, uint32_t *lab, uint32_t v, uint32_t c)
+{
+    if (atomicCAS(&cls[v], c, DEAD) != c) return false;
+    lab[v] = v | LAB_FINAL;
+    return true;
+}
+
+// Removals are staged in LDS (one global atomic per flush): a sweep can
+// remove millions of vertices.
+__global__ __launch_bounds__(256) void trim1_kernel(const uint32_t *__restrict__ Fr,
+                                                    const uint32_t *__restrict__ Fc,
+                                                    const uint32_t *__restrict__ Br,
+                                                    const uint32_t *__restrict__ Bc, uint32_t *cls,
+                                                    uint32_t *lab, uint32_t V, uint32_t *ctr, uint32_t *out)
+{
+    DECLARE_LDS_QUEUE(q);
     uint32_t removed = 0;
-    WAVE_LOOP(v, valid, V)
-    {
-        if (!valid) continue;
-        const uint32_t c = cls[v];
-        if (c == DEAD) continue;
-        bool keep = false;
-        for (uint32_t i = Br[v], e = Br[v + 1]; i < e; i++)
-            if (cls[Bc[i]] == c) { keep = true; break; }
-        if (keep) {
-            keep = false;
-            for (uint32_t i = Fr[v], e = Fr[v + 1]; i < e; i++)
-                if (cls[Fc[i]] == c) { keep = true; break; }
+    uint32_t *cnt_out = &ctr[CT(C_TLIST + 1)];
+    if (blockIdx.x == 0 && threadIdx.x == 0) ctr[CT(C_TLIST + 2)] = 0;
+    for (uint32_t base = blockIdx.x * 256; base < V; base += gridDim.x * 256) {
+        const uint32_t v = base + threadIdx.x + 1;
+        bool gone = false;
+        if (v <= V) {
+            const uint32_t c = cls[v];
+            if (c != DEAD && !trim1_keep(Fr, Fc, Br, Bc, cls, v, c)) gone = trim1_remove(cls, lab, v, c);
         }
-        if (!keep) {
-            lab[v] = v | LAB_FINAL;
-            __hip_atomic_store(&cls[v], DEAD, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            removed++;
-        }
+        removed += gone;
+        lq_push(q, gone, v);
+        lq_flush(q, cnt_out, out, false);
     }
+    lq_flush(q, cnt_out, out, true);
     wave_add32(&ctr[CT(C_TRIM1)], removed);
 }
 
-void k_trim1(hipStream_t s, const DevGraph &g, DevState &st)
+// TRIM_W lanes per removed vertex walk its out- and in-arcs, one arc per
+// lane per round, and test each live neighbour; the block runs as many
+// rounds as its longest list needs, so every lane reaches each flush.
+constexpr uint32_t TRIM_W = 8;
+constexpr uint32_t TRIM_PER_BLOCK = 256 / TRIM_W;
+
+__global__ __launch_bounds__(256) void trim1_list_kernel(const uint32_t *__restrict__ Fr,
+                                                         const uint32_t *__restrict__ Fc,
+                                                         const uint32_t *__restrict__ Br,
+                                                         const uint32_t *__restrict__ Bc, uint32_t *cls,
+                                                         uint32_t *lab, uint32_t *ctr, const uint32_t *in,
+                                                         uint32_t *out, uint32_t *stamp, uint32_t epoch,
+                                                         uint32_t pass)
+{
+    __shared__ uint32_t s_rounds;
+    DECLARE_LDS_QUEUE(q);
+    const uint32_t n = ctr[CT(C_TLIST + pass % 3)];
+    uint32_t *cnt_out = &ctr[CT(C_TLIST + (pass + 1) % 3)];
+    if (blockIdx.x == 0 && threadIdx.x == 0) ctr[CT(C_TLIST + (pass + 2) % 3)] = 0;
+    const uint32_t sub = threadIdx.x % TRIM_W;
+    uint32_t removed = 0;
+    for (uint32_t base = blockIdx.x * TRIM_PER_BLOCK; base < n; base += gridDim.x * TRIM_PER_BLOCK) {
+        const uint32_t i = base + threadIdx.x / TRIM_W;
+        uint32_t fb = 0, nf = 0, bb = 0, total = 0;
+        if (i < n) {
+            const uint32_t v = in[i];
+            fb = Fr[v];
+            nf = Fr[v + 1] - fb;
+            bb = Br[v];
+            total = nf + (Br[v + 1] - bb);
+        }
+        if (threadIdx.x == 0) s_rounds = 0;
+        __syncthreads();
+        if (sub == 0 && total) atomicMax(&s_rounds, (total + TRIM_W - 1) / TRIM_W);
+        __syncthreads();
+        const uint32_t rounds = s_rounds;
+        for (uint32_t r = 0; r < rounds; r++) {
+            const uint32_t j = r * TRIM_W + sub;
+            bool gone = false;
+            uint32_t w = 0;
+            if (j < total) {
+                w = j < nf ? Fc[fb + j] : Bc[bb + (j - nf)];
+                const uint32_t c = cls[w];
+                if (c != DEAD && stamp[w] != epoch && atomicExch(&stamp[w], epoch) != epoch &&
+                    !trim1_keep(Fr, Fc, Br, Bc, cls, w, c))
+                    gone = trim1_remove(cls, lab, w, c);
+            }
+            removed += gone;
+            lq_push(q, gone, w);
+            lq_flush(q, cnt_out, out, false);
+        }
+    }
+    lq_flush(q, cnt_out, out, true);
+    wave_add32(&ctr[CT(C_TRIM1)], removed);
+}
+
+void k_trim1(hipStream_t s, const DevGraph &g, DevState &st, uint32_t *out)
 {
     trim1_kernel<<<sweep_blocks(g.V), SWEEP_BLOCK, 0, s>>>(g.Fr, g.Fc, g.Br, g.Bc, st.cls, st.lab,
-                                                           g.V, st.ctr);
+                                                           g.V, st.ctr, out);
+}
+
+void k_trim1_list(hipStream_t s, const DevGraph &g, DevState &st, const uint32_t *in, uint32_t *out,
+                  uint32_t epoch, uint32_t pass, uint32_t n)
+{
+    uint64_t b = ((uint64_t)n * TRIM_W + SWEEP_BLOCK - 1) / SWEEP_BLOCK;
+    const uint32_t grid = (uint32_t)(b < 1 ? 1 : (b > 2048 ? 2048 : b));
+    trim1_list_kernel<<<grid, SWEEP_BLOCK, 0, s>>>(g.Fr, g.Fc, g.Br, g.Bc, st.cls, st.lab, st.ctr, in,
+                                                   out, st.stamp, epoch, pass);
 }
 
 // ---------------------------------------------------------- trim-2 ---
@@ -207,10 +304,11 @@ void k_trim2(hipStream_t s, const DevGraph &g, DevState &st)
 
 // ------------------------------------------------- visited bitmaps ---
 // Bits of finished vertices (and of slot 0 and the padding) start set in
-// both visited bitmaps; dbits keeps the finished set alone so a bottom-up
-// pass can tell "visited" from "finished".  Counts the live vertices.
+// the visited-or-finished words of both bitmaps; the finished words beside
+// them keep the finished set alone so a bottom-up pass can tell "visited"
+// from "finished".  Counts the live vertices.
 __global__ void init_vis_kernel(const uint32_t *__restrict__ cls, uint32_t *fvis, uint32_t *bvis,
-                                uint32_t *dbits, uint32_t *ctr, uint32_t V, uint32_t nchunks)
+                                uint32_t *ctr, uint32_t V, uint32_t nchunks)
 {
     const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const uint32_t nwaves = (gridDim.x * blockDim.x) >> 6;
@@ -220,9 +318,10 @@ __global__ void init_vis_kernel(const uint32_t *__restrict__ cls, uint32_t *fvis
         bool dead = v > V || cls[v] == DEAD;
         live += !dead;
         uint64_t m = __ballot(dead);
-        uint32_t w = (lane() & 1) ? (uint32_t)(m >> 32) : (uint32_t)m;
-        uint32_t *dst = lane() < 2 ? fvis : (lane() < 4 ? bvis : dbits);
-        if (lane() < 6) dst[2 * ch + (lane() & 1)] = w;
+        // the chunk's two groups, each as (visited-or-finished, finished)
+        const uint32_t w = (lane() & 2) ? (uint32_t)(m >> 32) : (uint32_t)m;
+        uint32_t *dst = lane() < 4 ? fvis : bvis;
+        if (lane() < 8) dst[4 * ch + (lane() & 3)] = w;
     }
     wave_add32(&ctr[CT(C_NLIVE)], live);
 }
@@ -232,8 +331,7 @@ void k_init_vis(hipStream_t s, const DevGraph &g, DevState &st)
     uint32_t nchunks = st.nwords / 2;
     uint32_t blocks = (nchunks + 3) / 4;
     blocks = blocks < 1 ? 1 : (blocks > 4096 ? 4096 : blocks);
-    init_vis_kernel<<<blocks, SWEEP_BLOCK, 0, s>>>(st.cls, st.fvis, st.bvis, st.dbits, st.ctr, g.V,
-                                                  nchunks);
+    init_vis_kernel<<<blocks, SWEEP_BLOCK, 0, s>>>(st.cls, st.fvis, st.bvis, st.ctr, g.V, nchunks);
 }
 
 // ------------------------------------------------------------ pivots ---
@@ -317,10 +415,10 @@ __global__ void select_kernel(const uint32_t *__restrict__ Fr, const uint32_t *_
         uint32_t i = atomicAdd(&ctr[CT(C_NPIV)], 1u);
         pivlist[i] = v;
         qf[i] = v;
-        atomicOr(&fvis[v >> 5], 1u << (v & 31));
+        atomicOr(&fvis[vword(v)], 1u << (v & 31));
         if (seed_bw) {
             qb[i] = v;
-            atomicOr(&bvis[v >> 5], 1u << (v & 31));
+            atomicOr(&bvis[vword(v)], 1u << (v & 31));
         }
     }
 }
@@ -359,7 +457,7 @@ void k_select(hipStream_t s, const DevGraph &g, DevState &st, int seed_bw)
 __global__ void seed_single_kernel(uint32_t *vis, uint32_t *q, uint32_t *ctr, uint32_t src, int dir)
 {
     q[0] = src;
-    vis[src >> 5] |= 1u << (src & 31);
+    vis[vword(src)] |= 1u << (src & 31);
     for (int i = 0; i < 3; i++) ctr[CT(C_FWCNT + i)] = ctr[CT(C_BWCNT + i)] = 0;
     ctr[CT((dir ? C_BWCNT : C_FWCNT) + 0)] = 1;
     zero_hub_counts(ctr);
@@ -579,7 +677,7 @@ __global__ void color_roots_kernel(uint32_t *cls, const uint32_t *__restrict__ a
             uint32_t i = atomicAdd(&ctr[CT(C_NPIV)], 1u);
             pivlist[i] = v;
             qf[i] = v;
-            atomicOr(&fvis[v >> 5], 1u << (v & 31));
+            atomicOr(&fvis[vword(v)], 1u << (v & 31));
         }
     }
 }

@@ -1,0 +1,58 @@
+// lds_queue.h -- workgroup-staged append to a global list.
+//
+// Same-address atomics serialise (about 10 ns each on MI355X), so a kernel
+// that appends many items stages them in LDS and reserves global space
+// with one atomic per flush (one per ~QBUF items) instead of one per wave.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace sccamd {
+
+constexpr uint32_t QBUF = 2048;
+
+struct LdsQueue {
+    uint32_t *buf;   // QBUF entries
+    uint32_t *n;     // entries held
+    uint32_t *base;  // broadcast of the reserved global offset
+};
+
+// Wave-aggregated append to the workgroup's LDS buffer.  The caller
+// guarantees room for one push per lane since the last flush check.
+__device__ __forceinline__ void lq_push(const LdsQueue &q, bool p, uint32_t v)
+{
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint64_t act = __ballot(1);
+    const uint64_t m = __ballot(p);
+    if (!m) return;
+    const int leader = __ffsll((unsigned long long)act) - 1;
+    uint32_t b = 0;
+    if ((int)lane == leader) b = atomicAdd(q.n, (uint32_t)__popcll(m));
+    b = __shfl(b, leader);
+    if (p) q.buf[b + (uint32_t)__popcll(m & ((1ull << lane) - 1))] = v;
+}
+
+// Flush when `force` or when another round of blockDim pushes might not
+// fit.  Must be reached by every thread of the workgroup.
+__device__ __forceinline__ void lq_flush(const LdsQueue &q, uint32_t *gcnt, uint32_t *gout, bool force)
+{
+    __syncthreads();
+    const uint32_t n = *q.n;
+    if (n == 0 || (!force && n + blockDim.x <= QBUF)) return;
+    if (threadIdx.x == 0) *q.base = atomicAdd(gcnt, n);
+    __syncthreads();
+    const uint32_t b = *q.base;
+    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) gout[b + i] = q.buf[i];
+    __syncthreads();
+    if (threadIdx.x == 0) *q.n = 0;
+    __syncthreads();
+}
+
+#define DECLARE_LDS_QUEUE(q)                                                                 \
+    __shared__ uint32_t q##_buf[QBUF];                                                       \
+    __shared__ uint32_t q##_n, q##_base;                                                     \
+    if (threadIdx.x == 0) q##_n = 0;                                                         \
+    __syncthreads();                                                                         \
+    const LdsQueue q{q##_buf, &q##_n, &q##_base}
+
+} // namespace sccamd

@@ -31,6 +31,12 @@ constexpr uint32_t LAB_FINAL = 0x80000000u;
 constexpr int SWEEP_BLOCK = 256;
 constexpr uint32_t LB_BIG = 512;   // degree from which a frontier vertex is a hub (LB mode)
 
+// Visited bitmaps (fvis, bvis) keep two words per group of 32 vertices: the
+// first has a bit for every visited or finished vertex, the second one for
+// every finished vertex, so a bottom-up parent probe reads both with one
+// 64-bit load.
+__host__ __device__ __forceinline__ uint32_t vword(uint32_t v) { return (v >> 5) << 1; }
+
 // Counters block: counter i lives at word CT(i), one 128-byte line per
 // counter, so atomics on different counters never share a line (same-line
 // atomics from every workgroup serialise on one L2 channel).  64-bit
@@ -58,6 +64,7 @@ enum : int {
     C_PIV0 = 30,     // first pivot of the latest election
     C_ANOM = 31,     // invariant violations caught by kernel guards (must stay 0)
     C_NLIVE = 32,    // live vertices when the visited bitmaps were initialised
+    C_TLIST = 33,    // 3 rotating counts of trim-1 removal lists (33..35)
     C_COUNT = 48
 };
 
@@ -69,15 +76,15 @@ struct DevGraph {
 struct DevState {
     uint32_t *cls = nullptr, *lab = nullptr, *aux = nullptr;
     uint32_t *fvis = nullptr, *bvis = nullptr;
-    uint32_t *dbits = nullptr;     // finished vertices (same layout as fvis/bvis)
     uint64_t *piv = nullptr;
     uint32_t *qf[2] = {nullptr, nullptr}, *qb[2] = {nullptr, nullptr};
     uint32_t *pivlist = nullptr;   // pivots of the current round
+    uint32_t *stamp = nullptr;     // trim-1 list passes: epoch of the last test of each vertex
     uint32_t *bigf[3] = {}, *bigb[3] = {};           // hub lists, rotating like the counts
     uint32_t *bigoff_f[3] = {}, *bigoff_b[3] = {};
     uint32_t *part = nullptr;      // partition ids (detectSCC)
     uint32_t *ctr = nullptr;       // C_COUNT words
-    uint32_t nwords = 0;           // bitmap words
+    uint32_t nwords = 0;           // words of a one-bit-per-vertex bitmap (fvis/bvis hold twice that)
 };
 
 // One search direction of a BFS level.
@@ -99,6 +106,15 @@ struct DirArgs {
     unsigned long long *bigc_zero;   // the list the next level appends to, cleared here
 };
 
+struct BottomUpArgs {
+    DirArgs d[2];
+    const uint32_t *PR[2], *PC[2];   // parent lists of each direction (the opposite CSR)
+    const uint32_t *cls;
+    uint32_t V;
+    int ndirs;
+    int level;
+};
+
 struct LevelArgs {
     DirArgs d[2];
     const uint32_t *cls;
@@ -109,7 +125,13 @@ struct LevelArgs {
 // ---- sweep kernels (kernels_sweep.hip) ----
 void k_fill_u32(hipStream_t s, uint32_t *p, uint32_t val, uint64_t n);
 void k_init_state(hipStream_t s, const DevGraph &g, DevState &st, const uint8_t *dead_dev);
-void k_trim1(hipStream_t s, const DevGraph &g, DevState &st);
+// trim-1 sweep over every vertex; removed vertices are listed in `out`
+void k_trim1(hipStream_t s, const DevGraph &g, DevState &st, uint32_t *out);
+// trim-1 pass `pass` >= 1 over the live neighbours of the vertices listed
+// in `in` (n of them, counted at C_TLIST + pass%3); removals go to `out`;
+// `epoch` is new for every pass of the context (st.stamp never holds it)
+void k_trim1_list(hipStream_t s, const DevGraph &g, DevState &st, const uint32_t *in, uint32_t *out,
+                  uint32_t epoch, uint32_t pass, uint32_t n);
 void k_trim2(hipStream_t s, const DevGraph &g, DevState &st);
 void k_init_vis(hipStream_t s, const DevGraph &g, DevState &st);
 // single_class: every live vertex is in class 0 (the first pivot)
@@ -134,10 +156,9 @@ void k_export_removed(hipStream_t s, const DevGraph &g, DevState &st, uint8_t *o
 // expand: SCC_EXPAND_VERTEX / VWARP / LB; warp = lanes per vertex for VWARP
 void k_expand_level(hipStream_t s, const LevelArgs &a, int expand, int warp, bool single_class,
                     uint32_t blocks_per_dir);
-// One bottom-up pass of direction d.dir: every live unvisited vertex scans
-// its parents (PR/PC = the opposite CSR) for a visited one of its class.
-void k_bottom_up(hipStream_t s, const DirArgs &d, const uint32_t *PR, const uint32_t *PC,
-                 const uint32_t *dbits, const uint32_t *cls, uint32_t V, int level, bool single_class);
+// One bottom-up pass of each listed direction: every live unvisited vertex
+// scans its parents (PR/PC = the opposite CSR) for a visited one of its class.
+void k_bottom_up(hipStream_t s, const BottomUpArgs &a, bool single_class);
 
 uint32_t sweep_blocks(uint32_t V);
 

@@ -126,7 +126,8 @@ typedef struct {
     int expand;       /* SCC_EXPAND_* */
     int block;        /* reserved, 0 */
     int check_every;  /* BFS levels between host checks, 0 = auto */
-    int sizes;        /* 1: also compute stats.largest (one more sweep) */
+    int sizes;        /* 1: also compute stats.largest and the first search's
+                         vertex/arc counts (two more sweeps) */
 } scc_opts;
 
 void scc_opts_default(scc_opts *o);
@@ -152,8 +153,8 @@ typedef struct {
     uint32_t rounds;         /* outer rounds of the main loop */
     uint32_t levels;         /* BFS levels launched, both directions */
     uint32_t first_pivot;    /* vertex elected by the first-pivot rule */
-    uint64_t first_fw_vertices, first_fw_arcs; /* |R|, |E_R| of the first FW */
-    uint64_t first_bw_vertices, first_bw_arcs; /* |R|, |E_R| of the first BW */
+    uint64_t first_fw_vertices, first_fw_arcs; /* |R|, |E_R| of the first FW (opts.sizes) */
+    uint64_t first_bw_vertices, first_bw_arcs; /* |R|, |E_R| of the first BW (opts.sizes) */
     double ms_scc;           /* CSR resident -> labels resident (device clock) */
     double ms_first_reach;   /* first-pivot FW||BW (device clock) */
     double ms_phase[SCC_PHASE_COUNT];
