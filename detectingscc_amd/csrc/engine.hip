@@ -435,12 +435,15 @@ scc_status Engine::reach(bool fw, bool bw, bool single, int expand, int W, uint3
 // trim-1 to its fixpoint (scc_WCC.cu:680-687), or one pass.  After the
 // first sweep each pass re-checks the neighbours of the previous pass's
 // removals (listed alternately in the FW queue buffers, which are free
-// outside a search) until a pass removes nothing.
+// outside a search) until a pass removes nothing; a pass that follows one
+// removing more than 1/kTrimSweepShare of the vertices sweeps instead.
+static constexpr uint32_t kTrimSweepShare = 32;
+
 scc_status Engine::trim1(bool fixpoint)
 {
     mark(SCC_PHASE_TRIM1);
     zero_ctr(C_TLIST, 3);
-    k_trim1(stream, g, st, st.qf[0]);
+    k_trim1(stream, g, st, st.qf[0], 0);
     scc_status s = read_ctr();
     if (s != SCC_OK) return s;
     for (uint32_t pass = 1; fixpoint; pass++) {
@@ -448,11 +451,15 @@ scc_status Engine::trim1(bool fixpoint)
         if (!n) break;
         // every pass but the last removes a vertex
         if (pass > limit()) return no_convergence("trim-1");
-        if (++trim_epoch == 0) {   // wrapped: no stale stamp may match a new epoch
-            k_fill_u32(stream, st.stamp, 0u, (uint64_t)g.V + 1);
-            trim_epoch = 1;
+        if (n > g.V / kTrimSweepShare) {
+            k_trim1(stream, g, st, st.qf[pass & 1], pass);
+        } else {
+            if (++trim_epoch == 0) {   // wrapped: no stale stamp may match a new epoch
+                k_fill_u32(stream, st.stamp, 0u, (uint64_t)g.V + 1);
+                trim_epoch = 1;
+            }
+            k_trim1_list(stream, g, st, st.qf[(pass + 1) & 1], st.qf[pass & 1], trim_epoch, pass, n);
         }
-        k_trim1_list(stream, g, st, st.qf[(pass + 1) & 1], st.qf[pass & 1], trim_epoch, pass, n);
         if ((s = read_ctr()) != SCC_OK) return s;
     }
     return SCC_OK;

@@ -113,7 +113,9 @@ void k_init_state(hipStream_t s, const DevGraph &g, DevState &st, const uint8_t 
 // The first pass sweeps every vertex; each later pass re-checks only the
 // live neighbours of the vertices the previous pass removed (a vertex can
 // only become removable when a neighbour goes), so the fixpoint costs one
-// sweep plus work proportional to the removals.  A candidate is tested once
+// sweep plus work proportional to the removals (while a pass removes a
+// large share of the vertices, the next one sweeps again: walking the arcs
+// of millions of fresh removals costs more than a sweep).  A candidate is tested once
 // per pass (the first lane to stamp it with the pass's epoch), and removal
 // is a CAS on the class word, so each removal is listed once.  Pass k reads list count C_TLIST + k%3, appends
 // to C_TLIST + (k+1)%3 and clears C_TLIST + (k+2)%3.
@@ -145,12 +147,13 @@ __global__ __launch_bounds__(256) void trim1_kernel(const uint32_t *__restrict__
                                                     const uint32_t *__restrict__ Fc,
                                                     const uint32_t *__restrict__ Br,
                                                     const uint32_t *__restrict__ Bc, uint32_t *cls,
-                                                    uint32_t *lab, uint32_t V, uint32_t *ctr, uint32_t *out)
+                                                    uint32_t *lab, uint32_t V, uint32_t *ctr, uint32_t *out,
+                                                    uint32_t pass)
 {
     DECLARE_LDS_QUEUE(q);
     uint32_t removed = 0;
-    uint32_t *cnt_out = &ctr[CT(C_TLIST + 1)];
-    if (blockIdx.x == 0 && threadIdx.x == 0) ctr[CT(C_TLIST + 2)] = 0;
+    uint32_t *cnt_out = &ctr[CT(C_TLIST + (pass + 1) % 3)];
+    if (blockIdx.x == 0 && threadIdx.x == 0) ctr[CT(C_TLIST + (pass + 2) % 3)] = 0;
     for (uint32_t base = blockIdx.x * 256; base < V; base += gridDim.x * 256) {
         const uint32_t v = base + threadIdx.x + 1;
         bool gone = false;
@@ -222,10 +225,10 @@ __global__ __launch_bounds__(256) void trim1_list_kernel(const uint32_t *__restr
     wave_add32(&ctr[CT(C_TRIM1)], removed);
 }
 
-void k_trim1(hipStream_t s, const DevGraph &g, DevState &st, uint32_t *out)
+void k_trim1(hipStream_t s, const DevGraph &g, DevState &st, uint32_t *out, uint32_t pass)
 {
     trim1_kernel<<<sweep_blocks(g.V), SWEEP_BLOCK, 0, s>>>(g.Fr, g.Fc, g.Br, g.Bc, st.cls, st.lab,
-                                                           g.V, st.ctr, out);
+                                                           g.V, st.ctr, out, pass);
 }
 
 void k_trim1_list(hipStream_t s, const DevGraph &g, DevState &st, const uint32_t *in, uint32_t *out,
