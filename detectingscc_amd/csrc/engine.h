@@ -71,6 +71,7 @@ class Engine {
     int check_every = 0;
     bool search_stats = false;        // count |R| and |E_R| of the first search
     uint32_t trim_epoch = 0;          // last epoch stamped by a trim-1 list pass
+    bool trace_levels = false;        // print each reachability decision to stderr
     std::vector<hipEvent_t> events;
     std::vector<int> mark_phase;
     size_t nmarks = 0;

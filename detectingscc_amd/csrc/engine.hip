@@ -81,6 +81,9 @@ scc_status Engine::init(int dev)
     HIP_TRY(hipHostMalloc((void **)&h_ctr, CT(C_COUNT) * sizeof(uint32_t), hipHostMallocDefault));
     HIP_TRY(hipMalloc((void **)&st.ctr, CT(C_COUNT) * sizeof(uint32_t)));
     HIP_TRY(hipMemset(st.ctr, 0, CT(C_COUNT) * sizeof(uint32_t)));
+    // diagnostic: SCC_TRACE_LEVELS=1 prints every host decision of a search
+    const char *tr = std::getenv("SCC_TRACE_LEVELS");
+    trace_levels = tr && tr[0] == '1';
     return SCC_OK;
 }
 
@@ -421,6 +424,11 @@ scc_status Engine::reach(bool fw, bool bw, bool single, int expand, int W, uint3
             nf_max = std::max(nf_max, nf);
         }
         counted_seeds = true;
+        if (trace_levels)
+            std::fprintf(stderr, "[scc reach] level %u: fw %s %u, bw %s %u, live %lld, batch next %d\n",
+                         level, bottom_up[0] ? "BU" : "TD", active[0] ? h_ctr[CT(C_FWCNT + level % 3)] : 0u,
+                         bottom_up[1] ? "BU" : "TD", active[1] ? h_ctr[CT(C_BWCNT + level % 3)] : 0u,
+                         (long long)nlive, spec ? kSpecLevels : 1);
         if (!any) break;
         // every level claims at least one vertex, so V+1 levels bound a search
         if (level > limit()) return no_convergence("reachability");
