@@ -226,8 +226,16 @@ static void launch_lb(hipStream_t s, const LevelArgs &a, bool single, uint32_t b
 // independent loads; vertices with longer parent lists and no hit yet are
 // listed in LDS and scanned by groups of BU_GROUP lanes, BU_GROUP parents
 // per step, with a ballot to stop each group at its first hit.
+//
+// The pass is wave-centric: each wave streams its own 64-vertex chunks with
+// a private LDS queue and long list and never waits on its workgroup, so
+// the waves of a CU overlap their chains of dependent loads (vertex word,
+// row offsets, parent ids, parent bits).  A wave whose queue fills reserves
+// global space itself; what remains at the end goes out with one atomic per
+// workgroup (same-address atomics serialise).
 constexpr uint32_t BU_SHORT = 4;
 constexpr uint32_t BU_GROUP = 8;
+constexpr uint32_t BU_WQ = 1024;
 
 // One 64-bit load gives p's visited-or-finished and finished bits.
 template <bool SINGLE>
@@ -238,13 +246,41 @@ __device__ __forceinline__ bool visited_parent(const uint32_t *vis, const uint32
     return ((uint32_t)w & ~(uint32_t)(w >> 32) & pb) && (SINGLE || cls[p] == c);
 }
 
+// LDS written by some lanes of a wave and read by others of the same wave
+__device__ __forceinline__ void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Append v for every lane with p to the wave's LDS queue (all 64 lanes call).
+__device__ __forceinline__ void wave_push(uint32_t *wq, uint32_t &nq, bool p, uint32_t v)
+{
+    const uint64_t m = __ballot(p);
+    if (p) wq[nq + (uint32_t)__popcll(m & ((1ull << wlane()) - 1))] = v;
+    nq += (uint32_t)__popcll(m);
+}
+
+__device__ __forceinline__ void wave_flush(uint32_t *wq, uint32_t &nq, uint32_t *gcnt, uint32_t *gout)
+{
+    if (!nq) return;
+    wave_sync();
+    uint32_t b = 0;
+    if (wlane() == 0) b = atomicAdd(gcnt, nq);
+    b = __shfl(b, 0);
+    for (uint32_t i = wlane(); i < nq; i += 64) gout[b + i] = wq[i];
+    nq = 0;
+    wave_sync();
+}
+
 // Both directions of a level share one launch: the first `bpd` workgroups
 // run direction a.d[0], the rest a.d[1].
 template <bool SINGLE>
 __global__ __launch_bounds__(256) void expand_bu_kernel(BottomUpArgs a, uint32_t bpd)
 {
-    __shared__ uint32_t s_long[256];
-    __shared__ uint32_t s_nlong;
+    __shared__ uint32_t s_q[4][BU_WQ];
+    __shared__ uint32_t s_long[4][64];
     const int dir = (a.ndirs == 2 && blockIdx.x >= bpd) ? 1 : 0;
     const uint32_t bid = blockIdx.x - (uint32_t)dir * bpd;
     const DirArgs &D = a.d[dir];
@@ -258,11 +294,16 @@ __global__ __launch_bounds__(256) void expand_bu_kernel(BottomUpArgs a, uint32_t
         D.cnt[CT((L + 2) % 3)] = 0;
         *D.bigc_zero = 0ull;
     }
-    if (threadIdx.x == 0) s_nlong = 0;
-    DECLARE_LDS_QUEUE(q);
-    for (uint32_t base = bid * 256; base < V; base += bpd * 256) {
-        const uint32_t u = base + threadIdx.x + 1;
-        bool found = false;
+    const uint32_t wv = threadIdx.x >> 6, lane = wlane();
+    uint32_t *wq = s_q[wv];
+    uint32_t *wl = s_long[wv];
+    uint32_t nq = 0;
+    const uint32_t sub = lane % BU_GROUP, grp = lane / BU_GROUP;
+    const uint32_t gshift = lane & ~(BU_GROUP - 1);
+    for (uint32_t base = (bid * 4 + wv) * 64; base < V; base += bpd * 256) {
+        if (nq + 128 > BU_WQ) wave_flush(wq, nq, cnt_out, D.qout);
+        const uint32_t u = base + lane + 1;
+        bool found = false, need_long = false;
         if (u <= V && !(D.vis[vword(u)] & (1u << (u & 31)))) {
             const uint32_t c = SINGLE ? 0u : cls[u];
             const uint32_t b = PR[u], e = PR[u + 1];
@@ -272,36 +313,49 @@ __global__ __launch_bounds__(256) void expand_bu_kernel(BottomUpArgs a, uint32_t
 #pragma unroll
             for (uint32_t t = 0; t < BU_SHORT; t++)
                 found |= b + t < e && visited_parent<SINGLE>(D.vis, cls, p[t], c);
-            if (!found && e - b > BU_SHORT) s_long[atomicAdd(&s_nlong, 1u)] = u;
+            need_long = !found && e - b > BU_SHORT;
             if (found) found = !(atomicOr(&D.vis[vword(u)], 1u << (u & 31)) & (1u << (u & 31)));
         }
-        lq_push(q, found, u);
-        __syncthreads();
-        // long parent lists: BU_GROUP lanes per vertex, 32 vertices per pass
-        // of the workgroup, each group stops at its first hit
-        const uint32_t nlong = s_nlong;
-        const uint32_t sub = threadIdx.x % BU_GROUP, grp = threadIdx.x / BU_GROUP;
-        const uint32_t gshift = wlane() & ~(BU_GROUP - 1);
-        for (uint32_t k = grp; k < nlong; k += blockDim.x / BU_GROUP) {
-            const uint32_t x = s_long[k];
-            const uint32_t c = SINGLE ? 0u : cls[x];
-            const uint32_t e = PR[x + 1];
+        wave_push(wq, nq, found, u);
+        const uint64_t lm = __ballot(need_long);
+        if (need_long) wl[__popcll(lm & ((1ull << lane) - 1))] = u;
+        const uint32_t nlong = (uint32_t)__popcll(lm);
+        wave_sync();
+        // long parent lists: 64 / BU_GROUP vertices per round, each group
+        // stops at its first hit
+        for (uint32_t k0 = 0; k0 < nlong; k0 += 64 / BU_GROUP) {
+            const uint32_t k = k0 + grp;
             bool hit = false;
-            for (uint32_t j0 = PR[x] + BU_SHORT; j0 < e && !hit; j0 += BU_GROUP) {
-                const uint32_t j = j0 + sub;
-                const bool h = j < e && visited_parent<SINGLE>(D.vis, cls, PC[j], c);
-                hit = ((__ballot(h) >> gshift) & ((1ull << BU_GROUP) - 1)) != 0;
+            uint32_t x = 0;
+            if (k < nlong) {
+                x = wl[k];
+                const uint32_t c = SINGLE ? 0u : cls[x];
+                const uint32_t e = PR[x + 1];
+                for (uint32_t j0 = PR[x] + BU_SHORT; j0 < e && !hit; j0 += BU_GROUP) {
+                    const uint32_t j = j0 + sub;
+                    const bool h = j < e && visited_parent<SINGLE>(D.vis, cls, PC[j], c);
+                    hit = ((__ballot(h) >> gshift) & ((1ull << BU_GROUP) - 1)) != 0;
+                }
             }
             bool claimed = false;
             if (hit && sub == 0)
                 claimed = !(atomicOr(&D.vis[vword(x)], 1u << (x & 31)) & (1u << (x & 31)));
-            lq_push(q, claimed, x);
+            wave_push(wq, nq, claimed, x);
         }
-        __syncthreads();
-        if (threadIdx.x == 0) s_nlong = 0;
-        lq_flush(q, cnt_out, D.qout, false);
+        wave_sync();   // the next chunk rewrites wl
     }
-    lq_flush(q, cnt_out, D.qout, true);
+    // one reservation for the four waves' remainders
+    __shared__ uint32_t s_n[4], s_base;
+    if (lane == 0) s_n[wv] = nq;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const uint32_t tot = s_n[0] + s_n[1] + s_n[2] + s_n[3];
+        s_base = tot ? atomicAdd(cnt_out, tot) : 0u;
+    }
+    __syncthreads();
+    uint32_t off = s_base;
+    for (uint32_t w = 0; w < wv; w++) off += s_n[w];
+    for (uint32_t i = lane; i < nq; i += 64) D.qout[off + i] = wq[i];
 }
 
 void k_bottom_up(hipStream_t s, const BottomUpArgs &a, bool single_class)
