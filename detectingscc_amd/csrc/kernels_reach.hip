@@ -246,34 +246,6 @@ __device__ __forceinline__ bool visited_parent(const uint32_t *vis, const uint32
     return ((uint32_t)w & ~(uint32_t)(w >> 32) & pb) && (SINGLE || cls[p] == c);
 }
 
-// LDS written by some lanes of a wave and read by others of the same wave
-__device__ __forceinline__ void wave_sync()
-{
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
-// Append v for every lane with p to the wave's LDS queue (all 64 lanes call).
-__device__ __forceinline__ void wave_push(uint32_t *wq, uint32_t &nq, bool p, uint32_t v)
-{
-    const uint64_t m = __ballot(p);
-    if (p) wq[nq + (uint32_t)__popcll(m & ((1ull << wlane()) - 1))] = v;
-    nq += (uint32_t)__popcll(m);
-}
-
-__device__ __forceinline__ void wave_flush(uint32_t *wq, uint32_t &nq, uint32_t *gcnt, uint32_t *gout)
-{
-    if (!nq) return;
-    wave_sync();
-    uint32_t b = 0;
-    if (wlane() == 0) b = atomicAdd(gcnt, nq);
-    b = __shfl(b, 0);
-    for (uint32_t i = wlane(); i < nq; i += 64) gout[b + i] = wq[i];
-    nq = 0;
-    wave_sync();
-}
-
 // Both directions of a level share one launch: the first `bpd` workgroups
 // run direction a.d[0], the rest a.d[1].
 template <bool SINGLE>
@@ -344,18 +316,7 @@ __global__ __launch_bounds__(256) void expand_bu_kernel(BottomUpArgs a, uint32_t
         }
         wave_sync();   // the next chunk rewrites wl
     }
-    // one reservation for the four waves' remainders
-    __shared__ uint32_t s_n[4], s_base;
-    if (lane == 0) s_n[wv] = nq;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        const uint32_t tot = s_n[0] + s_n[1] + s_n[2] + s_n[3];
-        s_base = tot ? atomicAdd(cnt_out, tot) : 0u;
-    }
-    __syncthreads();
-    uint32_t off = s_base;
-    for (uint32_t w = 0; w < wv; w++) off += s_n[w];
-    for (uint32_t i = lane; i < nq; i += 64) D.qout[off + i] = wq[i];
+    wave_flush_block(wq, nq, cnt_out, D.qout);
 }
 
 void k_bottom_up(hipStream_t s, const BottomUpArgs &a, bool single_class)

@@ -141,8 +141,10 @@ __device__ __forceinline__ bool trim1_remove(uint32_t *cls, uint32_t *lab, uint3
     return true;
 }
 
-// Removals are staged in LDS (one global atomic per flush): a sweep can
-// remove millions of vertices.
+// Removals go to wave-private LDS queues (a sweep can remove millions of
+// vertices; waves do not wait on each other).
+constexpr uint32_t TRIM_WQ = 1024;
+
 __global__ __launch_bounds__(256) void trim1_kernel(const uint32_t *__restrict__ Fr,
                                                     const uint32_t *__restrict__ Fc,
                                                     const uint32_t *__restrict__ Br,
@@ -150,22 +152,24 @@ __global__ __launch_bounds__(256) void trim1_kernel(const uint32_t *__restrict__
                                                     uint32_t *lab, uint32_t V, uint32_t *ctr, uint32_t *out,
                                                     uint32_t pass)
 {
-    DECLARE_LDS_QUEUE(q);
-    uint32_t removed = 0;
+    __shared__ uint32_t s_q[4][TRIM_WQ];
+    const uint32_t wv = threadIdx.x >> 6;
+    uint32_t *wq = s_q[wv];
+    uint32_t nq = 0, removed = 0;
     uint32_t *cnt_out = &ctr[CT(C_TLIST + (pass + 1) % 3)];
     if (blockIdx.x == 0 && threadIdx.x == 0) ctr[CT(C_TLIST + (pass + 2) % 3)] = 0;
-    for (uint32_t base = blockIdx.x * 256; base < V; base += gridDim.x * 256) {
-        const uint32_t v = base + threadIdx.x + 1;
+    for (uint32_t base = (blockIdx.x * 4 + wv) * 64; base < V; base += gridDim.x * 256) {
+        if (nq + 64 > TRIM_WQ) wave_flush(wq, nq, cnt_out, out);
+        const uint32_t v = base + lane() + 1;
         bool gone = false;
         if (v <= V) {
             const uint32_t c = cls[v];
             if (c != DEAD && !trim1_keep(Fr, Fc, Br, Bc, cls, v, c)) gone = trim1_remove(cls, lab, v, c);
         }
         removed += gone;
-        lq_push(q, gone, v);
-        lq_flush(q, cnt_out, out, false);
+        wave_push(wq, nq, gone, v);
     }
-    lq_flush(q, cnt_out, out, true);
+    wave_flush_block(wq, nq, cnt_out, out);
     wave_add32(&ctr[CT(C_TRIM1)], removed);
 }
 

@@ -48,6 +48,60 @@ __device__ __forceinline__ void lq_flush(const LdsQueue &q, uint32_t *gcnt, uint
     __syncthreads();
 }
 
+// ---- wave-private queues ----
+// A kernel whose waves should not wait on each other keeps one LDS queue
+// per wave (nq is wave-uniform), reserves global space itself when its
+// queue fills, and sends the remainders out with one reservation per
+// 256-lane workgroup at the end.
+
+// LDS written by some lanes of a wave and read by others of the same wave
+__device__ __forceinline__ void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Append v for every lane with p (all 64 lanes call).
+__device__ __forceinline__ void wave_push(uint32_t *wq, uint32_t &nq, bool p, uint32_t v)
+{
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint64_t m = __ballot(p);
+    if (p) wq[nq + (uint32_t)__popcll(m & ((1ull << lane) - 1))] = v;
+    nq += (uint32_t)__popcll(m);
+}
+
+__device__ __forceinline__ void wave_flush(uint32_t *wq, uint32_t &nq, uint32_t *gcnt, uint32_t *gout)
+{
+    if (!nq) return;
+    const uint32_t lane = threadIdx.x & 63u;
+    wave_sync();
+    uint32_t b = 0;
+    if (lane == 0) b = atomicAdd(gcnt, nq);
+    b = __shfl(b, 0);
+    for (uint32_t i = lane; i < nq; i += 64) gout[b + i] = wq[i];
+    nq = 0;
+    wave_sync();
+}
+
+// End of kernel: the four waves' remainders with one atomic (every thread
+// of the 256-lane workgroup calls).
+__device__ __forceinline__ void wave_flush_block(const uint32_t *wq, uint32_t nq, uint32_t *gcnt, uint32_t *gout)
+{
+    __shared__ uint32_t s_n[4], s_base;
+    const uint32_t lane = threadIdx.x & 63u, wv = threadIdx.x >> 6;
+    if (lane == 0) s_n[wv] = nq;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const uint32_t tot = s_n[0] + s_n[1] + s_n[2] + s_n[3];
+        s_base = tot ? atomicAdd(gcnt, tot) : 0u;
+    }
+    __syncthreads();
+    uint32_t off = s_base;
+    for (uint32_t w = 0; w < wv; w++) off += s_n[w];
+    for (uint32_t i = lane; i < nq; i += 64) gout[off + i] = wq[i];
+}
+
 #define DECLARE_LDS_QUEUE(q)                                                                 \
     __shared__ uint32_t q##_buf[QBUF];                                                       \
     __shared__ uint32_t q##_n, q##_base;                                                     \
