@@ -67,6 +67,9 @@ class Engine {
 
     uint32_t capV = 0, capE = 0;
     uint32_t *h_ctr = nullptr;        // pinned mirror of st.ctr
+    uint32_t *h_pub = nullptr;        // mapped coherent host memory the GPU publishes counters into
+    uint32_t *d_pub = nullptr;        // its device address
+    uint32_t pub_seq = 0;             // last publication requested
     uint8_t *d_tmp8 = nullptr;        // V+1 bytes for phase inputs/outputs
     int check_every = 0;
     bool search_stats = false;        // count |R| and |E_R| of the first search

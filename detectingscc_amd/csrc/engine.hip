@@ -58,6 +58,9 @@ void Engine::release()
     }
     sub.reset();
     if (h_ctr) (void)hipHostFree(h_ctr);
+    if (h_pub) (void)hipHostFree(h_pub);
+    h_pub = nullptr;
+    d_pub = nullptr;
     for (auto e : events) (void)hipEventDestroy(e);
     events.clear();
     if (stream) (void)hipStreamDestroy(stream);
@@ -81,6 +84,22 @@ scc_status Engine::init(int dev)
     HIP_TRY(hipHostMalloc((void **)&h_ctr, CT(C_COUNT) * sizeof(uint32_t), hipHostMallocDefault));
     HIP_TRY(hipMalloc((void **)&st.ctr, CT(C_COUNT) * sizeof(uint32_t)));
     HIP_TRY(hipMemset(st.ctr, 0, CT(C_COUNT) * sizeof(uint32_t)));
+    // counters published into mapped, coherent host memory: C_COUNT lines
+    // plus one line for the sequence word (without it, read-backs copy and
+    // synchronise the stream)
+    if (hipHostMalloc((void **)&h_pub, CT(C_COUNT + 1) * sizeof(uint32_t),
+                      hipHostMallocMapped | hipHostMallocCoherent) == hipSuccess) {
+        std::memset(h_pub, 0, CT(C_COUNT + 1) * sizeof(uint32_t));
+        if (hipHostGetDevicePointer((void **)&d_pub, h_pub, 0) != hipSuccess) {
+            (void)hipGetLastError();
+            (void)hipHostFree(h_pub);
+            h_pub = nullptr;
+            d_pub = nullptr;
+        }
+    } else {
+        (void)hipGetLastError();
+        h_pub = nullptr;
+    }
     // diagnostic: SCC_TRACE_LEVELS=1 prints every host decision of a search
     const char *tr = std::getenv("SCC_TRACE_LEVELS");
     trace_levels = tr && tr[0] == '1';
@@ -256,10 +275,54 @@ template <class T> scc_status Engine::scratch(DevBuf &b, size_t n, T **out)
     return SCC_OK;
 }
 
+// Both words of every counter line into host memory the GPU writes
+// directly, then the sequence number (system-scope release).
+__global__ void publish_ctr_kernel(const uint32_t *ctr, uint32_t *pub, uint32_t seq)
+{
+    const uint32_t i = threadIdx.x;
+    if (i < C_COUNT) {
+        pub[CT(i)] = ctr[CT(i)];
+        pub[CT(i) + 1] = ctr[CT(i) + 1];
+    }
+    __threadfence_system();
+    __syncthreads();
+    if (i == 0) __hip_atomic_store(pub + CT(C_COUNT), seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
+// The host waits for the counters of everything queued so far.  With the
+// published mirror it spins on the sequence word (a few microseconds after
+// the last kernel) instead of a copy plus a stream synchronisation (about
+// 25 us); after kPublishWaitMs it synchronises the stream anyway, which also
+// reports any device error.
+static constexpr double kPublishWaitMs = 2000.0;
+
 scc_status Engine::read_ctr()
 {
-    HIP_TRY(hipMemcpyAsync(h_ctr, st.ctr, CT(C_COUNT) * sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
-    HIP_TRY(hipStreamSynchronize(stream));
+    if (!h_pub) {
+        HIP_TRY(hipMemcpyAsync(h_ctr, st.ctr, CT(C_COUNT) * sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
+        HIP_TRY(hipStreamSynchronize(stream));
+        return SCC_OK;
+    }
+    if (++pub_seq == 0) pub_seq = 1;
+    const uint32_t seq = pub_seq;
+    publish_ctr_kernel<<<1, 64, 0, stream>>>(st.ctr, d_pub, seq);
+    HIP_TRY(hipGetLastError());
+    const uint32_t *flag = h_pub + CT(C_COUNT);
+    const auto t0 = std::chrono::steady_clock::now();
+    for (uint64_t spin = 0; __atomic_load_n(flag, __ATOMIC_ACQUIRE) != seq; spin++) {
+        if ((spin & 1023) == 1023 &&
+            std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count() > kPublishWaitMs) {
+            HIP_TRY(hipStreamSynchronize(stream));
+            if (__atomic_load_n(flag, __ATOMIC_ACQUIRE) != seq)
+                return fail(SCC_ERR_DEVICE, "counter publication %u did not arrive", seq);
+            break;
+        }
+        __builtin_ia32_pause();
+    }
+    for (int i = 0; i < C_COUNT; i++) {
+        h_ctr[CT(i)] = h_pub[CT(i)];
+        h_ctr[CT(i) + 1] = h_pub[CT(i) + 1];
+    }
     return SCC_OK;
 }
 
