@@ -507,32 +507,34 @@ scc_status Engine::reach(bool fw, bool bw, bool single, int expand, int W, uint3
 // first sweep each pass re-checks the neighbours of the previous pass's
 // removals (listed alternately in the FW queue buffers, which are free
 // outside a search) until a pass removes nothing; a pass that follows one
-// removing more than 1/kTrimSweepShare of the vertices sweeps instead.
+// removing more than 1/kTrimSweepShare of the vertices sweeps instead.  The
+// kernel makes that choice from the list count, so the host queues
+// kTrimQueued passes per look at the counters.
 static constexpr uint32_t kTrimSweepShare = 32;
+static constexpr uint32_t kTrimQueued = 4;
 
 scc_status Engine::trim1(bool fixpoint)
 {
     mark(SCC_PHASE_TRIM1);
     zero_ctr(C_TLIST, 3);
-    k_trim1(stream, g, st, st.qf[0], 0);
-    scc_status s = read_ctr();
-    if (s != SCC_OK) return s;
-    for (uint32_t pass = 1; fixpoint; pass++) {
-        const uint32_t n = h_ctr[CT(C_TLIST + pass % 3)];
-        if (!n) break;
-        // every pass but the last removes a vertex
-        if (pass > limit()) return no_convergence("trim-1");
-        if (n > g.V / kTrimSweepShare) {
-            k_trim1(stream, g, st, st.qf[pass & 1], pass);
-        } else {
+    k_trim1(stream, g, st, st.qf[1], st.qf[0], 0u, 0, kTrimSweepShare);
+    uint32_t pass = 1;
+    while (fixpoint) {
+        for (uint32_t k = 0; k < kTrimQueued; k++, pass++) {
             if (++trim_epoch == 0) {   // wrapped: no stale stamp may match a new epoch
                 k_fill_u32(stream, st.stamp, 0u, (uint64_t)g.V + 1);
                 trim_epoch = 1;
             }
-            k_trim1_list(stream, g, st, st.qf[(pass + 1) & 1], st.qf[pass & 1], trim_epoch, pass, n);
+            k_trim1(stream, g, st, st.qf[(pass + 1) & 1], st.qf[pass & 1], trim_epoch, pass, kTrimSweepShare);
         }
-        if ((s = read_ctr()) != SCC_OK) return s;
+        scc_status s = read_ctr();
+        if (s != SCC_OK) return s;
+        // the next pass's input: what the last queued pass removed
+        if (!h_ctr[CT(C_TLIST + pass % 3)]) break;
+        // every pass but the last removes a vertex
+        if (pass > limit()) return no_convergence("trim-1");
     }
+    if (!fixpoint) return read_ctr();
     return SCC_OK;
 }
 

@@ -141,107 +141,92 @@ __device__ __forceinline__ bool trim1_remove(uint32_t *cls, uint32_t *lab, uint3
     return true;
 }
 
-// Removals go to wave-private LDS queues (a sweep can remove millions of
-// vertices; waves do not wait on each other).
+// One pass of the fixpoint.  Pass 0, and any pass whose input list holds
+// more than V / sweep_share vertices, sweeps every vertex; the others test
+// the live neighbours of the listed removals, TRIM_W lanes per removed
+// vertex and one arc per lane per round (the workgroup runs as many rounds
+// as its longest list needs).  The pass reads list count C_TLIST + pass%3,
+// appends to C_TLIST + (pass+1)%3 and clears C_TLIST + (pass+2)%3, so the
+// host can queue several passes: once a pass removes nothing the following
+// ones find an empty list and only clear counters.  Removals go to
+// wave-private LDS queues (a sweep can remove millions of vertices).
 constexpr uint32_t TRIM_WQ = 1024;
+constexpr uint32_t TRIM_W = 8;
+constexpr uint32_t TRIM_PER_BLOCK = 256 / TRIM_W;
 
 __global__ __launch_bounds__(256) void trim1_kernel(const uint32_t *__restrict__ Fr,
                                                     const uint32_t *__restrict__ Fc,
                                                     const uint32_t *__restrict__ Br,
                                                     const uint32_t *__restrict__ Bc, uint32_t *cls,
-                                                    uint32_t *lab, uint32_t V, uint32_t *ctr, uint32_t *out,
-                                                    uint32_t pass)
+                                                    uint32_t *lab, uint32_t V, uint32_t *ctr,
+                                                    const uint32_t *in, uint32_t *out, uint32_t *stamp,
+                                                    uint32_t epoch, uint32_t pass, uint32_t sweep_share)
 {
     __shared__ uint32_t s_q[4][TRIM_WQ];
+    __shared__ uint32_t s_rounds;
     const uint32_t wv = threadIdx.x >> 6;
     uint32_t *wq = s_q[wv];
     uint32_t nq = 0, removed = 0;
+    const uint32_t n = pass ? ctr[CT(C_TLIST + pass % 3)] : 0u;
     uint32_t *cnt_out = &ctr[CT(C_TLIST + (pass + 1) % 3)];
     if (blockIdx.x == 0 && threadIdx.x == 0) ctr[CT(C_TLIST + (pass + 2) % 3)] = 0;
-    for (uint32_t base = (blockIdx.x * 4 + wv) * 64; base < V; base += gridDim.x * 256) {
-        if (nq + 64 > TRIM_WQ) wave_flush(wq, nq, cnt_out, out);
-        const uint32_t v = base + lane() + 1;
-        bool gone = false;
-        if (v <= V) {
-            const uint32_t c = cls[v];
-            if (c != DEAD && !trim1_keep(Fr, Fc, Br, Bc, cls, v, c)) gone = trim1_remove(cls, lab, v, c);
+    if (pass == 0 || n > V / sweep_share) {
+        for (uint32_t base = (blockIdx.x * 4 + wv) * 64; base < V; base += gridDim.x * 256) {
+            if (nq + 64 > TRIM_WQ) wave_flush(wq, nq, cnt_out, out);
+            const uint32_t v = base + lane() + 1;
+            bool gone = false;
+            if (v <= V) {
+                const uint32_t c = cls[v];
+                if (c != DEAD && !trim1_keep(Fr, Fc, Br, Bc, cls, v, c)) gone = trim1_remove(cls, lab, v, c);
+            }
+            removed += gone;
+            wave_push(wq, nq, gone, v);
         }
-        removed += gone;
-        wave_push(wq, nq, gone, v);
+    } else {
+        const uint32_t sub = threadIdx.x % TRIM_W;
+        for (uint32_t base = blockIdx.x * TRIM_PER_BLOCK; base < n; base += gridDim.x * TRIM_PER_BLOCK) {
+            const uint32_t i = base + threadIdx.x / TRIM_W;
+            uint32_t fb = 0, nf = 0, bb = 0, total = 0;
+            if (i < n) {
+                const uint32_t v = in[i];
+                fb = Fr[v];
+                nf = Fr[v + 1] - fb;
+                bb = Br[v];
+                total = nf + (Br[v + 1] - bb);
+            }
+            if (threadIdx.x == 0) s_rounds = 0;
+            __syncthreads();
+            if (sub == 0 && total) atomicMax(&s_rounds, (total + TRIM_W - 1) / TRIM_W);
+            __syncthreads();
+            const uint32_t rounds = s_rounds;
+            for (uint32_t r = 0; r < rounds; r++) {
+                if (nq + 64 > TRIM_WQ) wave_flush(wq, nq, cnt_out, out);
+                const uint32_t j = r * TRIM_W + sub;
+                bool gone = false;
+                uint32_t w = 0;
+                if (j < total) {
+                    w = j < nf ? Fc[fb + j] : Bc[bb + (j - nf)];
+                    const uint32_t c = cls[w];
+                    if (c != DEAD && stamp[w] != epoch && atomicExch(&stamp[w], epoch) != epoch &&
+                        !trim1_keep(Fr, Fc, Br, Bc, cls, w, c))
+                        gone = trim1_remove(cls, lab, w, c);
+                }
+                removed += gone;
+                wave_push(wq, nq, gone, w);
+            }
+            __syncthreads();   // s_rounds reuse
+        }
     }
     wave_flush_block(wq, nq, cnt_out, out);
     wave_add32(&ctr[CT(C_TRIM1)], removed);
 }
 
-// TRIM_W lanes per removed vertex walk its out- and in-arcs, one arc per
-// lane per round, and test each live neighbour; the block runs as many
-// rounds as its longest list needs, so every lane reaches each flush.
-constexpr uint32_t TRIM_W = 8;
-constexpr uint32_t TRIM_PER_BLOCK = 256 / TRIM_W;
-
-__global__ __launch_bounds__(256) void trim1_list_kernel(const uint32_t *__restrict__ Fr,
-                                                         const uint32_t *__restrict__ Fc,
-                                                         const uint32_t *__restrict__ Br,
-                                                         const uint32_t *__restrict__ Bc, uint32_t *cls,
-                                                         uint32_t *lab, uint32_t *ctr, const uint32_t *in,
-                                                         uint32_t *out, uint32_t *stamp, uint32_t epoch,
-                                                         uint32_t pass)
+void k_trim1(hipStream_t s, const DevGraph &g, DevState &st, const uint32_t *in, uint32_t *out,
+             uint32_t epoch, uint32_t pass, uint32_t sweep_share)
 {
-    __shared__ uint32_t s_rounds;
-    DECLARE_LDS_QUEUE(q);
-    const uint32_t n = ctr[CT(C_TLIST + pass % 3)];
-    uint32_t *cnt_out = &ctr[CT(C_TLIST + (pass + 1) % 3)];
-    if (blockIdx.x == 0 && threadIdx.x == 0) ctr[CT(C_TLIST + (pass + 2) % 3)] = 0;
-    const uint32_t sub = threadIdx.x % TRIM_W;
-    uint32_t removed = 0;
-    for (uint32_t base = blockIdx.x * TRIM_PER_BLOCK; base < n; base += gridDim.x * TRIM_PER_BLOCK) {
-        const uint32_t i = base + threadIdx.x / TRIM_W;
-        uint32_t fb = 0, nf = 0, bb = 0, total = 0;
-        if (i < n) {
-            const uint32_t v = in[i];
-            fb = Fr[v];
-            nf = Fr[v + 1] - fb;
-            bb = Br[v];
-            total = nf + (Br[v + 1] - bb);
-        }
-        if (threadIdx.x == 0) s_rounds = 0;
-        __syncthreads();
-        if (sub == 0 && total) atomicMax(&s_rounds, (total + TRIM_W - 1) / TRIM_W);
-        __syncthreads();
-        const uint32_t rounds = s_rounds;
-        for (uint32_t r = 0; r < rounds; r++) {
-            const uint32_t j = r * TRIM_W + sub;
-            bool gone = false;
-            uint32_t w = 0;
-            if (j < total) {
-                w = j < nf ? Fc[fb + j] : Bc[bb + (j - nf)];
-                const uint32_t c = cls[w];
-                if (c != DEAD && stamp[w] != epoch && atomicExch(&stamp[w], epoch) != epoch &&
-                    !trim1_keep(Fr, Fc, Br, Bc, cls, w, c))
-                    gone = trim1_remove(cls, lab, w, c);
-            }
-            removed += gone;
-            lq_push(q, gone, w);
-            lq_flush(q, cnt_out, out, false);
-        }
-    }
-    lq_flush(q, cnt_out, out, true);
-    wave_add32(&ctr[CT(C_TRIM1)], removed);
-}
-
-void k_trim1(hipStream_t s, const DevGraph &g, DevState &st, uint32_t *out, uint32_t pass)
-{
-    trim1_kernel<<<sweep_blocks(g.V), SWEEP_BLOCK, 0, s>>>(g.Fr, g.Fc, g.Br, g.Bc, st.cls, st.lab,
-                                                           g.V, st.ctr, out, pass);
-}
-
-void k_trim1_list(hipStream_t s, const DevGraph &g, DevState &st, const uint32_t *in, uint32_t *out,
-                  uint32_t epoch, uint32_t pass, uint32_t n)
-{
-    uint64_t b = ((uint64_t)n * TRIM_W + SWEEP_BLOCK - 1) / SWEEP_BLOCK;
-    const uint32_t grid = (uint32_t)(b < 1 ? 1 : (b > 2048 ? 2048 : b));
-    trim1_list_kernel<<<grid, SWEEP_BLOCK, 0, s>>>(g.Fr, g.Fc, g.Br, g.Bc, st.cls, st.lab, st.ctr, in,
-                                                   out, st.stamp, epoch, pass);
+    trim1_kernel<<<sweep_blocks(g.V), SWEEP_BLOCK, 0, s>>>(g.Fr, g.Fc, g.Br, g.Bc, st.cls, st.lab, g.V,
+                                                           st.ctr, in, out, st.stamp, epoch, pass,
+                                                           sweep_share);
 }
 
 // ---------------------------------------------------------- trim-2 ---

@@ -125,14 +125,14 @@ struct LevelArgs {
 // ---- sweep kernels (kernels_sweep.hip) ----
 void k_fill_u32(hipStream_t s, uint32_t *p, uint32_t val, uint64_t n);
 void k_init_state(hipStream_t s, const DevGraph &g, DevState &st, const uint8_t *dead_dev);
-// trim-1 sweep over every vertex as pass `pass`; removed vertices are
-// listed in `out`, counted at C_TLIST + (pass+1)%3
-void k_trim1(hipStream_t s, const DevGraph &g, DevState &st, uint32_t *out, uint32_t pass);
-// trim-1 pass `pass` >= 1 over the live neighbours of the vertices listed
-// in `in` (n of them, counted at C_TLIST + pass%3); removals go to `out`;
-// `epoch` is new for every pass of the context (st.stamp never holds it)
-void k_trim1_list(hipStream_t s, const DevGraph &g, DevState &st, const uint32_t *in, uint32_t *out,
-                  uint32_t epoch, uint32_t pass, uint32_t n);
+// trim-1 pass `pass` of a fixpoint: pass 0 (and a pass whose input list,
+// counted at C_TLIST + pass%3, holds more than V / sweep_share vertices)
+// sweeps every vertex, the others test the live neighbours of the vertices
+// listed in `in`; removals are listed in `out`, counted at
+// C_TLIST + (pass+1)%3.  `epoch` is new for every pass of the context
+// (st.stamp never holds it).
+void k_trim1(hipStream_t s, const DevGraph &g, DevState &st, const uint32_t *in, uint32_t *out,
+             uint32_t epoch, uint32_t pass, uint32_t sweep_share);
 void k_trim2(hipStream_t s, const DevGraph &g, DevState &st);
 void k_init_vis(hipStream_t s, const DevGraph &g, DevState &st);
 // single_class: every live vertex is in class 0 (the first pivot)
