@@ -63,7 +63,10 @@ struct G1 {
     uint32_t *cls, *lab, *aux;
     uint32_t *fvis, *bvis;
     uint64_t *piv;
-    uint32_t *qf[2], *qb[2];
+    uint32_t *q;          // four queues of V+2 entries: FW in/out, BW in/out
+    // queue k (0..3) by arithmetic: a runtime index into a pointer array
+    // would put the whole struct in scratch memory
+    __device__ __forceinline__ uint32_t *queue(uint32_t k) const { return q + (size_t)k * (V + 2); }
     uint32_t *pivlist;
 };
 
@@ -198,8 +201,8 @@ template <int BS> __device__ void g_reach(Lds &L, const G1 &g, bool do_bw, bool 
             const uint32_t n = d ? nb : nf;
             const uint32_t *R = d ? g.Br : g.Fr, *C = d ? g.Bc : g.Fc;
             uint32_t *vis = d ? g.bvis : g.fvis;
-            const uint32_t *qin = d ? g.qb[in] : g.qf[in];
-            uint32_t *qout = d ? g.qb[in ^ 1] : g.qf[in ^ 1];
+            const uint32_t *qin = g.queue(2 * d + in);
+            uint32_t *qout = g.queue(2 * d + (in ^ 1));
             uint32_t *cnt = &L.cnt[d ? 3 : 1];
             for (uint32_t i = grp; i < n; i += ngrp) {
                 const uint32_t v = qin[i];
@@ -266,8 +269,8 @@ template <int BS> __device__ uint32_t g_pivots(Lds &L, const G1 &g, bool first)
     __syncthreads();
     const uint32_t n = L.npiv;
     for (uint32_t i = threadIdx.x; i < n; i += BS) {
-        g.qf[0][i] = g.pivlist[i];
-        g.qb[0][i] = g.pivlist[i];
+        g.queue(0)[i] = g.pivlist[i];
+        g.queue(2)[i] = g.pivlist[i];
     }
     if (threadIdx.x == 0) {
         L.cnt[0] = n;
@@ -394,7 +397,7 @@ template <int BS> __device__ void g_color_loop(Lds &L, const G1 &g, uint32_t W, 
                 root = c == v;
                 if (root) atomicOr(&g.fvis[v >> 5], 1u << (v & 31));
             }
-            lpush(root, v, g.qf[0], &L.cnt[0]);
+            lpush(root, v, g.queue(0), &L.cnt[0]);
         }
         __syncthreads();
         g_reach<BS>(L, g, false, false, W);
@@ -460,10 +463,7 @@ __global__ __launch_bounds__(BS) void batch_scc_kernel(BatchDev B, uint32_t firs
         g.fvis = B.fvis + vo / 32;
         g.bvis = B.bvis + vo / 32;
         g.piv = B.piv + 3 * vo;
-        g.qf[0] = B.q + 4 * vo;
-        g.qf[1] = g.qf[0] + (g.V + 2);
-        g.qb[0] = g.qf[1] + (g.V + 2);
-        g.qb[1] = g.qb[0] + (g.V + 2);
+        g.q = B.q + 4 * vo;
         g.pivlist = B.pivlist + vo;
         const uint32_t cfg = B.cfg[gi];
         const uint32_t W = cfg_lanes(cfg);
