@@ -224,6 +224,19 @@ def write_gtgraph(path: str, g: Graph) -> None:
     _check(lib().scc_write_gtgraph(os.fsencode(path), ctypes.byref(hg)))
 
 
+def write_csr(path: str, g: Graph) -> None:
+    """Binary CSR cache: the four reference-layout arrays behind a header."""
+    hg = _HostGraph(g.V, g.E, _p32(g.Fr), _p32(g.Fc), _p32(g.Br), _p32(g.Bc))
+    _check(lib().scc_write_csr(os.fsencode(path), ctypes.byref(hg)))
+
+
+def load_graph(path: str) -> Graph:
+    """GTgraph text or binary CSR cache, told apart by the cache's magic."""
+    hg = _HostGraph()
+    _check(lib().scc_load_graph(os.fsencode(path), ctypes.byref(hg)))
+    return _take(hg)
+
+
 def write_metis(path: str, g: Graph) -> None:
     hg = _HostGraph(g.V, g.E, _p32(g.Fr), _p32(g.Fc), _p32(g.Br), _p32(g.Bc))
     _check(lib().scc_write_metis(os.fsencode(path), ctypes.byref(hg)))

@@ -363,6 +363,110 @@ scc_status scc_write_metis(const char *path, const scc_host_graph *g)
     return std::fclose(f) == 0 ? SCC_OK : fail(SCC_ERR_IO, "write to %s failed", path);
 }
 
+// Binary CSR cache: 8-byte magic, V, E (u32 each), 8 reserved bytes, then
+// Fr[V+2], Fc[E], Br[V+2], Bc[E] as little-endian u32.
+static const char kCsrMagic[8] = {'S', 'C', 'C', 'C', 'S', 'R', '0', '1'};
+
+scc_status scc_write_csr(const char *path, const scc_host_graph *g)
+{
+    clear_error();
+    if (!path || !g || !g->Fr || !g->Br || (g->E && (!g->Fc || !g->Bc)))
+        return fail(SCC_ERR_INVALID, "null argument");
+    FILE *f = std::fopen(path, "wb");
+    if (!f) return fail(SCC_ERR_IO, "cannot create %s", path);
+    uint32_t ve[2] = {g->V, g->E};
+    uint64_t reserved = 0;
+    const size_t rows = (size_t)g->V + 2;
+    bool ok = std::fwrite(kCsrMagic, 1, 8, f) == 8 && std::fwrite(ve, 4, 2, f) == 2 &&
+              std::fwrite(&reserved, 8, 1, f) == 1 && std::fwrite(g->Fr, 4, rows, f) == rows &&
+              std::fwrite(g->Fc, 4, g->E, f) == g->E && std::fwrite(g->Br, 4, rows, f) == rows &&
+              std::fwrite(g->Bc, 4, g->E, f) == g->E;
+    ok = (std::fclose(f) == 0) && ok;
+    return ok ? SCC_OK : fail(SCC_ERR_IO, "write to %s failed", path);
+}
+
+scc_status scc_load_csr(const char *path, scc_host_graph *out)
+{
+    clear_error();
+    if (!path || !out) return fail(SCC_ERR_INVALID, "null argument");
+    std::memset(out, 0, sizeof *out);
+    FILE *f = std::fopen(path, "rb");
+    if (!f) return fail(SCC_ERR_IO, "cannot open %s", path);
+    char magic[8];
+    uint32_t ve[2];
+    uint64_t reserved;
+    if (std::fread(magic, 1, 8, f) != 8 || std::memcmp(magic, kCsrMagic, 8) != 0 ||
+        std::fread(ve, 4, 2, f) != 2 || std::fread(&reserved, 8, 1, f) != 1) {
+        std::fclose(f);
+        return fail(SCC_ERR_PARSE, "%s: not a binary CSR file", path);
+    }
+    const uint32_t V = ve[0], E = ve[1];
+    if (V >= (1u << 30)) {
+        std::fclose(f);
+        return fail(SCC_ERR_PARSE, "%s: V = %u too large", path, V);
+    }
+    const size_t rows = (size_t)V + 2;
+    scc_host_graph g;
+    g.V = V;
+    g.E = E;
+    g.Fr = xalloc<uint32_t>(rows);
+    g.Br = xalloc<uint32_t>(rows);
+    g.Fc = xalloc<uint32_t>(E ? E : 1);
+    g.Bc = xalloc<uint32_t>(E ? E : 1);
+    if (!g.Fr || !g.Br || !g.Fc || !g.Bc) {
+        std::fclose(f);
+        scc_host_graph_free(&g);
+        return fail(SCC_ERR_NOMEM, "host allocation failed for %s", path);
+    }
+    const bool ok = std::fread(g.Fr, 4, rows, f) == rows && std::fread(g.Fc, 4, E, f) == E &&
+                    std::fread(g.Br, 4, rows, f) == rows && std::fread(g.Bc, 4, E, f) == E;
+    std::fclose(f);
+    if (!ok) {
+        scc_host_graph_free(&g);
+        return fail(SCC_ERR_PARSE, "%s: file shorter than its header says", path);
+    }
+    // the reference layout: rows from 0,0 to E, monotone; rows sorted and ids
+    // in [1, V]; the reverse CSR's in-degrees match the forward columns
+    std::atomic<int> bad{0};
+    if (g.Fr[0] || g.Fr[1] || g.Br[0] || g.Br[1] || g.Fr[V + 1] != E || g.Br[V + 1] != E) bad = 1;
+    if (!bad) {
+        parallel_chunks((uint64_t)V + 1, [&](uint64_t lo, uint64_t hi, int) {
+            for (uint64_t v = lo; v < hi && !bad; v++) {
+                if (g.Fr[v] > g.Fr[v + 1] || g.Br[v] > g.Br[v + 1]) { bad = 1; break; }
+                for (uint32_t i = g.Fr[v]; i < g.Fr[v + 1]; i++)
+                    if (g.Fc[i] < 1 || g.Fc[i] > V || (i > g.Fr[v] && g.Fc[i] < g.Fc[i - 1])) { bad = 1; break; }
+                for (uint32_t i = g.Br[v]; i < g.Br[v + 1]; i++)
+                    if (g.Bc[i] < 1 || g.Bc[i] > V || (i > g.Br[v] && g.Bc[i] < g.Bc[i - 1])) { bad = 1; break; }
+            }
+        });
+    }
+    if (!bad) {
+        std::vector<uint32_t> indeg((size_t)V + 2, 0);
+        for (uint32_t i = 0; i < E; i++) indeg[g.Fc[i]]++;
+        for (uint32_t v = 1; v <= V && !bad; v++)
+            if (g.Br[v + 1] - g.Br[v] != indeg[v]) bad = 1;
+    }
+    if (bad) {
+        scc_host_graph_free(&g);
+        return fail(SCC_ERR_PARSE, "%s: arrays are not a CSR + reverse CSR in the reference layout", path);
+    }
+    *out = g;
+    return SCC_OK;
+}
+
+scc_status scc_load_graph(const char *path, scc_host_graph *out)
+{
+    clear_error();
+    if (!path || !out) return fail(SCC_ERR_INVALID, "null argument");
+    FILE *f = std::fopen(path, "rb");
+    if (!f) return fail(SCC_ERR_IO, "cannot open %s", path);
+    char magic[8] = {0};
+    const size_t n = std::fread(magic, 1, 8, f);
+    std::fclose(f);
+    if (n == 8 && std::memcmp(magic, kCsrMagic, 8) == 0) return scc_load_csr(path, out);
+    return scc_load_gtgraph(path, out);
+}
+
 scc_status scc_size_histogram(uint32_t V, const uint32_t *lab, uint64_t *pairs, uint32_t cap,
                               uint32_t *npairs)
 {

@@ -65,7 +65,7 @@ int main(int argc, char **argv)
     }
     scc_host_graph g;
     std::memset(&g, 0, sizeof g);
-    scc_status s = scc_load_gtgraph(file, &g);
+    scc_status s = scc_load_graph(file, &g);
     if (s != SCC_OK) {
         printf("Error Reading graph file\n");
         fprintf(stderr, "%s\n", scc_last_error());

@@ -76,6 +76,17 @@ scc_status scc_csr_from_arcs(uint32_t V, uint64_t E, const uint32_t *src, const 
 scc_status scc_write_gtgraph(const char *path, const scc_host_graph *g);
 scc_status scc_write_metis(const char *path, const scc_host_graph *g);
 
+/* Binary CSR cache (no reference counterpart; the reference re-parses and
+ * re-sorts the GTgraph text on every run, load.cpp:32-113): magic
+ * "SCCCSR01", V and E as u32, 8 reserved bytes, then Fr[V+2], Fc[E],
+ * Br[V+2], Bc[E] as little-endian u32.  The loader checks the reference
+ * layout (rows from 0,0 to E, monotone, rows sorted, ids in [1, V], the
+ * reverse CSR's in-degrees equal to the forward columns' counts). */
+scc_status scc_write_csr(const char *path, const scc_host_graph *g);
+scc_status scc_load_csr(const char *path, scc_host_graph *out);
+/* Either format, told apart by the binary CSR magic (the `scc -f` path). */
+scc_status scc_load_graph(const char *path, scc_host_graph *out);
+
 /*
  * Seeded synthetic graphs (no reference counterpart; the reference ships
  * only sample files).  Counter-based RNG: output does not depend on the

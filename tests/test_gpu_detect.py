@@ -93,3 +93,18 @@ def test_cli_labels_output(tmp_path, fixture_path, known):
     lab = np.array([0] + [int(x) for x in out.read_text().split()], np.uint32)
     g = scc.load_full_graph(fixture_path("sample128.gr"))
     assert_same_labels(lab, orc.tarjan(g)[0], "cli -o")
+
+
+def test_cli_reads_csr_cache(tmp_path, fixture_path):
+    """`scc -f` takes the binary CSR cache as well as GTgraph text, with the
+    same output lines."""
+    g = scc.load_full_graph(fixture_path("sample128.gr"))
+    p = tmp_path / "sample128.csr"
+    scc.write_csr(str(p), g)
+    outs = []
+    for f in (fixture_path("sample128.gr"), str(p)):
+        r = subprocess.run([scc.CLI_PATH, "-a", "g", "-p", "1", "-q", "1", "-w", "1", "-f", f],
+                           stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True, timeout=120)
+        assert r.returncode == 0, r.stderr
+        outs.append(r.stdout.split("Time")[0])
+    assert outs[0] == outs[1] and "Number Of Sccs : 13" in outs[1]

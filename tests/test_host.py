@@ -276,3 +276,46 @@ def test_batch_from_graphs_roundtrip():
         h = b.graph(i)
         for name in ("Fr", "Fc", "Br", "Bc"):
             assert np.array_equal(getattr(h, name), getattr(g, name))
+
+
+def test_csr_cache_roundtrip(tmp_path, fixture_path):
+    for g in (scc.load_full_graph(fixture_path("sample128.gr")), scc.gen_rmat(10, seed=4),
+              scc.from_arcs(3, [], [])):
+        p = str(tmp_path / "g.csr")
+        scc.write_csr(p, g)
+        h = scc.load_graph(p)
+        assert (h.V, h.E) == (g.V, g.E)
+        for a in ["Fr", "Fc", "Br", "Bc"]:
+            assert np.array_equal(getattr(g, a), getattr(h, a))
+    # the text format still loads through the same entry point
+    t = scc.load_graph(fixture_path("sample1.gr"))
+    assert np.array_equal(t.Fc, scc.load_full_graph(fixture_path("sample1.gr")).Fc)
+
+
+def test_csr_cache_rejects_damage(tmp_path, fixture_path):
+    g = scc.load_full_graph(fixture_path("sample128.gr"))
+    p = tmp_path / "g.csr"
+    scc.write_csr(str(p), g)
+    raw = bytearray(p.read_bytes())
+    hdr = 24
+    cases = {
+        "truncated": bytes(raw[:-4]),
+        "bad magic": b"X" + bytes(raw[1:]),
+        # an out-of-range column id in Fc
+        "bad id": bytes(raw[:hdr + 4 * (g.V + 2)]) + (g.V + 1).to_bytes(4, "little") + bytes(raw[hdr + 4 * (g.V + 3):]),
+        # reverse CSR no longer matches: move one in-arc between vertices 1 and 2
+        "reverse": None,
+    }
+    br_off = hdr + 4 * (g.V + 2) + 4 * g.E
+    br = np.frombuffer(bytes(raw[br_off:br_off + 4 * (g.V + 2)]), dtype=np.uint32).copy()
+    if br[2] > br[1]:
+        br[2] -= 1
+    else:
+        br[2] += 1
+    cases["reverse"] = bytes(raw[:br_off]) + br.tobytes() + bytes(raw[br_off + 4 * (g.V + 2):])
+    for why, data in cases.items():
+        q = tmp_path / f"{why.replace(' ', '_')}.csr"
+        q.write_bytes(data)
+        with pytest.raises(scc.SCCError) as e:
+            scc.load_graph(str(q))   # without the magic it is parsed as GTgraph text
+        assert e.value.status == 3, why
