@@ -458,6 +458,14 @@ class Context:
         _check(lib().scc_labels(self._ctx, _p32(lab)))
         return lab
 
+    def condensation(self) -> tuple[Graph, np.ndarray]:
+        """The DAG of SCCs of the last run (node k = k-th smallest canonical
+        label) and each vertex's node (V+1 entries)."""
+        node_of = np.zeros(self.graph.V + 1, np.uint32)
+        hg = _HostGraph()
+        _check(lib().scc_condensation(self._ctx, ctypes.byref(hg), _p32(node_of)))
+        return _take(hg), node_of
+
     def solve(self, g: Graph, **kw) -> tuple[np.ndarray, Stats]:
         self.upload(g)
         st = self.run(**kw)

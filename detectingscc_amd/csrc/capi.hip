@@ -89,6 +89,14 @@ scc_status scc_labels(scc_ctx *ctx, uint32_t *lab)
     return ctx->eng.labels(lab);
 }
 
+scc_status scc_condensation(scc_ctx *ctx, scc_host_graph *dag, uint32_t *node_of)
+{
+    clear_error();
+    CTX_CHECK(ctx);
+    if (!dag) return fail(SCC_ERR_INVALID, "null output graph");
+    return ctx->eng.condense(dag, node_of);
+}
+
 scc_status scc_solve(scc_ctx *ctx, const scc_host_graph *g, const uint32_t *part,
                      const scc_opts *opts, uint32_t *lab, scc_stats *stats)
 {

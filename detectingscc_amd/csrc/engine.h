@@ -24,6 +24,7 @@ class Engine {
     scc_status run(const scc_opts &o, scc_stats *out);
     scc_status phase_local(uint32_t *out);
     scc_status labels(uint32_t *lab);
+    scc_status condense(scc_host_graph *out, uint32_t *node_of);
 
     scc_status phase_trim1(const uint8_t *dead, uint8_t *removed);
     scc_status phase_trim2(const uint8_t *dead, uint32_t *pair);

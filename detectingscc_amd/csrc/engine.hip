@@ -786,6 +786,91 @@ scc_status Engine::detect_phases(const scc_opts &o, int expand, int W, scc_stats
     return SCC_OK;
 }
 
+// The condensation DAG of the last run: node k = the SCC with the k-th
+// smallest canonical label, an arc per distinct pair of SCCs joined by an
+// arc.  The quotient kernels of the detectSCC merge count and place the arcs
+// between different SCCs on the device (no vertex excluded: an empty
+// finished bitmap); the host sorts, removes parallel arcs and builds the
+// reverse rows.
+scc_status Engine::condense(scc_host_graph *out, uint32_t *node_of)
+{
+    if (!ran) return fail(SCC_ERR_NOTREADY, "no completed run");
+    std::memset(out, 0, sizeof *out);
+    const size_t nv = (size_t)g.V + 1;
+    uint32_t *none, *isrep, *qid, *cout, *cin, *Qr, *QBr, *tiles;
+    scc_status s;
+    if ((s = scratch(buf[0], st.nwords, &none)) != SCC_OK || (s = scratch(buf[1], nv, &isrep)) != SCC_OK ||
+        (s = scratch(buf[2], nv, &qid)) != SCC_OK || (s = scratch(buf[4], nv + 1, &cout)) != SCC_OK ||
+        (s = scratch(buf[5], nv + 1, &cin)) != SCC_OK || (s = scratch(buf[6], nv + 1, &Qr)) != SCC_OK ||
+        (s = scratch(buf[7], nv + 1, &QBr)) != SCC_OK ||
+        (s = scratch(buf[8], scan_tile_count(nv + 1), &tiles)) != SCC_OK)
+        return s;
+    k_fill_u32(stream, none, 0u, st.nwords);
+    k_q_mark(stream, g, st, none, isrep);
+    k_exscan(stream, isrep, qid, nv, tiles);
+    std::vector<uint32_t> hq(nv), hlab(nv), hrep(nv);
+    HIP_TRY(hipMemcpyAsync(hq.data(), qid, sizeof(uint32_t) * nv, hipMemcpyDeviceToHost, stream));
+    HIP_TRY(hipMemcpyAsync(hrep.data(), isrep, sizeof(uint32_t) * nv, hipMemcpyDeviceToHost, stream));
+    HIP_TRY(hipMemcpyAsync(hlab.data(), st.lab, sizeof(uint32_t) * nv, hipMemcpyDeviceToHost, stream));
+    HIP_TRY(hipStreamSynchronize(stream));
+    const uint32_t QV = hq[g.V] + hrep[g.V];
+    if (node_of) {
+        node_of[0] = 0;
+        for (uint32_t v = 1; v <= g.V; v++) node_of[v] = hq[hlab[v]] + 1;
+    }
+    std::vector<uint32_t> src, dst;
+    if (QV) {
+        k_fill_u32(stream, cout, 0, (uint64_t)QV + 2);
+        k_fill_u32(stream, cin, 0, (uint64_t)QV + 2);
+        k_q_arcs(stream, 0, g, st, none, qid, cout, cin, nullptr, nullptr);
+        k_exscan(stream, cout, Qr, (uint64_t)QV + 2, tiles);
+        k_exscan(stream, cin, QBr, (uint64_t)QV + 2, tiles);
+        std::vector<uint32_t> hr((size_t)QV + 2);
+        HIP_TRY(hipMemcpyAsync(hr.data(), Qr, sizeof(uint32_t) * hr.size(), hipMemcpyDeviceToHost, stream));
+        HIP_TRY(hipStreamSynchronize(stream));
+        const uint32_t QE = hr[QV + 1];
+        if (QE) {
+            uint32_t *Qc, *QBc;
+            if ((s = scratch(buf[9], QE, &Qc)) != SCC_OK || (s = scratch(buf[10], QE, &QBc)) != SCC_OK)
+                return s;
+            HIP_TRY(hipMemcpyAsync(cout, Qr, sizeof(uint32_t) * ((size_t)QV + 2), hipMemcpyDeviceToDevice, stream));
+            HIP_TRY(hipMemcpyAsync(cin, QBr, sizeof(uint32_t) * ((size_t)QV + 2), hipMemcpyDeviceToDevice, stream));
+            k_q_arcs(stream, 1, g, st, none, qid, cout, cin, Qc, QBc);
+            dst.resize(QE);
+            HIP_TRY(hipMemcpyAsync(dst.data(), Qc, sizeof(uint32_t) * QE, hipMemcpyDeviceToHost, stream));
+            HIP_TRY(hipStreamSynchronize(stream));
+            src.resize(QE);
+            for (uint32_t q = 1; q <= QV; q++)
+                for (uint32_t i = hr[q]; i < hr[q + 1]; i++) src[i] = q;
+        }
+    }
+    scc_host_graph dag;
+    const uint32_t none_arc = 0;
+    if ((s = scc_csr_from_arcs(QV, src.size(), src.empty() ? &none_arc : src.data(),
+                               dst.empty() ? &none_arc : dst.data(), &dag)) != SCC_OK)
+        return s;
+    // parallel arcs out: rows are sorted, so keep the first of each run
+    auto dedupe = [&](uint32_t *R, uint32_t *C) {
+        uint32_t w = 0;
+        for (uint32_t v = 1; v <= QV; v++) {
+            const uint32_t b = R[v], e = R[v + 1];
+            R[v] = w;
+            for (uint32_t i = b; i < e; i++)
+                if (i == b || C[i] != C[i - 1]) C[w++] = C[i];
+        }
+        R[QV + 1] = w;
+        return w;
+    };
+    const uint32_t E1 = dedupe(dag.Fr, dag.Fc), E2 = dedupe(dag.Br, dag.Bc);
+    if (E1 != E2) {
+        scc_host_graph_free(&dag);
+        return fail(SCC_ERR_DEVICE, "condensation rows disagree (%u forward, %u reverse arcs)", E1, E2);
+    }
+    dag.E = E1;
+    *out = dag;
+    return SCC_OK;
+}
+
 scc_status Engine::labels(uint32_t *lab)
 {
     if (!ran) return fail(SCC_ERR_NOTREADY, "no completed run");

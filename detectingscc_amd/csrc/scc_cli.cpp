@@ -6,8 +6,9 @@
 // Same flags, parsing and stdout lines as the reference.  Extra options
 // (not in the reference): -P <partition file> for -a d (the reference reads
 // a hard-coded path, load.cpp:118), -o <file> writes the canonical label of
-// every vertex, one per line, -e <0..3> selects the frontier expansion,
-// -D <device> selects the HIP device.
+// every vertex, one per line, -c <file> writes the condensation DAG (one
+// node per SCC) as GTgraph text, -e <0..4> selects the frontier expansion,
+// -D <device> selects the HIP device.  -f also takes a binary CSR cache.
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -27,7 +28,8 @@ static void print_help()
     printf("-f input file\n");
     printf("Please note that the input graph must be in GTgraph format\n");
     printf("If not provided the results are undefined\n");
-    printf("Extra options: -P <partition file> (-a d), -o <label file>, -e <expand 0-3>, -D <device>\n");
+    printf("Extra options: -P <partition file> (-a d), -o <label file>, -c <condensation DAG file>,\n"
+           "  -e <expand 0-4>, -D <device>; -f also reads a binary CSR cache\n");
 }
 
 int main(int argc, char **argv)
@@ -36,12 +38,12 @@ int main(int argc, char **argv)
         print_help();
         return 1;
     }
-    const char *file = nullptr, *part_file = nullptr, *label_file = nullptr;
+    const char *file = nullptr, *part_file = nullptr, *label_file = nullptr, *dag_file = nullptr;
     char algo = 0;
     bool trim1 = true, trim2 = true;
     int warp = 1, expand = SCC_EXPAND_AUTO, device = 0;
     int c;
-    while ((c = getopt(argc, argv, "a:p:q:w:f:P:o:e:D:")) != -1) {
+    while ((c = getopt(argc, argv, "a:p:q:w:f:P:o:c:e:D:")) != -1) {
         switch (c) {
         case 'a': algo = optarg[0]; break;
         case 'p': trim1 = optarg[0] != '0'; break;   // scc.cu:39-41
@@ -50,6 +52,7 @@ int main(int argc, char **argv)
         case 'f': file = optarg; break;
         case 'P': part_file = optarg; break;
         case 'o': label_file = optarg; break;
+        case 'c': dag_file = optarg; break;
         case 'e': expand = atoi(optarg); break;
         case 'D': device = atoi(optarg); break;
         default: print_help(); return 1;
@@ -88,7 +91,7 @@ int main(int argc, char **argv)
         }
     }
     int rc;
-    if (expand == SCC_EXPAND_AUTO && !label_file && device == 0) {
+    if (expand == SCC_EXPAND_AUTO && !label_file && !dag_file && device == 0) {
         switch (algo) {
         case 'g': rc = vHong(g.E, g.V, g.Fc, g.Fr, g.Bc, g.Br, trim1, trim2); break;
         case 'h': rc = vSlota(g.E, g.V, g.Fc, g.Fr, g.Bc, g.Br, trim1, trim2); break;
@@ -130,6 +133,15 @@ int main(int argc, char **argv)
                 for (uint32_t v = 1; v <= g.V; v++) fprintf(f, "%u\n", lab[v]);
                 fclose(f);
             }
+        }
+        if (dag_file) {
+            scc_host_graph dag;
+            std::memset(&dag, 0, sizeof dag);
+            if (scc_condensation(ctx, &dag, nullptr) != SCC_OK || scc_write_gtgraph(dag_file, &dag) != SCC_OK) {
+                fprintf(stderr, "%s\n", scc_last_error());
+                rc = 1;
+            }
+            scc_host_graph_free(&dag);
         }
         scc_ctx_destroy(ctx);
     }

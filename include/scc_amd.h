@@ -187,6 +187,13 @@ scc_status scc_partition_upload(scc_ctx *ctx, const uint32_t *part);
 scc_status scc_run(scc_ctx *ctx, const scc_opts *opts, scc_stats *stats);
 /* Copy the canonical labels of the last run (V+1 entries). */
 scc_status scc_labels(scc_ctx *ctx, uint32_t *lab);
+/* The condensation DAG of the last scc_run (no reference counterpart: the
+ * reference prints only a count, scc_WCC.cu:966): node k (1-based) is the
+ * SCC with the k-th smallest canonical label, one arc per distinct ordered
+ * pair of SCCs joined by at least one arc, in the reference layout (arrays
+ * from malloc, free with scc_host_graph_free).  node_of (V+1 entries, may be
+ * NULL) receives each vertex's node. */
+scc_status scc_condensation(scc_ctx *ctx, scc_host_graph *dag, uint32_t *node_of);
 
 /* Upload + run + labels in one call. */
 scc_status scc_solve(scc_ctx *ctx, const scc_host_graph *g, const uint32_t *part,
