@@ -49,8 +49,9 @@ __device__ __forceinline__ void push_wave(bool claimed, uint32_t w, uint32_t *qo
 template <int W, bool SINGLE>
 __global__ __launch_bounds__(256) void expand_vw_kernel(LevelArgs a, uint32_t bpd)
 {
-    const int dir = (a.ndirs == 2 && blockIdx.x >= bpd) ? 1 : 0;
-    const uint32_t bid = blockIdx.x - (uint32_t)dir * bpd;
+    int dir;
+    uint32_t bid;
+    split_dirs(a.ndirs, bpd, dir, bid);
     const DirArgs &D = a.d[dir];
     const int L = a.level;
     const uint32_t n = D.cnt[CT(L % 3)];
@@ -116,8 +117,9 @@ __global__ __launch_bounds__(256) void expand_lb_kernel(LevelArgs a, uint32_t bp
     __shared__ uint32_t s_beg[LB_TILE];
     __shared__ uint32_t s_cls[LB_TILE];
     __shared__ uint32_t s_wsum[4];
-    const int dir = (a.ndirs == 2 && blockIdx.x >= bpd) ? 1 : 0;
-    const uint32_t bid = blockIdx.x - (uint32_t)dir * bpd;
+    int dir;
+    uint32_t bid;
+    split_dirs(a.ndirs, bpd, dir, bid);
     const DirArgs &D = a.d[dir];
     const int L = a.level;
     const uint32_t n = D.cnt[CT(L % 3)];
@@ -207,6 +209,7 @@ __global__ __launch_bounds__(256) void expand_lb_kernel(LevelArgs a, uint32_t bp
 
 static void launch_lb(hipStream_t s, const LevelArgs &a, bool single, uint32_t bpd)
 {
+    bpd = dir_blocks(bpd, a.ndirs);
     const uint32_t grid = bpd * (uint32_t)a.ndirs;
     if (single) expand_lb_kernel<true><<<grid, 256, 0, s>>>(a, bpd);
     else expand_lb_kernel<false><<<grid, 256, 0, s>>>(a, bpd);
@@ -253,8 +256,9 @@ __global__ __launch_bounds__(256) void expand_bu_kernel(BottomUpArgs a, uint32_t
 {
     __shared__ uint32_t s_q[4][BU_WQ];
     __shared__ uint32_t s_long[4][64];
-    const int dir = (a.ndirs == 2 && blockIdx.x >= bpd) ? 1 : 0;
-    const uint32_t bid = blockIdx.x - (uint32_t)dir * bpd;
+    int dir;
+    uint32_t bid;
+    split_dirs(a.ndirs, bpd, dir, bid);
     const DirArgs &D = a.d[dir];
     const uint32_t *__restrict__ PR = a.PR[dir];
     const uint32_t *__restrict__ PC = a.PC[dir];
@@ -322,7 +326,7 @@ __global__ __launch_bounds__(256) void expand_bu_kernel(BottomUpArgs a, uint32_t
 void k_bottom_up(hipStream_t s, const BottomUpArgs &a, bool single_class)
 {
     uint64_t b = ((uint64_t)a.V + 256) / 256;
-    const uint32_t bpd = (uint32_t)(b > 2048 ? 2048 : b);
+    const uint32_t bpd = dir_blocks((uint32_t)(b > 2048 ? 2048 : b), a.ndirs);
     const uint32_t grid = bpd * (uint32_t)a.ndirs;
     if (single_class)
         expand_bu_kernel<true><<<grid, 256, 0, s>>>(a, bpd);
@@ -333,6 +337,7 @@ void k_bottom_up(hipStream_t s, const BottomUpArgs &a, bool single_class)
 template <int W>
 static void launch_vw(hipStream_t s, const LevelArgs &a, bool single, uint32_t bpd)
 {
+    bpd = dir_blocks(bpd, a.ndirs);
     const uint32_t grid = bpd * (uint32_t)a.ndirs;
     if (single)
         expand_vw_kernel<W, true><<<grid, 256, 0, s>>>(a, bpd);

@@ -122,6 +122,28 @@ struct LevelArgs {
     int level;                   // selects the rotating counters
 };
 
+// Two-direction launches (grid = 2 * bpd).  Workgroups are dispatched to the
+// eight XCDs round-robin (workgroup x runs on XCD x mod 8), so taking the
+// direction from bit 2 of x runs direction 0 on XCDs 0-3 and direction 1 on
+// XCDs 4-7: each XCD's L2 then caches one direction's bitmap instead of
+// both.  Needs bpd % 4 == 0 when ndirs == 2 (dir_blocks rounds it up).
+__device__ __forceinline__ void split_dirs(int ndirs, uint32_t bpd, int &dir, uint32_t &bid)
+{
+    const uint32_t x = blockIdx.x;
+    if (ndirs == 2) {
+        dir = (int)((x >> 2) & 1u);
+        bid = ((x >> 3) << 2) | (x & 3u);
+    } else {
+        dir = 0;
+        bid = x;
+    }
+    (void)bpd;
+}
+__host__ __forceinline__ uint32_t dir_blocks(uint32_t bpd, int ndirs)
+{
+    return ndirs == 2 ? (bpd + 3u) & ~3u : bpd;
+}
+
 // ---- sweep kernels (kernels_sweep.hip) ----
 void k_fill_u32(hipStream_t s, uint32_t *p, uint32_t val, uint64_t n);
 void k_init_state(hipStream_t s, const DevGraph &g, DevState &st, const uint8_t *dead_dev);
