@@ -230,15 +230,18 @@ static void launch_lb(hipStream_t s, const LevelArgs &a, bool single, uint32_t b
 // listed in LDS and scanned by groups of BU_GROUP lanes, BU_GROUP parents
 // per step, with a ballot to stop each group at its first hit.
 //
-// The pass is wave-centric: each wave streams its own 64-vertex chunks with
-// a private LDS queue and long list and never waits on its workgroup, so
-// the waves of a CU overlap their chains of dependent loads (vertex word,
-// row offsets, parent ids, parent bits).  A wave whose queue fills reserves
-// global space itself; what remains at the end goes out with one atomic per
-// workgroup (same-address atomics serialise).
+// The pass is wave-centric: each wave streams its own 64-vertex chunks,
+// two at a time (each lane carries one vertex of each, so twice as many
+// chains of dependent loads -- vertex word, row offsets, parent ids, parent
+// bits -- are in flight), with a private LDS queue and long list, and never
+// waits on its workgroup.  A wave whose queue fills reserves global space
+// itself; what remains at the end goes out with one atomic per workgroup
+// (same-address atomics serialise).  Only u's lane sets u's bit in this
+// pass, so the claim is an atomicOr whose old value is not waited for.
 constexpr uint32_t BU_SHORT = 4;
 constexpr uint32_t BU_GROUP = 8;
 constexpr uint32_t BU_WQ = 1024;
+constexpr uint32_t BU_CHUNKS = 2;   // 64-vertex chunks per wave iteration
 
 // One 64-bit load gives p's visited-or-finished and finished bits.
 template <bool SINGLE>
@@ -249,13 +252,12 @@ __device__ __forceinline__ bool visited_parent(const uint32_t *vis, const uint32
     return ((uint32_t)w & ~(uint32_t)(w >> 32) & pb) && (SINGLE || cls[p] == c);
 }
 
-// Both directions of a level share one launch: the first `bpd` workgroups
-// run direction a.d[0], the rest a.d[1].
+// Both directions of a level share one launch (split_dirs).
 template <bool SINGLE>
 __global__ __launch_bounds__(256) void expand_bu_kernel(BottomUpArgs a, uint32_t bpd)
 {
     __shared__ uint32_t s_q[4][BU_WQ];
-    __shared__ uint32_t s_long[4][64];
+    __shared__ uint32_t s_long[4][BU_CHUNKS * 64];
     int dir;
     uint32_t bid;
     split_dirs(a.ndirs, bpd, dir, bid);
@@ -276,26 +278,44 @@ __global__ __launch_bounds__(256) void expand_bu_kernel(BottomUpArgs a, uint32_t
     uint32_t nq = 0;
     const uint32_t sub = lane % BU_GROUP, grp = lane / BU_GROUP;
     const uint32_t gshift = lane & ~(BU_GROUP - 1);
-    for (uint32_t base = (bid * 4 + wv) * 64; base < V; base += bpd * 256) {
-        if (nq + 128 > BU_WQ) wave_flush(wq, nq, cnt_out, D.qout);
-        const uint32_t u = base + lane + 1;
-        bool found = false, need_long = false;
-        if (u <= V && !(D.vis[vword(u)] & (1u << (u & 31)))) {
-            const uint32_t c = SINGLE ? 0u : cls[u];
-            const uint32_t b = PR[u], e = PR[u + 1];
-            uint32_t p[BU_SHORT];
+    const uint32_t stride = bpd * 256;   // one chunk of every wave of the grid
+    for (uint32_t base = (bid * 4 + wv) * 64; base < V; base += BU_CHUNKS * stride) {
+        if (nq + 2 * BU_CHUNKS * 64 > BU_WQ) wave_flush(wq, nq, cnt_out, D.qout);
+        uint32_t u[BU_CHUNKS], c[BU_CHUNKS], b[BU_CHUNKS], e[BU_CHUNKS];
+        bool cand[BU_CHUNKS], found[BU_CHUNKS], need_long[BU_CHUNKS];
 #pragma unroll
-            for (uint32_t t = 0; t < BU_SHORT; t++) p[t] = b + t < e ? PC[b + t] : 0u;
+        for (uint32_t h = 0; h < BU_CHUNKS; h++) {
+            u[h] = base + h * stride + lane + 1;
+            cand[h] = u[h] <= V && !(D.vis[vword(u[h])] & (1u << (u[h] & 31)));
+        }
+#pragma unroll
+        for (uint32_t h = 0; h < BU_CHUNKS; h++) {
+            c[h] = (SINGLE || !cand[h]) ? 0u : cls[u[h]];
+            b[h] = cand[h] ? PR[u[h]] : 0u;
+            e[h] = cand[h] ? PR[u[h] + 1] : 0u;
+        }
+        uint32_t p[BU_CHUNKS][BU_SHORT];
+#pragma unroll
+        for (uint32_t h = 0; h < BU_CHUNKS; h++)
+#pragma unroll
+            for (uint32_t t = 0; t < BU_SHORT; t++) p[h][t] = b[h] + t < e[h] ? PC[b[h] + t] : 0u;
+#pragma unroll
+        for (uint32_t h = 0; h < BU_CHUNKS; h++) {
+            found[h] = false;
 #pragma unroll
             for (uint32_t t = 0; t < BU_SHORT; t++)
-                found |= b + t < e && visited_parent<SINGLE>(D.vis, cls, p[t], c);
-            need_long = !found && e - b > BU_SHORT;
-            if (found) found = !(atomicOr(&D.vis[vword(u)], 1u << (u & 31)) & (1u << (u & 31)));
+                found[h] |= b[h] + t < e[h] && visited_parent<SINGLE>(D.vis, cls, p[h][t], c[h]);
+            need_long[h] = !found[h] && e[h] - b[h] > BU_SHORT;
+            if (found[h]) atomicOr(&D.vis[vword(u[h])], 1u << (u[h] & 31));
         }
-        wave_push(wq, nq, found, u);
-        const uint64_t lm = __ballot(need_long);
-        if (need_long) wl[__popcll(lm & ((1ull << lane) - 1))] = u;
-        const uint32_t nlong = (uint32_t)__popcll(lm);
+        uint32_t nlong = 0;
+#pragma unroll
+        for (uint32_t h = 0; h < BU_CHUNKS; h++) {
+            wave_push(wq, nq, found[h], u[h]);
+            const uint64_t lm = __ballot(need_long[h]);
+            if (need_long[h]) wl[nlong + __popcll(lm & ((1ull << lane) - 1))] = u[h];
+            nlong += (uint32_t)__popcll(lm);
+        }
         wave_sync();
         // long parent lists: 64 / BU_GROUP vertices per round, each group
         // stops at its first hit
@@ -305,20 +325,19 @@ __global__ __launch_bounds__(256) void expand_bu_kernel(BottomUpArgs a, uint32_t
             uint32_t x = 0;
             if (k < nlong) {
                 x = wl[k];
-                const uint32_t c = SINGLE ? 0u : cls[x];
-                const uint32_t e = PR[x + 1];
-                for (uint32_t j0 = PR[x] + BU_SHORT; j0 < e && !hit; j0 += BU_GROUP) {
+                const uint32_t cx = SINGLE ? 0u : cls[x];
+                const uint32_t ex = PR[x + 1];
+                for (uint32_t j0 = PR[x] + BU_SHORT; j0 < ex && !hit; j0 += BU_GROUP) {
                     const uint32_t j = j0 + sub;
-                    const bool h = j < e && visited_parent<SINGLE>(D.vis, cls, PC[j], c);
+                    const bool h = j < ex && visited_parent<SINGLE>(D.vis, cls, PC[j], cx);
                     hit = ((__ballot(h) >> gshift) & ((1ull << BU_GROUP) - 1)) != 0;
                 }
             }
-            bool claimed = false;
-            if (hit && sub == 0)
-                claimed = !(atomicOr(&D.vis[vword(x)], 1u << (x & 31)) & (1u << (x & 31)));
+            const bool claimed = hit && sub == 0;
+            if (claimed) atomicOr(&D.vis[vword(x)], 1u << (x & 31));
             wave_push(wq, nq, claimed, x);
         }
-        wave_sync();   // the next chunk rewrites wl
+        wave_sync();   // the next iteration rewrites wl
     }
     wave_flush_block(wq, nq, cnt_out, D.qout);
 }

@@ -108,30 +108,51 @@ void k_init_state(hipStream_t s, const DevGraph &g, DevState &st, const uint8_t 
 // ---------------------------------------------------------- trim-1 ---
 // scc_kernels.cu:187-233: a live vertex with no live same-class
 // in-neighbour, or no live same-class out-neighbour, is a singleton SCC.
-
 //
 // The first pass sweeps every vertex; each later pass re-checks only the
 // live neighbours of the vertices the previous pass removed (a vertex can
 // only become removable when a neighbour goes), so the fixpoint costs one
 // sweep plus work proportional to the removals (while a pass removes a
 // large share of the vertices, the next one sweeps again: walking the arcs
-// of millions of fresh removals costs more than a sweep).  A candidate is tested once
-// per pass (the first lane to stamp it with the pass's epoch), and removal
-// is a CAS on the class word, so each removal is listed once.  Pass k reads list count C_TLIST + k%3, appends
-// to C_TLIST + (k+1)%3 and clears C_TLIST + (k+2)%3.
+// of millions of fresh removals costs more than a sweep).  In a list pass a
+// candidate is tested once (the first lane to stamp it with the pass's
+// epoch) and removal is a CAS on the class word, so each removal is listed
+// once; in a sweep each vertex has one lane.  Pass k reads list count
+// C_TLIST + k%3, appends to C_TLIST + (k+1)%3 and clears C_TLIST + (k+2)%3.
 
-// v (live, class c) keeps a live same-class in-neighbour and out-neighbour
+// v (live, class c) keeps a live same-class in-neighbour and out-neighbour.
+// The first TRIM_SHORT arcs each way are probed with independent loads (row
+// offsets, then column entries, then class words: three dependent steps
+// instead of seven), and only a vertex with no hit among them walks the rest
+// of a list.  The sweep is bound by the latency of these chains.
+constexpr uint32_t TRIM_SHORT = 1;
+
+__device__ __forceinline__ bool trim1_keep_at(const uint32_t *__restrict__ Fc, const uint32_t *__restrict__ Bc,
+                                              const uint32_t *cls, uint32_t c, uint32_t fb, uint32_t fe,
+                                              uint32_t bb, uint32_t be)
+{
+    uint32_t pin[TRIM_SHORT], pout[TRIM_SHORT];
+#pragma unroll
+    for (uint32_t t = 0; t < TRIM_SHORT; t++) {
+        pin[t] = bb + t < be ? Bc[bb + t] : 0u;
+        pout[t] = fb + t < fe ? Fc[fb + t] : 0u;
+    }
+    bool kin = false, kout = false;
+#pragma unroll
+    for (uint32_t t = 0; t < TRIM_SHORT; t++) {
+        kin |= bb + t < be && cls[pin[t]] == c;
+        kout |= fb + t < fe && cls[pout[t]] == c;
+    }
+    for (uint32_t i = bb + TRIM_SHORT; !kin && i < be; i++) kin = cls[Bc[i]] == c;
+    for (uint32_t i = fb + TRIM_SHORT; kin && !kout && i < fe; i++) kout = cls[Fc[i]] == c;
+    return kin && kout;
+}
+
 __device__ __forceinline__ bool trim1_keep(const uint32_t *__restrict__ Fr, const uint32_t *__restrict__ Fc,
                                            const uint32_t *__restrict__ Br, const uint32_t *__restrict__ Bc,
                                            const uint32_t *cls, uint32_t v, uint32_t c)
 {
-    bool keep = false;
-    for (uint32_t i = Br[v], e = Br[v + 1]; i < e; i++)
-        if (cls[Bc[i]] == c) { keep = true; break; }
-    if (!keep) return false;
-    for (uint32_t i = Fr[v], e = Fr[v + 1]; i < e; i++)
-        if (cls[Fc[i]] == c) return true;
-    return false;
+    return trim1_keep_at(Fc, Bc, cls, c, Fr[v], Fr[v + 1], Br[v], Br[v + 1]);
 }
 
 __device__ __forceinline__ bool trim1_remove(uint32_t *cls, uint32_t *lab, uint32_t v, uint32_t c)
@@ -176,8 +197,15 @@ __global__ __launch_bounds__(256) void trim1_kernel(const uint32_t *__restrict__
             const uint32_t v = base + lane() + 1;
             bool gone = false;
             if (v <= V) {
+                // the row offsets load beside the class word; only this
+                // lane writes v's words in a sweep, so removal is two stores
                 const uint32_t c = cls[v];
-                if (c != DEAD && !trim1_keep(Fr, Fc, Br, Bc, cls, v, c)) gone = trim1_remove(cls, lab, v, c);
+                const uint32_t fb = Fr[v], fe = Fr[v + 1], bb = Br[v], be = Br[v + 1];
+                if (c != DEAD && !trim1_keep_at(Fc, Bc, cls, c, fb, fe, bb, be)) {
+                    cls[v] = DEAD;
+                    lab[v] = v | LAB_FINAL;
+                    gone = true;
+                }
             }
             removed += gone;
             wave_push(wq, nq, gone, v);
