@@ -49,6 +49,8 @@ class Engine {
     void collect_times(scc_stats *s);
     uint32_t bpd(int W) const;
     scc_status reach(bool fw, bool bw, bool single, int expand, int W, uint32_t *levels_out);
+    scc_status reach_do(bool fw, bool bw, bool single, uint32_t *levels_out);
+    DirArgs dir_args(int d, uint32_t level) const;
     scc_status trim1(bool fixpoint);
     void init_vis();
     uint64_t limit() const;

@@ -381,126 +381,148 @@ void Engine::init_vis()
     k_init_vis(stream, g, st);
 }
 
-// Direction-optimising switch points (Beamer et al.): go bottom-up when the
-// frontier holds more than 1/kToBottomUp of the vertices still unvisited,
-// back to top-down when it falls under 1/kToTopDown of the live vertices.
-static constexpr int64_t kToBottomUp = 16;
-static constexpr int64_t kToTopDown = 48;
-// While every active direction is top-down, has no hubs pending, grew at
-// most kSpecRise-fold since the previous check and would have to grow
-// kSpecGrowth-fold more to reach the bottom-up switch point, the host queues
-// kSpecLevels levels before it reads the counters again (high-diameter
-// graphs run thousands of small levels; a host round trip per level would
-// cost more than the level).  A frontier that empties inside the batch
-// turns the remaining launches into no-ops.
-static constexpr int kSpecLevels = 16;
-static constexpr int64_t kSpecGrowth = 8;
-static constexpr int64_t kSpecRise = 2;
+// The level-L arguments of direction d (0 = FW, 1 = BW): queues alternate
+// with the level, counts and hub lists rotate through three slots.
+DirArgs Engine::dir_args(int d, uint32_t level) const
+{
+    DirArgs D;
+    const int in = level & 1, out = in ^ 1;
+    const int hub = d ? C_BIGB : C_BIGF;
+    const int cur = level % 3, prev = (level + 2) % 3, next = (level + 1) % 3;
+    D.R = d ? g.Br : g.Fr;
+    D.C = d ? g.Bc : g.Fc;
+    D.vis = d ? st.bvis : st.fvis;
+    D.qin = d ? st.qb[in] : st.qf[in];
+    D.qout = d ? st.qb[out] : st.qf[out];
+    D.cnt = st.ctr + CT(d ? C_BWCNT : C_FWCNT);
+    D.big = d ? st.bigb[cur] : st.bigf[cur];
+    D.bigoff = d ? st.bigoff_b[cur] : st.bigoff_f[cur];
+    D.bigc = reinterpret_cast<unsigned long long *>(st.ctr + CT(hub + cur));
+    D.pbig = d ? st.bigb[prev] : st.bigf[prev];
+    D.pbigoff = d ? st.bigoff_b[prev] : st.bigoff_f[prev];
+    D.pbigc = reinterpret_cast<unsigned long long *>(st.ctr + CT(hub + prev));
+    D.bigc_zero = reinterpret_cast<unsigned long long *>(st.ctr + CT(hub + next));
+    return D;
+}
 
-// FW and/or BW search from the seeded queues until both frontiers are empty.
+// FW and/or BW search from the seeded queues until both frontiers are
+// empty, `K` levels per look at the counters (reference mappings and LB).
 scc_status Engine::reach(bool fw, bool bw, bool single, int expand, int W, uint32_t *levels_out)
 {
-    const bool dopt = expand == SCC_EXPAND_DO;
-    const int td_expand = dopt ? SCC_EXPAND_LB : expand;
-    bool active[2] = {fw, bw};
-    bool bottom_up[2] = {false, false};
-    int64_t visited[2] = {0, 0};
-    int64_t nf_prev[2] = {0, 0};
-    bool counted_seeds = false;
     if (!fw && !bw) return SCC_OK;
-    // DO mode decides after every level unless the frontiers are far from
-    // the switch point (then kSpecLevels levels are queued per decision)
-    const int K = dopt ? 1 : (check_every > 0 ? check_every : 2);
-    const uint32_t blocks = bpd(td_expand == SCC_EXPAND_VWARP ? W : 1);
+    if (expand == SCC_EXPAND_DO) return reach_do(fw, bw, single, levels_out);
+    const int K = check_every > 0 ? check_every : 2;
+    const uint32_t blocks = bpd(expand == SCC_EXPAND_VWARP ? W : 1);
     uint32_t level = 0;
-    int queued = 0, batch = 1;
-    uint32_t spec_blocks = blocks;
     for (;;) {
-        const int in = level & 1, out = in ^ 1;
-        DirArgs D[2];
-        for (int d = 0; d < 2; d++) {
-            const int hub = d ? C_BIGB : C_BIGF;
-            D[d].R = d ? g.Br : g.Fr;
-            D[d].C = d ? g.Bc : g.Fc;
-            D[d].vis = d ? st.bvis : st.fvis;
-            D[d].qin = d ? st.qb[in] : st.qf[in];
-            D[d].qout = d ? st.qb[out] : st.qf[out];
-            D[d].cnt = st.ctr + CT(d ? C_BWCNT : C_FWCNT);
-            const int cur = level % 3, prev = (level + 2) % 3, next = (level + 1) % 3;
-            D[d].big = d ? st.bigb[cur] : st.bigf[cur];
-            D[d].bigoff = d ? st.bigoff_b[cur] : st.bigoff_f[cur];
-            D[d].bigc = reinterpret_cast<unsigned long long *>(st.ctr + CT(hub + cur));
-            D[d].pbig = d ? st.bigb[prev] : st.bigf[prev];
-            D[d].pbigoff = d ? st.bigoff_b[prev] : st.bigoff_f[prev];
-            D[d].pbigc = reinterpret_cast<unsigned long long *>(st.ctr + CT(hub + prev));
-            D[d].bigc_zero = reinterpret_cast<unsigned long long *>(st.ctr + CT(hub + next));
-        }
         LevelArgs a;
         std::memset(&a, 0, sizeof a);
         a.cls = st.cls;
         a.level = (int)level;
-        for (int d = 0; d < 2; d++)
-            if (active[d] && !bottom_up[d]) a.d[a.ndirs++] = D[d];
-        if (a.ndirs) k_expand_level(stream, a, td_expand, W, single, batch > 1 ? spec_blocks : blocks);
-        BottomUpArgs b;
-        std::memset(&b, 0, sizeof b);
-        b.cls = st.cls;
-        b.V = g.V;
-        b.level = (int)level;
-        for (int d = 0; d < 2; d++)
-            if (active[d] && bottom_up[d]) {
-                b.PR[b.ndirs] = d ? g.Fr : g.Br;
-                b.PC[b.ndirs] = d ? g.Fc : g.Bc;
-                b.d[b.ndirs++] = D[d];
-            }
-        if (b.ndirs) k_bottom_up(stream, b, single);
+        if (fw) a.d[a.ndirs++] = dir_args(0, level);
+        if (bw) a.d[a.ndirs++] = dir_args(1, level);
+        k_expand_level(stream, a, expand, W, single, blocks);
         level++;
-        if (dopt ? ++queued < batch : level % K != 0) continue;
-        queued = 0;
+        if (level % K != 0) continue;
         scc_status s = read_ctr();
         if (s != SCC_OK) return s;
-        const int64_t nlive = h_ctr[CT(C_NLIVE)];
-        bool any = false, spec = dopt;
-        int64_t nf_max = 0;
+        bool any = false;
         for (int d = 0; d < 2; d++) {
-            if (!active[d]) continue;
-            const int64_t nf = h_ctr[CT((d ? C_BWCNT : C_FWCNT) + level % 3)];
-            // hubs of the last top-down level still to expand (a bottom-up
-            // pass needs no list: it finds them as visited parents)
-            const uint64_t hubs = bottom_up[d] ? 0 : *reinterpret_cast<const unsigned long long *>(
+            if (!(d ? bw : fw)) continue;
+            const uint32_t nf = h_ctr[CT((d ? C_BWCNT : C_FWCNT) + level % 3)];
+            const uint64_t hubs = *reinterpret_cast<const unsigned long long *>(
                 h_ctr + CT((d ? C_BIGB : C_BIGF) + (level + 2) % 3)) >> 32;
-            if (!nf && !hubs) {
-                if (dopt) active[d] = false;
-                continue;
-            }
-            any = true;
-            if (!dopt) continue;
-            if (!counted_seeds) visited[d] += h_ctr[CT(C_NPIV)];
-            visited[d] += nf;
-            const int64_t remaining = nlive - visited[d];
-            if (!bottom_up[d] && nf * kToBottomUp > remaining) bottom_up[d] = true;
-            else if (bottom_up[d] && nf * kToTopDown < nlive) bottom_up[d] = false;
-            if (bottom_up[d] || hubs || !nf_prev[d] || nf > nf_prev[d] * kSpecRise ||
-                nf * kToBottomUp * kSpecGrowth > remaining)
-                spec = false;
-            nf_prev[d] = nf;
-            nf_max = std::max(nf_max, nf);
+            any |= nf || hubs;
         }
-        counted_seeds = true;
-        if (trace_levels)
-            std::fprintf(stderr, "[scc reach] level %u: fw %s %u, bw %s %u, live %lld, batch next %d\n",
-                         level, bottom_up[0] ? "BU" : "TD", active[0] ? h_ctr[CT(C_FWCNT + level % 3)] : 0u,
-                         bottom_up[1] ? "BU" : "TD", active[1] ? h_ctr[CT(C_BWCNT + level % 3)] : 0u,
-                         (long long)nlive, spec ? kSpecLevels : 1);
         if (!any) break;
         // every level claims at least one vertex, so V+1 levels bound a search
         if (level > limit()) return no_convergence("reachability");
-        batch = spec ? kSpecLevels : 1;
-        // queued levels get a grid sized for a few times the current frontier
-        spec_blocks = (uint32_t)std::min<int64_t>(blocks, std::max<int64_t>(32, (nf_max * 4 + 255) / 256));
     }
     if (levels_out) *levels_out += level;
     return SCC_OK;
+}
+
+// Direction-optimised search.  Each level is one launch whose workgroups
+// choose top-down or bottom-up per direction on the device (dir_decide), so
+// the host queues a window of levels per look at the counters: kDoWindow
+// levels on the full grid, or kSpecLevels levels on a grid sized for a few
+// times the current frontier while every direction is top-down, has no hubs
+// pending, grew at most kSpecRise-fold since the previous look and would
+// have to grow kSpecGrowth-fold more to reach the bottom-up switch point
+// (high-diameter graphs run thousands of small levels).  Levels queued past
+// the end of the search are no-ops.
+static constexpr int kDoWindow = 4;
+static constexpr int kSpecLevels = 16;
+static constexpr int64_t kSpecGrowth = 8;
+static constexpr int64_t kSpecRise = 2;
+
+scc_status Engine::reach_do(bool fw, bool bw, bool single, uint32_t *levels_out)
+{
+    int dirs[2], nd = 0;   // argument position -> direction
+    if (fw) dirs[nd++] = 0;
+    if (bw) dirs[nd++] = 1;
+    const uint32_t full = do_blocks(g.V, nd);
+    k_do_begin(stream, st.ctr, nd);
+    DoArgs a;
+    std::memset(&a, 0, sizeof a);
+    a.cls = st.cls;
+    a.ctr = st.ctr;
+    a.V = g.V;
+    a.ndirs = nd;
+    for (int i = 0; i < nd; i++) {
+        a.PR[i] = dirs[i] ? g.Fr : g.Br;
+        a.PC[i] = dirs[i] ? g.Fc : g.Bc;
+    }
+    int64_t nf_prev[2] = {0, 0};
+    uint32_t level = 0, blocks = full;
+    int window = kDoWindow;
+    for (;;) {
+        for (int k = 0; k < window; k++, level++) {
+            for (int i = 0; i < nd; i++) a.d[i] = dir_args(dirs[i], level);
+            a.level = (int)level;
+            k_do_level(stream, a, single, blocks);
+        }
+        scc_status s = read_ctr();
+        if (s != SCC_OK) return s;
+        // the state the next level would compute: levels 0..level-1 ran
+        const int64_t nlive = h_ctr[CT(C_NLIVE)];
+        bool any = false, spec = true;
+        int64_t nf_max = 0;
+        uint32_t end = 0;
+        for (int i = 0; i < nd; i++) {
+            const int d = dirs[i];
+            const uint32_t last = level - 1;
+            const DirState prev = load_dir_state(h_ctr, last == 0 ? 0 : last & 1, i);
+            const uint32_t nf = h_ctr[CT((d ? C_BWCNT : C_FWCNT) + level % 3)];
+            const uint32_t hubs = (uint32_t)(*reinterpret_cast<const unsigned long long *>(
+                                                 h_ctr + CT((d ? C_BIGB : C_BIGF) + last % 3)) >> 32);
+            const DirState next = dir_decide(prev, nf, hubs, (uint32_t)nlive, level);
+            if (trace_levels)
+                std::fprintf(stderr, "[scc reach] after level %u: %s %s %u (hubs %u), visited %llu\n", last,
+                             d ? "bw" : "fw",
+                             next.mode == DIR_OFF ? "done" : next.mode == DIR_BU ? "BU" : "TD", nf,
+                             prev.mode == DIR_BU ? 0u : hubs, (unsigned long long)next.visited);
+            if (next.mode == DIR_OFF) {
+                end = std::max(end, next.end);
+                continue;
+            }
+            any = true;
+            const int64_t remaining = nlive - (int64_t)next.visited;
+            if (next.mode == DIR_BU || (prev.mode == DIR_TD && hubs) || !nf_prev[i] ||
+                (int64_t)nf > nf_prev[i] * kSpecRise || (int64_t)nf * kToBottomUp * kSpecGrowth > remaining)
+                spec = false;
+            nf_prev[i] = nf;
+            nf_max = std::max<int64_t>(nf_max, nf);
+        }
+        if (!any) {
+            if (levels_out) *levels_out += end;
+            return SCC_OK;
+        }
+        // every level claims at least one vertex, so V+1 levels bound a search
+        if (level > limit()) return no_convergence("reachability");
+        window = spec ? kSpecLevels : kDoWindow;
+        blocks = spec ? (uint32_t)std::min<int64_t>(full, std::max<int64_t>(32, (nf_max * 4 + 255) / 256)) : full;
+    }
 }
 
 // trim-1 to its fixpoint (scc_WCC.cu:680-687), or one pass.  After the

@@ -110,18 +110,16 @@ __device__ __forceinline__ uint32_t block_exscan256(uint32_t x, uint32_t *wsum, 
     return base + incl - x;
 }
 
+// LDS words of a top-down level: the staging queue and its two words, then
+// the tile's adjacency offsets, row starts, classes and scan scratch.
+constexpr uint32_t LB_LDS = QBUF + 2 + (LB_TILE + 1) + 3 * LB_TILE + 4;
+
+// One top-down level of direction D for workgroup `bid` of `bpd`; `lds`
+// holds LB_LDS words.  Called by the whole workgroup.
 template <bool SINGLE>
-__global__ __launch_bounds__(256) void expand_lb_kernel(LevelArgs a, uint32_t bpd)
+__device__ __forceinline__ void lb_level(const DirArgs &D, const uint32_t *cls, int L, uint32_t bid,
+                                         uint32_t bpd, uint32_t *lds)
 {
-    __shared__ uint32_t s_off[LB_TILE + 1];
-    __shared__ uint32_t s_beg[LB_TILE];
-    __shared__ uint32_t s_cls[LB_TILE];
-    __shared__ uint32_t s_wsum[4];
-    int dir;
-    uint32_t bid;
-    split_dirs(a.ndirs, bpd, dir, bid);
-    const DirArgs &D = a.d[dir];
-    const int L = a.level;
     const uint32_t n = D.cnt[CT(L % 3)];
     const unsigned long long packed = *D.pbigc;
     const uint32_t items = (uint32_t)(packed >> 32), chunks = (uint32_t)packed;
@@ -131,7 +129,13 @@ __global__ __launch_bounds__(256) void expand_lb_kernel(LevelArgs a, uint32_t bp
         *D.bigc_zero = 0ull;
     }
     if (n == 0 && items == 0) return;
-    DECLARE_LDS_QUEUE(q);
+    const LdsQueue q{lds, lds + QBUF, lds + QBUF + 1};
+    uint32_t *s_off = lds + QBUF + 2;
+    uint32_t *s_beg = s_off + LB_TILE + 1;
+    uint32_t *s_cls = s_beg + LB_TILE;
+    uint32_t *s_wsum = s_cls + LB_TILE;
+    if (threadIdx.x == 0) *q.n = 0;
+    __syncthreads();
     const uint32_t *__restrict__ R = D.R;
     const uint32_t *__restrict__ C = D.C;
     // hubs of the previous level: chunk g belongs to the item whose chunk
@@ -145,7 +149,7 @@ __global__ __launch_bounds__(256) void expand_lb_kernel(LevelArgs a, uint32_t bp
             else hi = mid - 1;
         }
         const uint32_t v = D.pbig[lo];
-        const uint32_t c = SINGLE ? 0u : a.cls[v];
+        const uint32_t c = SINGLE ? 0u : cls[v];
         const uint32_t b = R[v] + (gch - D.pbigoff[lo]) * LB_CHUNK;
         const uint32_t e = min(R[v + 1], b + LB_CHUNK);
         for (uint32_t j0 = b; j0 < e; j0 += 256) {
@@ -154,7 +158,7 @@ __global__ __launch_bounds__(256) void expand_lb_kernel(LevelArgs a, uint32_t bp
             uint32_t w = 0;
             if (j < e) {
                 w = C[j];
-                claimed = try_claim<SINGLE>(D.vis, a.cls, w, c);
+                claimed = try_claim<SINGLE>(D.vis, cls, w, c);
             }
             lq_push(q, claimed, w);
             lq_flush(q, cnt_out, D.qout, false);
@@ -167,7 +171,7 @@ __global__ __launch_bounds__(256) void expand_lb_kernel(LevelArgs a, uint32_t bp
             v = D.qin[i];
             beg = R[v];
             deg = R[v + 1] - beg;
-            if (!SINGLE) c = a.cls[v];
+            if (!SINGLE) c = cls[v];
             if (deg >= LB_BIG) {
                 const unsigned long long nch = (deg + LB_CHUNK - 1) / LB_CHUNK;
                 const unsigned long long old = atomicAdd(D.bigc, (1ull << 32) | nch);
@@ -197,7 +201,7 @@ __global__ __launch_bounds__(256) void expand_lb_kernel(LevelArgs a, uint32_t bp
                     else hi = mid - 1;
                 }
                 w = C[s_beg[lo] + (j - s_off[lo])];
-                claimed = try_claim<SINGLE>(D.vis, a.cls, w, SINGLE ? 0u : s_cls[lo]);
+                claimed = try_claim<SINGLE>(D.vis, cls, w, SINGLE ? 0u : s_cls[lo]);
             }
             lq_push(q, claimed, w);
             lq_flush(q, cnt_out, D.qout, false);
@@ -205,6 +209,16 @@ __global__ __launch_bounds__(256) void expand_lb_kernel(LevelArgs a, uint32_t bp
         __syncthreads();   // LDS reuse by the next tile
     }
     lq_flush(q, cnt_out, D.qout, true);
+}
+
+template <bool SINGLE>
+__global__ __launch_bounds__(256) void expand_lb_kernel(LevelArgs a, uint32_t bpd)
+{
+    __shared__ uint32_t s_lds[LB_LDS];
+    int dir;
+    uint32_t bid;
+    split_dirs(a.ndirs, bpd, dir, bid);
+    lb_level<SINGLE>(a.d[dir], a.cls, a.level, bid, bpd, s_lds);
 }
 
 static void launch_lb(hipStream_t s, const LevelArgs &a, bool single, uint32_t bpd)
@@ -252,29 +266,24 @@ __device__ __forceinline__ bool visited_parent(const uint32_t *vis, const uint32
     return ((uint32_t)w & ~(uint32_t)(w >> 32) & pb) && (SINGLE || cls[p] == c);
 }
 
-// Both directions of a level share one launch (split_dirs).
+// LDS words of a bottom-up level: each wave's queue and long list.
+constexpr uint32_t BU_LDS = 4 * BU_WQ + 4 * BU_CHUNKS * 64;
+
+// One bottom-up level of direction D (parents in PR/PC) for workgroup `bid`
+// of `bpd`; `lds` holds BU_LDS words.  Called by the whole workgroup.
 template <bool SINGLE>
-__global__ __launch_bounds__(256) void expand_bu_kernel(BottomUpArgs a, uint32_t bpd)
+__device__ __forceinline__ void bu_level(const DirArgs &D, const uint32_t *__restrict__ PR,
+                                         const uint32_t *__restrict__ PC, const uint32_t *cls, uint32_t V,
+                                         int L, uint32_t bid, uint32_t bpd, uint32_t *lds)
 {
-    __shared__ uint32_t s_q[4][BU_WQ];
-    __shared__ uint32_t s_long[4][BU_CHUNKS * 64];
-    int dir;
-    uint32_t bid;
-    split_dirs(a.ndirs, bpd, dir, bid);
-    const DirArgs &D = a.d[dir];
-    const uint32_t *__restrict__ PR = a.PR[dir];
-    const uint32_t *__restrict__ PC = a.PC[dir];
-    const uint32_t *cls = a.cls;
-    const uint32_t V = a.V;
-    const int L = a.level;
     uint32_t *cnt_out = &D.cnt[CT((L + 1) % 3)];
     if (bid == 0 && threadIdx.x == 0) {
         D.cnt[CT((L + 2) % 3)] = 0;
         *D.bigc_zero = 0ull;
     }
     const uint32_t wv = threadIdx.x >> 6, lane = wlane();
-    uint32_t *wq = s_q[wv];
-    uint32_t *wl = s_long[wv];
+    uint32_t *wq = lds + wv * BU_WQ;
+    uint32_t *wl = lds + 4 * BU_WQ + wv * BU_CHUNKS * 64;
     uint32_t nq = 0;
     const uint32_t sub = lane % BU_GROUP, grp = lane / BU_GROUP;
     const uint32_t gshift = lane & ~(BU_GROUP - 1);
@@ -342,15 +351,61 @@ __global__ __launch_bounds__(256) void expand_bu_kernel(BottomUpArgs a, uint32_t
     wave_flush_block(wq, nq, cnt_out, D.qout);
 }
 
-void k_bottom_up(hipStream_t s, const BottomUpArgs &a, bool single_class)
+// ---------------------------------------------------------------------
+// Direction-optimised level decided on the device.  Every workgroup of the
+// level-L launch recomputes each direction's mode from the mode and visited
+// count the level L-1 launch stored (slot (L-1)&1) and the counters level
+// L-1 left (vertices queued, hubs deferred); workgroup 0 of each direction
+// stores the result in slot L&1 for level L+1.  So the host can queue any
+// number of levels without reading counters in between; a direction that
+// has finished turns its workgroups into no-ops.
+template <bool SINGLE>
+__global__ __launch_bounds__(256) void expand_do_kernel(DoArgs a, uint32_t bpd)
 {
-    uint64_t b = ((uint64_t)a.V + 256) / 256;
-    const uint32_t bpd = dir_blocks((uint32_t)(b > 2048 ? 2048 : b), a.ndirs);
+    __shared__ uint32_t s_lds[BU_LDS > LB_LDS ? BU_LDS : LB_LDS];
+    int dir;
+    uint32_t bid;
+    split_dirs(a.ndirs, bpd, dir, bid);
+    const DirArgs &D = a.d[dir];
+    const int L = a.level;
+    DirState st = load_dir_state(a.ctr, L == 0 ? 0 : (L - 1) & 1, dir);
+    if (L > 0) {
+        st = dir_decide(st, D.cnt[CT(L % 3)], (uint32_t)(*D.pbigc >> 32), a.ctr[CT(C_NLIVE)], (uint32_t)L);
+        if (bid == 0 && threadIdx.x == 0) store_dir_state(a.ctr, L & 1, dir, st);
+    }
+    if (st.mode == DIR_TD)
+        lb_level<SINGLE>(D, a.cls, L, bid, bpd, s_lds);
+    else if (st.mode == DIR_BU)
+        bu_level<SINGLE>(D, a.PR[dir], a.PC[dir], a.cls, a.V, L, bid, bpd, s_lds);
+}
+
+__global__ void do_begin_kernel(uint32_t *ctr, int ndirs)
+{
+    const int d = (int)threadIdx.x;
+    if (d >= 2) return;
+    DirState st;
+    st.mode = d < ndirs ? DIR_TD : DIR_OFF;
+    st.end = 0;
+    st.visited = ctr[CT(C_NPIV)];
+    store_dir_state(ctr, 0, d, st);
+}
+
+uint32_t do_blocks(uint32_t V, int ndirs)
+{
+    const uint64_t b = ((uint64_t)V + 256) / 256;
+    return dir_blocks((uint32_t)(b > 2048 ? 2048 : b), ndirs);
+}
+
+void k_do_begin(hipStream_t s, uint32_t *ctr, int ndirs) { do_begin_kernel<<<1, 64, 0, s>>>(ctr, ndirs); }
+
+void k_do_level(hipStream_t s, const DoArgs &a, bool single_class, uint32_t bpd)
+{
+    bpd = dir_blocks(bpd, a.ndirs);
     const uint32_t grid = bpd * (uint32_t)a.ndirs;
     if (single_class)
-        expand_bu_kernel<true><<<grid, 256, 0, s>>>(a, bpd);
+        expand_do_kernel<true><<<grid, 256, 0, s>>>(a, bpd);
     else
-        expand_bu_kernel<false><<<grid, 256, 0, s>>>(a, bpd);
+        expand_do_kernel<false><<<grid, 256, 0, s>>>(a, bpd);
 }
 
 template <int W>

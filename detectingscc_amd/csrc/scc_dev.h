@@ -65,6 +65,7 @@ enum : int {
     C_ANOM = 31,     // invariant violations caught by kernel guards (must stay 0)
     C_NLIVE = 32,    // live vertices when the visited bitmaps were initialised
     C_TLIST = 33,    // 3 rotating counts of trim-1 removal lists (33..35)
+    C_DOST = 36,     // direction-optimised search state: 2 slots x 2 directions (36..39)
     C_COUNT = 48
 };
 
@@ -106,14 +107,72 @@ struct DirArgs {
     unsigned long long *bigc_zero;   // the list the next level appends to, cleared here
 };
 
-struct BottomUpArgs {
+// A direction-optimised level (top-down or bottom-up per direction, chosen
+// on the device).
+struct DoArgs {
     DirArgs d[2];
     const uint32_t *PR[2], *PC[2];   // parent lists of each direction (the opposite CSR)
     const uint32_t *cls;
+    uint32_t *ctr;                   // counters: search state (C_DOST), C_NLIVE, C_NPIV
     uint32_t V;
     int ndirs;
     int level;
 };
+
+// Direction-optimising switch points (Beamer et al.): go bottom-up when the
+// frontier holds more than 1/kToBottomUp of the vertices still unvisited,
+// back to top-down when it falls under 1/kToTopDown of the live vertices.
+constexpr int64_t kToBottomUp = 16;
+constexpr int64_t kToTopDown = 48;
+
+enum : uint32_t { DIR_OFF = 0, DIR_TD = 1, DIR_BU = 2 };
+
+// Search state of one direction: the mode its level runs in, the level at
+// which it finished, and the vertices it has visited.  Held in counter line
+// C_DOST + 2 * slot + dir (words 0, 1 and 2..3).
+struct DirState {
+    uint32_t mode, end;
+    uint64_t visited;
+};
+
+// The state of level L from that of level L-1, given the vertices level L-1
+// queued (nf), the hubs it deferred to level L (top-down only: a bottom-up
+// pass defers nothing) and the live vertices of the search.
+__host__ __device__ __forceinline__ DirState dir_decide(DirState s, uint32_t nf, uint32_t hubs, uint32_t nlive,
+                                                        uint32_t level)
+{
+    if (s.mode == DIR_OFF) return s;
+    if (s.mode == DIR_BU) hubs = 0;
+    if (!nf && !hubs) {
+        s.mode = DIR_OFF;
+        s.end = level;
+        return s;
+    }
+    s.visited += nf;
+    const int64_t remaining = (int64_t)nlive - (int64_t)s.visited;
+    if (s.mode == DIR_TD && (int64_t)nf * kToBottomUp > remaining) s.mode = DIR_BU;
+    else if (s.mode == DIR_BU && (int64_t)nf * kToTopDown < (int64_t)nlive) s.mode = DIR_TD;
+    return s;
+}
+
+__host__ __device__ __forceinline__ DirState load_dir_state(const uint32_t *ctr, int slot, int dir)
+{
+    const uint32_t *w = ctr + CT(C_DOST + 2 * slot + dir);
+    DirState s;
+    s.mode = w[0];
+    s.end = w[1];
+    s.visited = (uint64_t)w[2] | ((uint64_t)w[3] << 32);
+    return s;
+}
+
+__device__ __forceinline__ void store_dir_state(uint32_t *ctr, int slot, int dir, const DirState &s)
+{
+    uint32_t *w = ctr + CT(C_DOST + 2 * slot + dir);
+    w[0] = s.mode;
+    w[1] = s.end;
+    w[2] = (uint32_t)s.visited;
+    w[3] = (uint32_t)(s.visited >> 32);
+}
 
 struct LevelArgs {
     DirArgs d[2];
@@ -181,7 +240,11 @@ void k_expand_level(hipStream_t s, const LevelArgs &a, int expand, int warp, boo
                     uint32_t blocks_per_dir);
 // One bottom-up pass of each listed direction: every live unvisited vertex
 // scans its parents (PR/PC = the opposite CSR) for a visited one of its class.
-void k_bottom_up(hipStream_t s, const BottomUpArgs &a, bool single_class);
+// Direction-optimised search: k_do_begin sets slot 0 (the first `ndirs`
+// directions top-down, visited = the seeds), then one k_do_level per level.
+void k_do_begin(hipStream_t s, uint32_t *ctr, int ndirs);
+void k_do_level(hipStream_t s, const DoArgs &a, bool single_class, uint32_t blocks_per_dir);
+uint32_t do_blocks(uint32_t V, int ndirs);
 
 uint32_t sweep_blocks(uint32_t V);
 
