@@ -277,13 +277,15 @@ template <class T> scc_status Engine::scratch(DevBuf &b, size_t n, T **out)
 
 // Both words of every counter line into host memory the GPU writes
 // directly, then the sequence number (system-scope release).
+// Words of each counter line the host reads: 32- and 64-bit counters use
+// words 0..1, the search state (DirState) words 0..3.
+constexpr int kPubWords = 4;
+
 __global__ void publish_ctr_kernel(const uint32_t *ctr, uint32_t *pub, uint32_t seq)
 {
     const uint32_t i = threadIdx.x;
-    if (i < C_COUNT) {
-        pub[CT(i)] = ctr[CT(i)];
-        pub[CT(i) + 1] = ctr[CT(i) + 1];
-    }
+    if (i < C_COUNT)
+        for (int w = 0; w < kPubWords; w++) pub[CT(i) + w] = ctr[CT(i) + w];
     __threadfence_system();
     __syncthreads();
     if (i == 0) __hip_atomic_store(pub + CT(C_COUNT), seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
@@ -319,10 +321,8 @@ scc_status Engine::read_ctr()
         }
         __builtin_ia32_pause();
     }
-    for (int i = 0; i < C_COUNT; i++) {
-        h_ctr[CT(i)] = h_pub[CT(i)];
-        h_ctr[CT(i) + 1] = h_pub[CT(i) + 1];
-    }
+    for (int i = 0; i < C_COUNT; i++)
+        for (int w = 0; w < kPubWords; w++) h_ctr[CT(i) + w] = h_pub[CT(i) + w];
     return SCC_OK;
 }
 

@@ -77,7 +77,8 @@ __global__ __launch_bounds__(256) void expand_vw_kernel(LevelArgs a, uint32_t bp
 
 // ---------------------------------------------------------------------
 // Load-balanced expansion (SCC_EXPAND_LB).  A 256-lane workgroup takes 256
-// frontier vertices, scans their degrees in LDS and walks the concatenated
+// frontier vertices (or, when tiles are fewer than workgroups, one slice of
+// their arcs), scans their degrees in LDS and walks the concatenated
 // adjacency slots with all lanes, so consecutive lanes read consecutive
 // column entries whatever the degree mix.  A vertex of degree >= LB_BIG
 // (a hub) is instead appended to this level's hub list; the next level's
@@ -87,6 +88,7 @@ __global__ __launch_bounds__(256) void expand_vw_kernel(LevelArgs a, uint32_t bp
 // and makes a level a single launch.
 constexpr uint32_t LB_TILE = 256;
 constexpr uint32_t LB_CHUNK = 2048;
+constexpr uint32_t LB_SLICES = 16;   // most workgroups sharing one tile
 
 // Exclusive scan of x over the 256-lane workgroup; returns the prefix and
 // writes the total to *total.  `wsum` is LDS scratch of 4 words.
@@ -164,7 +166,15 @@ __device__ __forceinline__ void lb_level(const DirArgs &D, const uint32_t *cls, 
             lq_flush(q, cnt_out, D.qout, false);
         }
     }
-    for (uint32_t base = bid * LB_TILE; base < n; base += bpd * LB_TILE) {
+    // With fewer tiles than workgroups, each tile's arcs are cut into
+    // `slices` equal ranges taken by different workgroups (a few hundred
+    // frontier vertices of degree up to LB_BIG would otherwise keep a few
+    // workgroups busy for the whole level while the rest idle).  Every
+    // slice scans the tile's degrees; slice 0 alone lists its hubs.
+    const uint32_t tiles = (n + LB_TILE - 1) / LB_TILE;
+    const uint32_t slices = (tiles == 0 || tiles >= bpd) ? 1u : min(LB_SLICES, bpd / tiles);
+    for (uint32_t unit = bid; unit < tiles * slices; unit += bpd) {
+        const uint32_t base = (unit / slices) * LB_TILE, slice = unit % slices;
         const uint32_t i = base + threadIdx.x;
         uint32_t v = 0, beg = 0, deg = 0, c = 0;
         if (i < n) {
@@ -172,14 +182,14 @@ __device__ __forceinline__ void lb_level(const DirArgs &D, const uint32_t *cls, 
             beg = R[v];
             deg = R[v + 1] - beg;
             if (!SINGLE) c = cls[v];
-            if (deg >= LB_BIG) {
+            if (deg >= LB_BIG && slice == 0) {
                 const unsigned long long nch = (deg + LB_CHUNK - 1) / LB_CHUNK;
                 const unsigned long long old = atomicAdd(D.bigc, (1ull << 32) | nch);
                 const uint32_t item = (uint32_t)(old >> 32);
                 D.big[item] = v;
                 D.bigoff[item] = (uint32_t)old;
-                deg = 0;
             }
+            if (deg >= LB_BIG) deg = 0;
         }
         uint32_t total;
         const uint32_t off = block_exscan256(deg, s_wsum, &total);
@@ -188,11 +198,13 @@ __device__ __forceinline__ void lb_level(const DirArgs &D, const uint32_t *cls, 
         s_cls[threadIdx.x] = c;
         if (threadIdx.x == 0) s_off[LB_TILE] = total;
         __syncthreads();
-        for (uint32_t j0 = 0; j0 < total; j0 += LB_TILE) {
+        const uint32_t jlo = (uint32_t)((uint64_t)total * slice / slices);
+        const uint32_t jhi = (uint32_t)((uint64_t)total * (slice + 1) / slices);
+        for (uint32_t j0 = jlo; j0 < jhi; j0 += LB_TILE) {
             const uint32_t j = j0 + threadIdx.x;
             bool claimed = false;
             uint32_t w = 0;
-            if (j < total) {
+            if (j < jhi) {
                 // owner = last k with s_off[k] <= j (degree-0 entries share offsets)
                 uint32_t lo = 0, hi = LB_TILE - 1;
                 while (lo < hi) {
