@@ -262,8 +262,13 @@ static void launch_lb(hipStream_t s, const LevelArgs &a, bool single, uint32_t b
 // bits -- are in flight), with a private LDS queue and long list, and never
 // waits on its workgroup.  A wave whose queue fills reserves global space
 // itself; what remains at the end goes out with one atomic per workgroup
-// (same-address atomics serialise).  Only u's lane sets u's bit in this
-// pass, so the claim is an atomicOr whose old value is not waited for.
+// (same-address atomics serialise).
+//
+// Claims need no atomics: chunks start at multiples of 64, so the two
+// visited words of a chunk belong to one wave for the whole pass, and the
+// wave ORs its hits into each with one plain store (device-wide atomics
+// run at about 10 G/s on MI355X, which would bound a pass that visits
+// millions of vertices).
 constexpr uint32_t BU_SHORT = 4;
 constexpr uint32_t BU_GROUP = 8;
 constexpr uint32_t BU_WQ = 1024;
@@ -278,8 +283,9 @@ __device__ __forceinline__ bool visited_parent(const uint32_t *vis, const uint32
     return ((uint32_t)w & ~(uint32_t)(w >> 32) & pb) && (SINGLE || cls[p] == c);
 }
 
-// LDS words of a bottom-up level: each wave's queue and long list.
-constexpr uint32_t BU_LDS = 4 * BU_WQ + 4 * BU_CHUNKS * 64;
+// LDS words of a bottom-up level: each wave's queue, long list and the
+// long-list hits of its chunks (two words per chunk).
+constexpr uint32_t BU_LDS = 4 * BU_WQ + 4 * BU_CHUNKS * 64 + 4 * BU_CHUNKS * 2;
 
 // One bottom-up level of direction D (parents in PR/PC) for workgroup `bid`
 // of `bpd`; `lds` holds BU_LDS words.  Called by the whole workgroup.
@@ -296,18 +302,23 @@ __device__ __forceinline__ void bu_level(const DirArgs &D, const uint32_t *__res
     const uint32_t wv = threadIdx.x >> 6, lane = wlane();
     uint32_t *wq = lds + wv * BU_WQ;
     uint32_t *wl = lds + 4 * BU_WQ + wv * BU_CHUNKS * 64;
+    uint32_t *wh = lds + 4 * BU_WQ + 4 * BU_CHUNKS * 64 + wv * BU_CHUNKS * 2;
     uint32_t nq = 0;
     const uint32_t sub = lane % BU_GROUP, grp = lane / BU_GROUP;
     const uint32_t gshift = lane & ~(BU_GROUP - 1);
     const uint32_t stride = bpd * 256;   // one chunk of every wave of the grid
-    for (uint32_t base = (bid * 4 + wv) * 64; base < V; base += BU_CHUNKS * stride) {
+    for (uint32_t base = (bid * 4 + wv) * 64; base <= V; base += BU_CHUNKS * stride) {
         if (nq + 2 * BU_CHUNKS * 64 > BU_WQ) wave_flush(wq, nq, cnt_out, D.qout);
-        uint32_t u[BU_CHUNKS], c[BU_CHUNKS], b[BU_CHUNKS], e[BU_CHUNKS];
+        uint32_t u[BU_CHUNKS], word[BU_CHUNKS], c[BU_CHUNKS], b[BU_CHUNKS], e[BU_CHUNKS];
         bool cand[BU_CHUNKS], found[BU_CHUNKS], need_long[BU_CHUNKS];
+        uint64_t hits[BU_CHUNKS];
 #pragma unroll
         for (uint32_t h = 0; h < BU_CHUNKS; h++) {
-            u[h] = base + h * stride + lane + 1;
-            cand[h] = u[h] <= V && !(D.vis[vword(u[h])] & (1u << (u[h] & 31)));
+            // lane l of chunk h holds vertex base + h * stride + l (vertex 0,
+            // the dummy, is finished)
+            u[h] = base + h * stride + lane;
+            word[h] = u[h] <= V ? D.vis[vword(u[h])] : ~0u;
+            cand[h] = u[h] != 0 && !(word[h] & (1u << (u[h] & 31)));
         }
 #pragma unroll
         for (uint32_t h = 0; h < BU_CHUNKS; h++) {
@@ -327,7 +338,7 @@ __device__ __forceinline__ void bu_level(const DirArgs &D, const uint32_t *__res
             for (uint32_t t = 0; t < BU_SHORT; t++)
                 found[h] |= b[h] + t < e[h] && visited_parent<SINGLE>(D.vis, cls, p[h][t], c[h]);
             need_long[h] = !found[h] && e[h] - b[h] > BU_SHORT;
-            if (found[h]) atomicOr(&D.vis[vword(u[h])], 1u << (u[h] & 31));
+            hits[h] = __ballot(found[h]);
         }
         uint32_t nlong = 0;
 #pragma unroll
@@ -337,6 +348,7 @@ __device__ __forceinline__ void bu_level(const DirArgs &D, const uint32_t *__res
             if (need_long[h]) wl[nlong + __popcll(lm & ((1ull << lane) - 1))] = u[h];
             nlong += (uint32_t)__popcll(lm);
         }
+        if (lane < BU_CHUNKS * 2) wh[lane] = 0;
         wave_sync();
         // long parent lists: 64 / BU_GROUP vertices per round, each group
         // stops at its first hit
@@ -355,10 +367,19 @@ __device__ __forceinline__ void bu_level(const DirArgs &D, const uint32_t *__res
                 }
             }
             const bool claimed = hit && sub == 0;
-            if (claimed) atomicOr(&D.vis[vword(x)], 1u << (x & 31));
+            if (claimed) {
+                const uint32_t hx = (x - base) / stride, off = x - base - hx * stride;
+                atomicOr(&wh[hx * 2 + (off >> 5)], 1u << (off & 31));   // LDS
+            }
             wave_push(wq, nq, claimed, x);
         }
-        wave_sync();   // the next iteration rewrites wl
+        wave_sync();   // long-list hits are in wh; the next iteration rewrites wl
+#pragma unroll
+        for (uint32_t h = 0; h < BU_CHUNKS; h++) {
+            // lanes 0 and 32 own the chunk's two visited words
+            const uint32_t m = (lane == 0 ? (uint32_t)hits[h] : (uint32_t)(hits[h] >> 32)) | wh[h * 2 + (lane >> 5)];
+            if ((lane & 31) == 0 && m) D.vis[vword(u[h])] = word[h] | m;
+        }
     }
     wave_flush_block(wq, nq, cnt_out, D.qout);
 }
