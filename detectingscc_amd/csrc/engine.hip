@@ -278,8 +278,8 @@ template <class T> scc_status Engine::scratch(DevBuf &b, size_t n, T **out)
 // Both words of every counter line into host memory the GPU writes
 // directly, then the sequence number (system-scope release).
 // Words of each counter line the host reads: 32- and 64-bit counters use
-// words 0..1, the search state (DirState) words 0..3.
-constexpr int kPubWords = 4;
+// words 0..1, the search state (DirState) words 0..kDirStateWords-1.
+constexpr int kPubWords = kDirStateWords;
 
 __global__ void publish_ctr_kernel(const uint32_t *ctr, uint32_t *pub, uint32_t seq)
 {

@@ -419,6 +419,8 @@ __global__ void do_begin_kernel(uint32_t *ctr, int ndirs)
     DirState st;
     st.mode = d < ndirs ? DIR_TD : DIR_OFF;
     st.end = 0;
+    st.nf = ctr[CT(C_NPIV)];
+    st.pad = 0;
     st.visited = ctr[CT(C_NPIV)];
     store_dir_state(ctr, 0, d, st);
 }

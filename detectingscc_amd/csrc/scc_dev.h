@@ -121,19 +121,25 @@ struct DoArgs {
 
 // Direction-optimising switch points (Beamer et al.): go bottom-up when the
 // frontier holds more than 1/kToBottomUp of the vertices still unvisited,
-// back to top-down when it falls under 1/kToTopDown of the live vertices.
+// or more than 1/kToBottomUpBurst of them right after growing kBurst-fold
+// in one level (a frontier that bursts -- the neighbours of a hub -- has a
+// high average degree, which the vertex counts alone do not show), and back
+// to top-down when it falls under 1/kToTopDown of the live vertices.
 constexpr int64_t kToBottomUp = 16;
+constexpr int64_t kToBottomUpBurst = 128;
+constexpr int64_t kBurst = 4;
 constexpr int64_t kToTopDown = 48;
 
 enum : uint32_t { DIR_OFF = 0, DIR_TD = 1, DIR_BU = 2 };
 
 // Search state of one direction: the mode its level runs in, the level at
-// which it finished, and the vertices it has visited.  Held in counter line
-// C_DOST + 2 * slot + dir (words 0, 1 and 2..3).
+// which it finished, the frontier it started from and the vertices it has
+// visited.  Held in counter line C_DOST + 2 * slot + dir (words 0..5).
 struct DirState {
-    uint32_t mode, end;
+    uint32_t mode, end, nf, pad;
     uint64_t visited;
 };
+constexpr int kDirStateWords = 6;
 
 // The state of level L from that of level L-1, given the vertices level L-1
 // queued (nf), the hubs it deferred to level L (top-down only: a bottom-up
@@ -148,9 +154,11 @@ __host__ __device__ __forceinline__ DirState dir_decide(DirState s, uint32_t nf,
         s.end = level;
         return s;
     }
+    const int64_t grew = (int64_t)nf > kBurst * (int64_t)s.nf;
+    s.nf = nf;
     s.visited += nf;
     const int64_t remaining = (int64_t)nlive - (int64_t)s.visited;
-    if (s.mode == DIR_TD && (int64_t)nf * kToBottomUp > remaining) s.mode = DIR_BU;
+    if (s.mode == DIR_TD && (int64_t)nf * (grew ? kToBottomUpBurst : kToBottomUp) > remaining) s.mode = DIR_BU;
     else if (s.mode == DIR_BU && (int64_t)nf * kToTopDown < (int64_t)nlive) s.mode = DIR_TD;
     return s;
 }
@@ -161,7 +169,9 @@ __host__ __device__ __forceinline__ DirState load_dir_state(const uint32_t *ctr,
     DirState s;
     s.mode = w[0];
     s.end = w[1];
-    s.visited = (uint64_t)w[2] | ((uint64_t)w[3] << 32);
+    s.nf = w[2];
+    s.pad = 0;
+    s.visited = (uint64_t)w[4] | ((uint64_t)w[5] << 32);
     return s;
 }
 
@@ -170,8 +180,10 @@ __device__ __forceinline__ void store_dir_state(uint32_t *ctr, int slot, int dir
     uint32_t *w = ctr + CT(C_DOST + 2 * slot + dir);
     w[0] = s.mode;
     w[1] = s.end;
-    w[2] = (uint32_t)s.visited;
-    w[3] = (uint32_t)(s.visited >> 32);
+    w[2] = s.nf;
+    w[3] = 0;
+    w[4] = (uint32_t)s.visited;
+    w[5] = (uint32_t)(s.visited >> 32);
 }
 
 struct LevelArgs {
