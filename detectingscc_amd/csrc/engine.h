@@ -51,7 +51,8 @@ class Engine {
     scc_status reach(bool fw, bool bw, bool single, int expand, int W, uint32_t *levels_out);
     scc_status reach_do(bool fw, bool bw, bool single, uint32_t *levels_out);
     DirArgs dir_args(int d, uint32_t level) const;
-    scc_status trim1(bool fixpoint);
+    // single: every live vertex has class 0 (the first trim of a decomposition)
+    scc_status trim1(bool fixpoint, bool single = false);
     void init_vis();
     uint64_t limit() const;
     scc_status no_convergence(const char *what);

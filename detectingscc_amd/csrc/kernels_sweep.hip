@@ -120,46 +120,34 @@ void k_init_state(hipStream_t s, const DevGraph &g, DevState &st, const uint8_t 
 // once; in a sweep each vertex has one lane.  Pass k reads list count
 // C_TLIST + k%3, appends to C_TLIST + (k+1)%3 and clears C_TLIST + (k+2)%3.
 
+// Is w live and in class c?  With BITS every live vertex is in one class
+// (the first trim of a decomposition) and the answer is one bit of the
+// live bitmap `live` (1 bit per vertex, 2 MB at 16M vertices: it stays in
+// L2, where the class words -- 4 bytes per vertex -- do not); otherwise it
+// is a class compare.
+template <bool BITS>
+__device__ __forceinline__ bool same_live(const uint32_t *cls, const uint32_t *live, uint32_t w, uint32_t c)
+{
+    if (BITS) return (live[w >> 5] >> (w & 31)) & 1u;
+    return cls[w] == c;
+}
+
 // v (live, class c) keeps a live same-class in-neighbour and out-neighbour.
-// The first TRIM_SHORT arcs each way are probed with independent loads (row
-// offsets, then column entries, then class words: three dependent steps
-// instead of seven), and only a vertex with no hit among them walks the rest
-// of a list.  The sweep is bound by the latency of these chains.
-constexpr uint32_t TRIM_SHORT = 1;
-
+// The first arc each way is probed with independent loads (row offsets,
+// then column entries, then class words: three dependent steps instead of
+// seven), and only a vertex with no hit there walks the rest of a list.
+// Probing more arcs up front adds random lines and is slower.
+template <bool BITS>
 __device__ __forceinline__ bool trim1_keep_at(const uint32_t *__restrict__ Fc, const uint32_t *__restrict__ Bc,
-                                              const uint32_t *cls, uint32_t c, uint32_t fb, uint32_t fe,
-                                              uint32_t bb, uint32_t be)
+                                              const uint32_t *cls, const uint32_t *live, uint32_t c, uint32_t fb,
+                                              uint32_t fe, uint32_t bb, uint32_t be)
 {
-    uint32_t pin[TRIM_SHORT], pout[TRIM_SHORT];
-#pragma unroll
-    for (uint32_t t = 0; t < TRIM_SHORT; t++) {
-        pin[t] = bb + t < be ? Bc[bb + t] : 0u;
-        pout[t] = fb + t < fe ? Fc[fb + t] : 0u;
-    }
-    bool kin = false, kout = false;
-#pragma unroll
-    for (uint32_t t = 0; t < TRIM_SHORT; t++) {
-        kin |= bb + t < be && cls[pin[t]] == c;
-        kout |= fb + t < fe && cls[pout[t]] == c;
-    }
-    for (uint32_t i = bb + TRIM_SHORT; !kin && i < be; i++) kin = cls[Bc[i]] == c;
-    for (uint32_t i = fb + TRIM_SHORT; kin && !kout && i < fe; i++) kout = cls[Fc[i]] == c;
+    const uint32_t pin = bb < be ? Bc[bb] : 0u, pout = fb < fe ? Fc[fb] : 0u;
+    bool kin = bb < be && same_live<BITS>(cls, live, pin, c);
+    bool kout = fb < fe && same_live<BITS>(cls, live, pout, c);
+    for (uint32_t i = bb + 1; !kin && i < be; i++) kin = same_live<BITS>(cls, live, Bc[i], c);
+    for (uint32_t i = fb + 1; kin && !kout && i < fe; i++) kout = same_live<BITS>(cls, live, Fc[i], c);
     return kin && kout;
-}
-
-__device__ __forceinline__ bool trim1_keep(const uint32_t *__restrict__ Fr, const uint32_t *__restrict__ Fc,
-                                           const uint32_t *__restrict__ Br, const uint32_t *__restrict__ Bc,
-                                           const uint32_t *cls, uint32_t v, uint32_t c)
-{
-    return trim1_keep_at(Fc, Bc, cls, c, Fr[v], Fr[v + 1], Br[v], Br[v + 1]);
-}
-
-__device__ __forceinline__ bool trim1_remove(uint32_t *cls, uint32_t *lab, uint32_t v, uint32_t c)
-{
-    if (atomicCAS(&cls[v], c, DEAD) != c) return false;
-    lab[v] = v | LAB_FINAL;
-    return true;
 }
 
 // One pass of the fixpoint.  Pass 0, and any pass whose input list holds
@@ -171,15 +159,22 @@ __device__ __forceinline__ bool trim1_remove(uint32_t *cls, uint32_t *lab, uint3
 // host can queue several passes: once a pass removes nothing the following
 // ones find an empty list and only clear counters.  Removals go to
 // wave-private LDS queues (a sweep can remove millions of vertices).
+//
+// A sweep gives each wave 64-aligned chunks, so the wave owns its chunk's
+// class and label words and (BITS) its two live-bitmap words: removal is
+// plain stores.  A list pass may test one vertex from several lanes: the
+// first to stamp it with the pass's epoch tests it, removal is a CAS on its
+// class word and (BITS) an atomic clear of its live bit.
 constexpr uint32_t TRIM_WQ = 1024;
 constexpr uint32_t TRIM_W = 8;
 constexpr uint32_t TRIM_PER_BLOCK = 256 / TRIM_W;
 
+template <bool BITS>
 __global__ __launch_bounds__(256) void trim1_kernel(const uint32_t *__restrict__ Fr,
                                                     const uint32_t *__restrict__ Fc,
                                                     const uint32_t *__restrict__ Br,
                                                     const uint32_t *__restrict__ Bc, uint32_t *cls,
-                                                    uint32_t *lab, uint32_t V, uint32_t *ctr,
+                                                    uint32_t *lab, uint32_t *live, uint32_t V, uint32_t *ctr,
                                                     const uint32_t *in, uint32_t *out, uint32_t *stamp,
                                                     uint32_t epoch, uint32_t pass, uint32_t sweep_share)
 {
@@ -192,20 +187,31 @@ __global__ __launch_bounds__(256) void trim1_kernel(const uint32_t *__restrict__
     uint32_t *cnt_out = &ctr[CT(C_TLIST + (pass + 1) % 3)];
     if (blockIdx.x == 0 && threadIdx.x == 0) ctr[CT(C_TLIST + (pass + 2) % 3)] = 0;
     if (pass == 0 || n > V / sweep_share) {
-        for (uint32_t base = (blockIdx.x * 4 + wv) * 64; base < V; base += gridDim.x * 256) {
+        for (uint32_t base = (blockIdx.x * 4 + wv) * 64; base <= V; base += gridDim.x * 256) {
             if (nq + 64 > TRIM_WQ) wave_flush(wq, nq, cnt_out, out);
-            const uint32_t v = base + lane() + 1;
-            bool gone = false;
+            const uint32_t v = base + lane();   // vertex 0, the dummy, is dead
+            uint32_t word = 0, c = DEAD;
             if (v <= V) {
-                // the row offsets load beside the class word; only this
-                // lane writes v's words in a sweep, so removal is two stores
-                const uint32_t c = cls[v];
+                if (BITS) {
+                    word = live[v >> 5];
+                    c = (word >> (v & 31)) & 1u ? 0u : DEAD;
+                } else {
+                    c = cls[v];
+                }
+            }
+            bool gone = false;
+            if (c != DEAD) {
                 const uint32_t fb = Fr[v], fe = Fr[v + 1], bb = Br[v], be = Br[v + 1];
-                if (c != DEAD && !trim1_keep_at(Fc, Bc, cls, c, fb, fe, bb, be)) {
+                if (!trim1_keep_at<BITS>(Fc, Bc, cls, live, c, fb, fe, bb, be)) {
                     cls[v] = DEAD;
                     lab[v] = v | LAB_FINAL;
                     gone = true;
                 }
+            }
+            if (BITS) {
+                const uint64_t m = __ballot(gone);
+                const uint32_t mw = lane() < 32 ? (uint32_t)m : (uint32_t)(m >> 32);
+                if ((lane() & 31) == 0 && mw) live[v >> 5] = word & ~mw;
             }
             removed += gone;
             wave_push(wq, nq, gone, v);
@@ -234,10 +240,14 @@ __global__ __launch_bounds__(256) void trim1_kernel(const uint32_t *__restrict__
                 uint32_t w = 0;
                 if (j < total) {
                     w = j < nf ? Fc[fb + j] : Bc[bb + (j - nf)];
-                    const uint32_t c = cls[w];
+                    const uint32_t c = BITS ? ((live[w >> 5] >> (w & 31)) & 1u ? 0u : DEAD) : cls[w];
                     if (c != DEAD && stamp[w] != epoch && atomicExch(&stamp[w], epoch) != epoch &&
-                        !trim1_keep(Fr, Fc, Br, Bc, cls, w, c))
-                        gone = trim1_remove(cls, lab, w, c);
+                        !trim1_keep_at<BITS>(Fc, Bc, cls, live, c, Fr[w], Fr[w + 1], Br[w], Br[w + 1]) &&
+                        atomicCAS(&cls[w], c, DEAD) == c) {
+                        lab[w] = w | LAB_FINAL;
+                        if (BITS) atomicAnd(&live[w >> 5], ~(1u << (w & 31)));
+                        gone = true;
+                    }
                 }
                 removed += gone;
                 wave_push(wq, nq, gone, w);
@@ -249,12 +259,32 @@ __global__ __launch_bounds__(256) void trim1_kernel(const uint32_t *__restrict__
     wave_add32(&ctr[CT(C_TRIM1)], removed);
 }
 
-void k_trim1(hipStream_t s, const DevGraph &g, DevState &st, const uint32_t *in, uint32_t *out,
-             uint32_t epoch, uint32_t pass, uint32_t sweep_share)
+// The live bitmap of the single-class trim: bit v set for every vertex
+// whose class word is not DEAD (vertex 0 included, as dead).
+__global__ void live_bits_kernel(const uint32_t *__restrict__ cls, uint32_t *live, uint32_t V)
 {
-    trim1_kernel<<<sweep_blocks(g.V), SWEEP_BLOCK, 0, s>>>(g.Fr, g.Fc, g.Br, g.Bc, st.cls, st.lab, g.V,
-                                                           st.ctr, in, out, st.stamp, epoch, pass,
-                                                           sweep_share);
+    const uint32_t nwaves = (gridDim.x * blockDim.x) >> 6;
+    for (uint32_t base = ((blockIdx.x * blockDim.x + threadIdx.x) >> 6) * 64; base <= V; base += nwaves * 64) {
+        const uint32_t v = base + lane();
+        const uint64_t m = __ballot(v <= V && cls[v] != DEAD);
+        if ((lane() & 31) == 0) live[v >> 5] = lane() < 32 ? (uint32_t)m : (uint32_t)(m >> 32);
+    }
+}
+
+void k_trim1(hipStream_t s, const DevGraph &g, DevState &st, const uint32_t *in, uint32_t *out,
+             uint32_t epoch, uint32_t pass, uint32_t sweep_share, uint32_t *live)
+{
+    if (live) {
+        if (pass == 0)
+            live_bits_kernel<<<sweep_blocks(g.V), SWEEP_BLOCK, 0, s>>>(st.cls, live, g.V);
+        trim1_kernel<true><<<sweep_blocks(g.V), SWEEP_BLOCK, 0, s>>>(g.Fr, g.Fc, g.Br, g.Bc, st.cls, st.lab,
+                                                                     live, g.V, st.ctr, in, out, st.stamp,
+                                                                     epoch, pass, sweep_share);
+    } else {
+        trim1_kernel<false><<<sweep_blocks(g.V), SWEEP_BLOCK, 0, s>>>(g.Fr, g.Fc, g.Br, g.Bc, st.cls, st.lab,
+                                                                      nullptr, g.V, st.ctr, in, out,
+                                                                      st.stamp, epoch, pass, sweep_share);
+    }
 }
 
 // ---------------------------------------------------------- trim-2 ---

@@ -223,9 +223,11 @@ void k_init_state(hipStream_t s, const DevGraph &g, DevState &st, const uint8_t 
 // sweeps every vertex, the others test the live neighbours of the vertices
 // listed in `in`; removals are listed in `out`, counted at
 // C_TLIST + (pass+1)%3.  `epoch` is new for every pass of the context
-// (st.stamp never holds it).
+// (st.stamp never holds it).  With `live` (only while every live vertex has
+// class 0), neighbour tests read a live bitmap there instead of classes;
+// pass 0 builds it from the class words.
 void k_trim1(hipStream_t s, const DevGraph &g, DevState &st, const uint32_t *in, uint32_t *out,
-             uint32_t epoch, uint32_t pass, uint32_t sweep_share);
+             uint32_t epoch, uint32_t pass, uint32_t sweep_share, uint32_t *live);
 void k_trim2(hipStream_t s, const DevGraph &g, DevState &st);
 void k_init_vis(hipStream_t s, const DevGraph &g, DevState &st);
 // single_class: every live vertex is in class 0 (the first pivot)
