@@ -53,7 +53,7 @@ class Engine {
     DirArgs dir_args(int d, uint32_t level) const;
     // single: every live vertex has class 0 (the first trim of a decomposition)
     scc_status trim1(bool fixpoint, bool single = false);
-    void init_vis();
+    void init_vis(bool elect_first = false);
     uint64_t limit() const;
     scc_status no_convergence(const char *what);
     scc_status pivot_round(bool first, int expand, int W, scc_stats *stats, uint32_t *live);

@@ -51,7 +51,7 @@ void Engine::release()
     for (int i = 0; i < 3; i++) {
         f(st.bigf[i]); f(st.bigb[i]); f(st.bigoff_f[i]); f(st.bigoff_b[i]);
     }
-    f(st.pivlist); f(st.stamp); f(st.part); f(st.ctr); f(d_tmp8);
+    f(st.pivlist); f(st.stamp); f(st.part); f(st.ctr); f(st.bred); f(d_tmp8);
     for (auto &b : buf) {
         f(b.p);
         b = DevBuf();
@@ -129,7 +129,7 @@ scc_status Engine::reserve(uint32_t V, uint32_t E)
     for (int i = 0; i < 3; i++) {
         f(st.bigf[i]); f(st.bigb[i]); f(st.bigoff_f[i]); f(st.bigoff_b[i]);
     }
-    f(st.pivlist); f(st.stamp); f(st.part); f(d_tmp8);
+    f(st.pivlist); f(st.stamp); f(st.part); f(st.bred); f(d_tmp8);
     uint32_t *ctr = st.ctr;
     g = DevGraph();
     st = DevState();
@@ -156,6 +156,7 @@ scc_status Engine::reserve(uint32_t V, uint32_t E)
         A(dalloc(&st.bigoff_f[i], nhub)); A(dalloc(&st.bigoff_b[i], nhub));
     }
     A(dalloc(&st.pivlist, nv)); A(dalloc(&st.stamp, nv)); A(dalloc(&st.part, nv + 1)); A(dalloc(&d_tmp8, nv));
+    A(dalloc(&st.bred, 2 * (size_t)VIS_BLOCKS));
     if (e != hipSuccess) {
         (void)hipGetLastError();
         return fail(e == hipErrorOutOfMemory ? SCC_ERR_NOMEM : SCC_ERR_DEVICE,
@@ -375,11 +376,7 @@ uint32_t Engine::bpd(int W) const
     return (uint32_t)(b > 2048 ? 2048 : b);
 }
 
-void Engine::init_vis()
-{
-    zero_ctr(C_NLIVE, 1);
-    k_init_vis(stream, g, st);
-}
+void Engine::init_vis(bool elect_first) { k_init_vis(stream, g, st, elect_first); }
 
 // The level-L arguments of direction d (0 = FW, 1 = BW): queues alternate
 // with the level, counts and hub lists rotate through three slots.
@@ -575,10 +572,12 @@ scc_status Engine::no_convergence(const char *what)
 scc_status Engine::pivot_round(bool first, int expand, int W, scc_stats *stats, uint32_t *live)
 {
     mark(SCC_PHASE_PIVOT);
-    init_vis();
-    zero_ctr(C_NPIV, 1);
-    k_elect(stream, g, st, first);
-    k_select(stream, g, st, 1);
+    init_vis(first);
+    if (!first) {
+        zero_ctr(C_NPIV, 1);
+        k_elect(stream, g, st);
+        k_select(stream, g, st, 1);
+    }
     mark(SCC_PHASE_REACH);
     const int first_mark = (int)nmarks - 1;
     scc_status s = reach(true, true, first, expand, W, &stats->levels);
@@ -981,9 +980,7 @@ scc_status Engine::phase_first_pivot(const uint8_t *dead, uint32_t *pivot)
 {
     scc_status s;
     if ((s = phase_begin(dead)) != SCC_OK) return s;
-    init_vis();
-    k_elect(stream, g, st, true);
-    k_select(stream, g, st, 1);
+    init_vis(true);
     if ((s = read_ctr()) != SCC_OK) return s;
     *pivot = 0;
     if (h_ctr[CT(C_NPIV)] > 1) return fail(SCC_ERR_DEVICE, "first-pivot election chose %u vertices", h_ctr[CT(C_NPIV)]);

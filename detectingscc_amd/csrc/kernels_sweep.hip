@@ -352,36 +352,130 @@ void k_trim2(hipStream_t s, const DevGraph &g, DevState &st)
                                                            g.V, st.ctr);
 }
 
+__device__ __forceinline__ void zero_hub_counts(uint32_t *ctr)
+{
+    for (int i = 0; i < 3; i++) {
+        *reinterpret_cast<unsigned long long *>(ctr + CT(C_BIGF + i)) = 0ull;
+        *reinterpret_cast<unsigned long long *>(ctr + CT(C_BIGB + i)) = 0ull;
+    }
+}
+
 // ------------------------------------------------- visited bitmaps ---
 // Bits of finished vertices (and of slot 0 and the padding) start set in
 // the visited-or-finished words of both bitmaps; the finished words beside
 // them keep the finished set alone so a bottom-up pass can tell "visited"
 // from "finished".  Counts the live vertices.
-__global__ void init_vis_kernel(const uint32_t *__restrict__ cls, uint32_t *fvis, uint32_t *bvis,
-                                uint32_t *ctr, uint32_t V, uint32_t nchunks)
+//
+// With ELECT (a round where every live vertex is in one class) the same
+// sweep elects the first pivot: the largest pivot key, whose low word names
+// its vertex, so no second sweep is needed to find it.  Each workgroup
+// leaves its count and key in `bred`; vis_finish_kernel reduces them with
+// one workgroup (thousands of same-address atomics would serialise).
+template <bool ELECT>
+__global__ __launch_bounds__(256) void init_vis_kernel(const uint32_t *__restrict__ cls,
+                                                       const uint32_t *__restrict__ Fr,
+                                                       const uint32_t *__restrict__ Br, uint32_t *fvis,
+                                                       uint32_t *bvis, uint64_t *bred, uint32_t V,
+                                                       uint32_t nchunks)
 {
+    __shared__ uint64_t s_best[4];
     const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const uint32_t nwaves = (gridDim.x * blockDim.x) >> 6;
     uint32_t live = 0;
+    uint64_t best = 0;
     for (uint32_t ch = wave; ch < nchunks; ch += nwaves) {
         uint32_t v = ch * 64 + lane();
         bool dead = v > V || cls[v] == DEAD;
         live += !dead;
+        if (ELECT && !dead) {
+            const uint64_t k = pivot_key(Fr, Br, v);
+            best = k > best ? k : best;
+        }
         uint64_t m = __ballot(dead);
         // the chunk's two groups, each as (visited-or-finished, finished)
         const uint32_t w = (lane() & 2) ? (uint32_t)(m >> 32) : (uint32_t)m;
         uint32_t *dst = lane() < 4 ? fvis : bvis;
         if (lane() < 8) dst[4 * ch + (lane() & 3)] = w;
     }
-    wave_add32(&ctr[CT(C_NLIVE)], live);
+    if (ELECT) {
+        for (int o = 32; o > 0; o >>= 1) {
+            const uint64_t x = __shfl_xor(best, o);
+            best = x > best ? x : best;
+        }
+        if (lane() == 0) s_best[threadIdx.x >> 6] = best;
+    }
+    const uint64_t t = block_sum(live);   // synchronises the workgroup
+    if (threadIdx.x == 0) {
+        bred[2 * blockIdx.x] = t;
+        if (ELECT) {
+            for (uint32_t w = 1; w < 4; w++) best = s_best[w] > best ? s_best[w] : best;
+            bred[2 * blockIdx.x + 1] = best;
+        }
+    }
 }
 
-void k_init_vis(hipStream_t s, const DevGraph &g, DevState &st)
+// Sums the live counts into C_NLIVE; with ELECT also stores the winning key
+// in piv[0] and seeds its vertex (if any) in both directions, like
+// select_kernel + seed_counts_kernel.
+template <bool ELECT>
+__global__ __launch_bounds__(1024) void vis_finish_kernel(const uint64_t *__restrict__ bred, uint32_t nblocks,
+                                                          uint32_t *ctr, uint64_t *piv, uint32_t *fvis,
+                                                          uint32_t *bvis, uint32_t *qf, uint32_t *qb,
+                                                          uint32_t *pivlist)
+{
+    __shared__ uint64_t s_best[16];
+    uint64_t live = 0, best = 0;
+    for (uint32_t i = threadIdx.x; i < nblocks; i += blockDim.x) {
+        live += bred[2 * i];
+        if (ELECT) best = bred[2 * i + 1] > best ? bred[2 * i + 1] : best;
+    }
+    if (ELECT) {
+        for (int o = 32; o > 0; o >>= 1) {
+            const uint64_t x = __shfl_xor(best, o);
+            best = x > best ? x : best;
+        }
+        if (lane() == 0) s_best[threadIdx.x >> 6] = best;
+    }
+    const uint64_t t = block_sum(live);
+    if (threadIdx.x != 0) return;
+    ctr[CT(C_NLIVE)] = (uint32_t)t;
+    if (!ELECT) return;
+    for (uint32_t w = 1; w < (blockDim.x >> 6); w++) best = s_best[w] > best ? s_best[w] : best;
+    piv[0] = best;
+    const uint32_t n = best ? 1u : 0u;
+    const uint32_t v = key_vertex(best);
+    if (n) {
+        pivlist[0] = v;
+        qf[0] = v;
+        qb[0] = v;
+        fvis[vword(v)] |= 1u << (v & 31);
+        bvis[vword(v)] |= 1u << (v & 31);
+    }
+    ctr[CT(C_NPIV)] = n;
+    ctr[CT(C_PIV0)] = n ? v : 0u;
+    for (int i = 0; i < 3; i++) {
+        ctr[CT(C_FWCNT + i)] = i == 0 ? n : 0u;
+        ctr[CT(C_BWCNT + i)] = i == 0 ? n : 0u;
+    }
+    zero_hub_counts(ctr);
+}
+
+void k_init_vis(hipStream_t s, const DevGraph &g, DevState &st, bool elect_first)
 {
     uint32_t nchunks = st.nwords / 2;
     uint32_t blocks = (nchunks + 3) / 4;
-    blocks = blocks < 1 ? 1 : (blocks > 4096 ? 4096 : blocks);
-    init_vis_kernel<<<blocks, SWEEP_BLOCK, 0, s>>>(st.cls, st.fvis, st.bvis, st.ctr, g.V, nchunks);
+    blocks = blocks < 1 ? 1 : (blocks > VIS_BLOCKS ? VIS_BLOCKS : blocks);
+    if (elect_first) {
+        init_vis_kernel<true><<<blocks, SWEEP_BLOCK, 0, s>>>(st.cls, g.Fr, g.Br, st.fvis, st.bvis, st.bred, g.V,
+                                                             nchunks);
+        vis_finish_kernel<true><<<1, 1024, 0, s>>>(st.bred, blocks, st.ctr, st.piv, st.fvis, st.bvis, st.qf[0],
+                                                   st.qb[0], st.pivlist);
+    } else {
+        init_vis_kernel<false><<<blocks, SWEEP_BLOCK, 0, s>>>(st.cls, g.Fr, g.Br, st.fvis, st.bvis, st.bred, g.V,
+                                                              nchunks);
+        vis_finish_kernel<false><<<1, 1024, 0, s>>>(st.bred, blocks, st.ctr, st.piv, st.fvis, st.bvis, st.qf[0],
+                                                    st.qb[0], st.pivlist);
+    }
 }
 
 // ------------------------------------------------------------ pivots ---
@@ -416,38 +510,10 @@ __global__ void elect_kernel(const uint32_t *__restrict__ Fr, const uint32_t *__
 
 // First pivot: every live vertex is in class 0, so each workgroup reduces
 // its maximum key first and issues one atomicMax.
-__global__ void elect_single_kernel(const uint32_t *__restrict__ Fr,
-                                    const uint32_t *__restrict__ Br,
-                                    const uint32_t *__restrict__ cls, unsigned long long *piv,
-                                    uint32_t V)
+void k_elect(hipStream_t s, const DevGraph &g, DevState &st)
 {
-    __shared__ unsigned long long s_max[16];
-    unsigned long long m = 0;
-    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x + 1; v <= V; v += gridDim.x * blockDim.x)
-        if (cls[v] != DEAD) {
-            const unsigned long long k = pivot_key(Fr, Br, v);
-            m = k > m ? k : m;
-        }
-    for (int o = 32; o > 0; o >>= 1) {
-        unsigned long long x = __shfl_xor(m, o);
-        m = x > m ? x : m;
-    }
-    if (lane() == 0) s_max[threadIdx.x >> 6] = m;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        for (uint32_t w = 1; w < (blockDim.x >> 6); w++) m = s_max[w] > m ? s_max[w] : m;
-        if (m > piv[0]) atomicMax(&piv[0], m);
-    }
-}
-
-void k_elect(hipStream_t s, const DevGraph &g, DevState &st, bool single_class)
-{
-    if (single_class)
-        elect_single_kernel<<<sweep_blocks(g.V), SWEEP_BLOCK, 0, s>>>(
-            g.Fr, g.Br, st.cls, reinterpret_cast<unsigned long long *>(st.piv), g.V);
-    else
-        elect_kernel<<<sweep_blocks(g.V), SWEEP_BLOCK, 0, s>>>(
-            g.Fr, g.Br, st.cls, reinterpret_cast<unsigned long long *>(st.piv), g.V);
+    elect_kernel<<<sweep_blocks(g.V), SWEEP_BLOCK, 0, s>>>(g.Fr, g.Br, st.cls,
+                                                           reinterpret_cast<unsigned long long *>(st.piv), g.V);
 }
 
 // selectPivots (scc_kernels.cu:4-19): the winner of each class seeds the
@@ -470,14 +536,6 @@ __global__ void select_kernel(const uint32_t *__restrict__ Fr, const uint32_t *_
             qb[i] = v;
             atomicOr(&bvis[vword(v)], 1u << (v & 31));
         }
-    }
-}
-
-__device__ __forceinline__ void zero_hub_counts(uint32_t *ctr)
-{
-    for (int i = 0; i < 3; i++) {
-        *reinterpret_cast<unsigned long long *>(ctr + CT(C_BIGF + i)) = 0ull;
-        *reinterpret_cast<unsigned long long *>(ctr + CT(C_BIGB + i)) = 0ull;
     }
 }
 

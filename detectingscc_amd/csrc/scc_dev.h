@@ -30,6 +30,7 @@ constexpr uint32_t DEAD = 0xFFFFFFFFu;
 constexpr uint32_t LAB_FINAL = 0x80000000u;
 constexpr int SWEEP_BLOCK = 256;
 constexpr uint32_t LB_BIG = 512;   // degree from which a frontier vertex is a hub (LB mode)
+constexpr uint32_t VIS_BLOCKS = 4096;   // most workgroups of the bitmap initialisation
 
 // Visited bitmaps (fvis, bvis) keep two words per group of 32 vertices: the
 // first has a bit for every visited or finished vertex, the second one for
@@ -85,6 +86,7 @@ struct DevState {
     uint32_t *bigoff_f[3] = {}, *bigoff_b[3] = {};
     uint32_t *part = nullptr;      // partition ids (detectSCC)
     uint32_t *ctr = nullptr;       // C_COUNT words
+    uint64_t *bred = nullptr;      // per-workgroup (live count, best key) of the bitmap initialisation
     uint32_t nwords = 0;           // words of a one-bit-per-vertex bitmap (fvis/bvis hold twice that)
 };
 
@@ -229,9 +231,11 @@ void k_init_state(hipStream_t s, const DevGraph &g, DevState &st, const uint8_t 
 void k_trim1(hipStream_t s, const DevGraph &g, DevState &st, const uint32_t *in, uint32_t *out,
              uint32_t epoch, uint32_t pass, uint32_t sweep_share, uint32_t *live);
 void k_trim2(hipStream_t s, const DevGraph &g, DevState &st);
-void k_init_vis(hipStream_t s, const DevGraph &g, DevState &st);
+// Visited bitmaps of a search and C_NLIVE; with elect_first (every live
+// vertex in class 0) also the first pivot, seeded in both directions.
+void k_init_vis(hipStream_t s, const DevGraph &g, DevState &st, bool elect_first);
 // single_class: every live vertex is in class 0 (the first pivot)
-void k_elect(hipStream_t s, const DevGraph &g, DevState &st, bool single_class);
+void k_elect(hipStream_t s, const DevGraph &g, DevState &st);
 void k_select(hipStream_t s, const DevGraph &g, DevState &st, int seed_bw);
 void k_update(hipStream_t s, const DevGraph &g, DevState &st);
 void k_clear_piv(hipStream_t s, DevState &st, uint32_t max_pivots);
