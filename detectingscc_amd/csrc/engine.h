@@ -51,8 +51,9 @@ class Engine {
     scc_status reach(bool fw, bool bw, bool single, int expand, int W, uint32_t *levels_out);
     scc_status reach_do(bool fw, bool bw, bool single, uint32_t *levels_out);
     DirArgs dir_args(int d, uint32_t level) const;
-    // single: every live vertex has class 0 (the first trim of a decomposition)
-    scc_status trim1(bool fixpoint, bool single = false);
+    // single: every live vertex has class 0 (the first trim of a
+    // decomposition); none_dead: besides, no vertex is dead yet
+    scc_status trim1(bool fixpoint, bool single = false, bool none_dead = false);
     void init_vis(bool elect_first = false);
     uint64_t limit() const;
     scc_status no_convergence(const char *what);

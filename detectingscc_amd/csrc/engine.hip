@@ -532,14 +532,14 @@ scc_status Engine::reach_do(bool fw, bool bw, bool single, uint32_t *levels_out)
 static constexpr uint32_t kTrimSweepShare = 32;
 static constexpr uint32_t kTrimQueued = 4;
 
-scc_status Engine::trim1(bool fixpoint, bool single)
+scc_status Engine::trim1(bool fixpoint, bool single, bool none_dead)
 {
     mark(SCC_PHASE_TRIM1);
     zero_ctr(C_TLIST, 3);
     // the single-class live bitmap lives in the FW visited bitmap, which is
     // free until the next search
     uint32_t *live = single ? st.fvis : nullptr;
-    k_trim1(stream, g, st, st.qf[1], st.qf[0], 0u, 0, kTrimSweepShare, live);
+    k_trim1(stream, g, st, st.qf[1], st.qf[0], 0u, 0, kTrimSweepShare, live, fixpoint && single && none_dead);
     uint32_t pass = 1;
     while (fixpoint) {
         for (uint32_t k = 0; k < kTrimQueued; k++, pass++) {
@@ -547,7 +547,8 @@ scc_status Engine::trim1(bool fixpoint, bool single)
                 k_fill_u32(stream, st.stamp, 0u, (uint64_t)g.V + 1);
                 trim_epoch = 1;
             }
-            k_trim1(stream, g, st, st.qf[(pass + 1) & 1], st.qf[pass & 1], trim_epoch, pass, kTrimSweepShare, live);
+            k_trim1(stream, g, st, st.qf[(pass + 1) & 1], st.qf[pass & 1], trim_epoch, pass, kTrimSweepShare, live,
+                    true);
         }
         scc_status s = read_ctr();
         if (s != SCC_OK) return s;
@@ -678,7 +679,7 @@ scc_status Engine::run(const scc_opts &o, scc_stats *out)
     if (A == 'd') {
         if ((s = detect_phases(o, loop_expand, W, &stats, false)) != SCC_OK) return s;
     } else if (hong) {
-        if (o.trim1 && (s = trim1(true, true)) != SCC_OK) return s;
+        if (o.trim1 && (s = trim1(true, true, true)) != SCC_OK) return s;
         if ((s = pivot_round(true, first_expand, W, &stats, &live)) != SCC_OK) return s;
         if (o.trim1 && (s = trim1(true)) != SCC_OK) return s;
         if (o.trim2) {
@@ -737,7 +738,7 @@ scc_status Engine::detect_phases(const scc_opts &o, int expand, int W, scc_stats
 {
     if (!has_part) return fail(SCC_ERR_NOTREADY, "detectSCC needs a partition vector");
     scc_status s;
-    if (o.trim1 && (s = trim1(true, true)) != SCC_OK) return s;
+    if (o.trim1 && (s = trim1(true, true, true)) != SCC_OK) return s;
     if (o.trim2) {
         mark(SCC_PHASE_TRIM2);
         k_trim2(stream, g, st);
@@ -955,7 +956,7 @@ scc_status Engine::phase_trim1(const uint8_t *dead, uint8_t *removed)
     scc_status s;
     if ((s = phase_begin(dead)) != SCC_OK) return s;
     nmarks = 0;
-    if ((s = trim1(true, true)) != SCC_OK) return s;
+    if ((s = trim1(true, true, dead == nullptr)) != SCC_OK) return s;
     k_export_removed(stream, g, st, d_tmp8);
     if ((s = fetch8(removed)) != SCC_OK) return s;
     if (dead)

@@ -176,7 +176,8 @@ __global__ __launch_bounds__(256) void trim1_kernel(const uint32_t *__restrict__
                                                     const uint32_t *__restrict__ Bc, uint32_t *cls,
                                                     uint32_t *lab, uint32_t *live, uint32_t V, uint32_t *ctr,
                                                     const uint32_t *in, uint32_t *out, uint32_t *stamp,
-                                                    uint32_t epoch, uint32_t pass, uint32_t sweep_share)
+                                                    uint32_t epoch, uint32_t pass, uint32_t sweep_share,
+                                                    bool degree_pass)
 {
     __shared__ uint32_t s_q[4][TRIM_WQ];
     __shared__ uint32_t s_rounds;
@@ -202,7 +203,13 @@ __global__ __launch_bounds__(256) void trim1_kernel(const uint32_t *__restrict__
             bool gone = false;
             if (c != DEAD) {
                 const uint32_t fb = Fr[v], fe = Fr[v + 1], bb = Br[v], be = Br[v + 1];
-                if (!trim1_keep_at<BITS>(Fc, Bc, cls, live, c, fb, fe, bb, be)) {
+                // pass 0 of a fixpoint from a state with no dead vertex: every
+                // neighbour is live, so only a missing in- or out-arc removes
+                // a vertex (the next pass tests the neighbours of what this
+                // one removed)
+                const bool keep = (BITS && degree_pass) ? (fb < fe && bb < be)
+                                                        : trim1_keep_at<BITS>(Fc, Bc, cls, live, c, fb, fe, bb, be);
+                if (!keep) {
                     cls[v] = DEAD;
                     lab[v] = v | LAB_FINAL;
                     gone = true;
@@ -272,18 +279,19 @@ __global__ void live_bits_kernel(const uint32_t *__restrict__ cls, uint32_t *liv
 }
 
 void k_trim1(hipStream_t s, const DevGraph &g, DevState &st, const uint32_t *in, uint32_t *out,
-             uint32_t epoch, uint32_t pass, uint32_t sweep_share, uint32_t *live)
+             uint32_t epoch, uint32_t pass, uint32_t sweep_share, uint32_t *live, bool degree_first)
 {
     if (live) {
         if (pass == 0)
             live_bits_kernel<<<sweep_blocks(g.V), SWEEP_BLOCK, 0, s>>>(st.cls, live, g.V);
         trim1_kernel<true><<<sweep_blocks(g.V), SWEEP_BLOCK, 0, s>>>(g.Fr, g.Fc, g.Br, g.Bc, st.cls, st.lab,
                                                                      live, g.V, st.ctr, in, out, st.stamp,
-                                                                     epoch, pass, sweep_share);
+                                                                     epoch, pass, sweep_share,
+                                                                     degree_first && pass == 0);
     } else {
         trim1_kernel<false><<<sweep_blocks(g.V), SWEEP_BLOCK, 0, s>>>(g.Fr, g.Fc, g.Br, g.Bc, st.cls, st.lab,
                                                                       nullptr, g.V, st.ctr, in, out,
-                                                                      st.stamp, epoch, pass, sweep_share);
+                                                                      st.stamp, epoch, pass, sweep_share, false);
     }
 }
 

@@ -227,9 +227,11 @@ void k_init_state(hipStream_t s, const DevGraph &g, DevState &st, const uint8_t 
 // C_TLIST + (pass+1)%3.  `epoch` is new for every pass of the context
 // (st.stamp never holds it).  With `live` (only while every live vertex has
 // class 0), neighbour tests read a live bitmap there instead of classes;
-// pass 0 builds it from the class words.
+// pass 0 builds it from the class words.  With degree_first (a fixpoint
+// from a state where no vertex is dead yet) pass 0 removes only vertices
+// without an in- or out-arc: every neighbour is live then.
 void k_trim1(hipStream_t s, const DevGraph &g, DevState &st, const uint32_t *in, uint32_t *out,
-             uint32_t epoch, uint32_t pass, uint32_t sweep_share, uint32_t *live);
+             uint32_t epoch, uint32_t pass, uint32_t sweep_share, uint32_t *live, bool degree_first);
 void k_trim2(hipStream_t s, const DevGraph &g, DevState &st);
 // Visited bitmaps of a search and C_NLIVE; with elect_first (every live
 // vertex in class 0) also the first pivot, seeded in both directions.
