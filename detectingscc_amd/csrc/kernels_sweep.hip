@@ -853,8 +853,11 @@ __global__ void canon_reset_kernel(uint32_t *aux, uint32_t *cnt, uint32_t V)
 // instead of one per vertex.
 constexpr int MATCH_ROUNDS = 8;
 
+// Like the size counts below, each wave keeps a running minimum for one
+// label several of its lanes shared and writes it once at the end.
 __global__ void canon_min_kernel(uint32_t *lab, uint32_t *aux, uint32_t *ctr, uint32_t V)
 {
+    uint32_t heavy = DEAD, hmin = DEAD;
     WAVE_LOOP(v, valid, V)
     {
         uint32_t r = valid ? lab[v] : DEAD;
@@ -872,22 +875,34 @@ __global__ void canon_min_kernel(uint32_t *lab, uint32_t *aux, uint32_t *ctr, ui
             const int L = __ffsll((unsigned long long)pm) - 1;
             const uint32_t rl = __shfl(r, L);
             const bool mine = pending && r == rl;
+            const uint64_t mm = __ballot(mine);
             uint32_t m = mine ? v : DEAD;
             for (int o = 32; o > 0; o >>= 1) {
                 uint32_t x = __shfl_xor(m, o);
                 m = x < m ? x : m;
             }
-            if ((int)lane() == L && m < aux[rl]) atomicMin(&aux[rl], m);
+            if (rl == heavy) {
+                hmin = m < hmin ? m : hmin;
+            } else if (heavy == DEAD && __popcll(mm) > 1) {
+                heavy = rl;
+                hmin = m;
+            } else if ((int)lane() == L && m < aux[rl]) {
+                atomicMin(&aux[rl], m);
+            }
             pending = pending && !mine;
         }
         if (pending && v < aux[r]) atomicMin(&aux[r], v);
     }
+    if (heavy != DEAD && lane() == 0 && hmin < aux[heavy]) atomicMin(&aux[heavy], hmin);
 }
 
+// With sizes, each wave also keeps a running count for one label that
+// several of its lanes shared (a giant SCC recurs in every wave), added to
+// memory once at the end instead of once per wave iteration.
 __global__ void canon_apply_kernel(uint32_t *lab, const uint32_t *__restrict__ aux, uint32_t *cnt,
                                    uint32_t *ctr, uint32_t V, int sizes)
 {
-    uint32_t roots = 0;
+    uint32_t roots = 0, heavy = DEAD, hcount = 0;
     WAVE_LOOP(v, valid, V)
     {
         uint32_t r = DEAD;
@@ -904,12 +919,20 @@ __global__ void canon_apply_kernel(uint32_t *lab, const uint32_t *__restrict__ a
             if (!pm) break;
             const int L = __ffsll((unsigned long long)pm) - 1;
             const uint32_t rl = __shfl(r, L);
-            const uint64_t mm = __ballot(pending && r == rl);
-            if ((int)lane() == L) atomicAdd(&cnt[rl], (uint32_t)__popcll(mm));
+            const uint32_t n = (uint32_t)__popcll(__ballot(pending && r == rl));
+            if (rl == heavy) {
+                hcount += n;
+            } else if (heavy == DEAD && n > 1) {
+                heavy = rl;
+                hcount = n;
+            } else if ((int)lane() == L) {
+                atomicAdd(&cnt[rl], n);
+            }
             pending = pending && r != rl;
         }
         if (pending) atomicAdd(&cnt[r], 1u);
     }
+    if (hcount && lane() == 0) atomicAdd(&cnt[heavy], hcount);
     wave_add32(&ctr[CT(C_NSCC)], roots);
 }
 

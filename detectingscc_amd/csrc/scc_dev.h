@@ -126,11 +126,15 @@ struct DoArgs {
 // or more than 1/kToBottomUpBurst of them right after growing kBurst-fold
 // in one level (a frontier that bursts -- the neighbours of a hub -- has a
 // high average degree, which the vertex counts alone do not show), and back
-// to top-down when it falls under 1/kToTopDown of the live vertices.
+// to top-down when it falls under 1/kToTopDown of the live vertices.  A
+// bottom-up pass reads every vertex's bitmap word, so a frontier under
+// 1/kBottomUpFloor of the live vertices stays top-down even when few
+// vertices remain unvisited (the tail of a search).
 constexpr int64_t kToBottomUp = 16;
 constexpr int64_t kToBottomUpBurst = 128;
 constexpr int64_t kBurst = 4;
 constexpr int64_t kToTopDown = 48;
+constexpr int64_t kBottomUpFloor = 256;
 
 enum : uint32_t { DIR_OFF = 0, DIR_TD = 1, DIR_BU = 2 };
 
@@ -160,7 +164,9 @@ __host__ __device__ __forceinline__ DirState dir_decide(DirState s, uint32_t nf,
     s.nf = nf;
     s.visited += nf;
     const int64_t remaining = (int64_t)nlive - (int64_t)s.visited;
-    if (s.mode == DIR_TD && (int64_t)nf * (grew ? kToBottomUpBurst : kToBottomUp) > remaining) s.mode = DIR_BU;
+    if (s.mode == DIR_TD && (int64_t)nf * (grew ? kToBottomUpBurst : kToBottomUp) > remaining &&
+        (int64_t)nf * kBottomUpFloor >= (int64_t)nlive)
+        s.mode = DIR_BU;
     else if (s.mode == DIR_BU && (int64_t)nf * kToTopDown < (int64_t)nlive) s.mode = DIR_TD;
     return s;
 }
