@@ -439,10 +439,18 @@ template <int BS> __device__ uint32_t g_canon(Lds &L, const G1 &g)
 
 } // namespace
 
+// Words of one visited bitmap held in LDS: graphs of up to 2^16 vertices
+// keep both bitmaps there, so probes and claims are LDS operations (a
+// device-wide atomic costs a share of a chip-wide rate of about 10 G/s);
+// larger graphs use their slots in global memory.  The views are generic
+// pointers, so the phase code is the same either way.
+constexpr uint32_t BVIS_LDS_WORDS = 2 * ((65536 + 64) / 64);
+
 template <int BS>
 __global__ __launch_bounds__(BS) void batch_scc_kernel(BatchDev B, uint32_t first, uint32_t count)
 {
     __shared__ Lds L;
+    __shared__ uint32_t s_vis[2][BVIS_LDS_WORDS];
     for (;;) {
         __syncthreads();
         if (threadIdx.x == 0) L.graph = atomicAdd(B.next, 1u);
@@ -460,8 +468,10 @@ __global__ __launch_bounds__(BS) void batch_scc_kernel(BatchDev B, uint32_t firs
         g.cls = B.cls + vo;
         g.lab = B.lab + vo;
         g.aux = B.aux + vo;
-        g.fvis = B.fvis + vo / 32;
-        g.bvis = B.bvis + vo / 32;
+        // g_init_vis writes 2 * ((V + 64) / 64) words of each bitmap
+        const bool lds_vis = 2 * ((g.V + 64) / 64) <= BVIS_LDS_WORDS;
+        g.fvis = lds_vis ? s_vis[0] : B.fvis + vo / 32;
+        g.bvis = lds_vis ? s_vis[1] : B.bvis + vo / 32;
         g.piv = B.piv + 3 * vo;
         g.q = B.q + 4 * vo;
         g.pivlist = B.pivlist + vo;
