@@ -248,11 +248,27 @@ template <int BS> __device__ uint32_t g_pivots(Lds &L, const G1 &g, bool first)
         __syncthreads();
         if (threadIdx.x == 0) g.piv[0] = L.key;
     } else {
-        for (uint32_t v = threadIdx.x + 1; v <= g.V; v += BS) {
-            const uint32_t c = g.cls[v];
-            if (c == DEAD) continue;
-            const unsigned long long k = bkey(g.Fr, g.Br, v);
-            if (k > g.piv[c]) atomicMax(reinterpret_cast<unsigned long long *>(&g.piv[c]), k);
+        // a wave whose live lanes share a class does one atomic
+        unsigned long long *piv = reinterpret_cast<unsigned long long *>(g.piv);
+        for (uint32_t base = threadIdx.x & ~63u; base <= g.V; base += BS) {
+            const uint32_t v = base + ln();
+            const uint32_t c = (v >= 1 && v <= g.V) ? g.cls[v] : DEAD;
+            const bool ok = c != DEAD;
+            const uint64_t live = __ballot(ok);
+            if (!live) continue;
+            const int lead = __ffsll((unsigned long long)live) - 1;
+            const uint32_t c0 = __shfl(c, lead);
+            unsigned long long k = ok ? bkey(g.Fr, g.Br, v) : 0ull;
+            if (__all(!ok || c == c0)) {
+                for (int o = 32; o > 0; o >>= 1) {
+                    const unsigned long long x = __shfl_xor(k, o);
+                    k = x > k ? x : k;
+                }
+                if ((int)ln() == lead && k > __hip_atomic_load(&piv[c0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+                    atomicMax(&piv[c0], k);
+            } else if (ok && k > __hip_atomic_load(&piv[c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+                atomicMax(&piv[c], k);
+            }
         }
     }
     if (threadIdx.x == 0) L.npiv = 0;
@@ -417,13 +433,36 @@ template <int BS> __device__ void g_color_loop(Lds &L, const G1 &g, uint32_t W, 
 }
 
 // canonical labels; returns the number of SCCs
+// The minimum per label: lanes of a wave that share a label combine first
+// (up to four labels per wave step, then lane by lane), and the guard reads
+// the current minimum at agent scope (the CU's L1 may hold a stale copy),
+// so a large SCC costs about one atomic per wave instead of one per vertex.
 template <int BS> __device__ uint32_t g_canon(Lds &L, const G1 &g)
 {
     for (uint32_t v = threadIdx.x; v <= g.V; v += BS) g.aux[v] = DEAD;
     __syncthreads();
-    for (uint32_t v = threadIdx.x + 1; v <= g.V; v += BS) {
-        const uint32_t l = g.lab[v];
-        if (!(l & LAB_FINAL) && v < g.aux[l]) atomicMin(&g.aux[l], v);
+    for (uint32_t base = threadIdx.x & ~63u; base <= g.V; base += BS) {
+        const uint32_t v = base + ln();
+        const uint32_t l = (v >= 1 && v <= g.V) ? g.lab[v] : LAB_FINAL;
+        bool pending = !(l & LAB_FINAL);
+        for (int it = 0; it < 4; it++) {
+            const uint64_t pm = __ballot(pending);
+            if (!pm) break;
+            const int lead = __ffsll((unsigned long long)pm) - 1;
+            const uint32_t rl = __shfl(l, lead);
+            const bool mine = pending && l == rl;
+            uint32_t m = mine ? v : DEAD;
+            for (int o = 32; o > 0; o >>= 1) {
+                const uint32_t x = __shfl_xor(m, o);
+                m = x < m ? x : m;
+            }
+            if ((int)ln() == lead &&
+                m < __hip_atomic_load(&g.aux[rl], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+                atomicMin(&g.aux[rl], m);
+            pending = pending && !mine;
+        }
+        if (pending && v < __hip_atomic_load(&g.aux[l], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+            atomicMin(&g.aux[l], v);
     }
     __syncthreads();
     uint32_t roots = 0;
