@@ -96,7 +96,7 @@ def cpu_baseline(sample_scale: int, threads: int):
                       f"median of {len(runs)} runs"}
 
 
-def bench_batched(graphs=4096, steps=3, sample=1024):
+def bench_batched(graphs=4096, steps=3, sample=1024, serial_baseline=True):
     import detectingscc_amd as scc
     t0 = time.perf_counter()
     b = scc.gen_batch_mix(graphs, 8, 16, 2.0, 16.0, seed=4096)
@@ -109,19 +109,25 @@ def bench_batched(graphs=4096, steps=3, sample=1024):
         runs = [ctx.batch_run(autotuned=True) for _ in range(steps)]
         best = min(runs, key=lambda r: r.ms)
         labs, nscc = ctx.batch_labels()
-        serial = ctx.batch_run_serial(expand=4)              # one graph per launch, DO expansion
-        slabs, snscc = ctx.batch_labels()
-    same = bool(np.array_equal(nscc, snscc)) and all(np.array_equal(x, y) for x, y in zip(labs, slabs))
-    log(f"batched {best.ms:.2f} ms ({best.graphs_per_s:.0f} graphs/s), one-graph-per-launch "
-        f"{serial.ms:.0f} ms ({serial.graphs_per_s:.0f} graphs/s), labels identical: {same}")
+        serial = None
+        if serial_baseline:
+            serial = ctx.batch_run_serial(expand=4)          # one graph per launch, DO expansion
+            slabs, snscc = ctx.batch_labels()
+    same = None
+    if serial is not None:
+        same = bool(np.array_equal(nscc, snscc)) and all(np.array_equal(x, y) for x, y in zip(labs, slabs))
+        log(f"batched {best.ms:.2f} ms ({best.graphs_per_s:.0f} graphs/s), one-graph-per-launch "
+            f"{serial.ms:.0f} ms ({serial.graphs_per_s:.0f} graphs/s), labels identical: {same}")
     return {
         "workload": f"{graphs} graphs, V log-uniform [2^8, 2^16], degree [2, 16], 40% ER / 40% R-MAT / "
                     "20% planted cycles 2..512, seed 4096",
         "vertices": int(b.V.sum()), "arcs": int(b.E.sum()),
         "graphs_per_s": best.graphs_per_s, "ms": best.ms, "p50_ms": best.p50_ms, "p99_ms": best.p99_ms,
         "launches": best.launches, "steps": steps,
-        "serial_graphs_per_s": serial.graphs_per_s, "serial_ms": serial.ms, "serial_p99_ms": serial.p99_ms,
-        "speedup_vs_one_graph_per_launch": best.graphs_per_s / serial.graphs_per_s,
+        "serial_graphs_per_s": serial.graphs_per_s if serial else None,
+        "serial_ms": serial.ms if serial else None,
+        "serial_p99_ms": serial.p99_ms if serial else None,
+        "speedup_vs_one_graph_per_launch": best.graphs_per_s / serial.graphs_per_s if serial else None,
         "labels_identical": same,
         "autotune": report.strip().splitlines(),
     }
@@ -180,6 +186,8 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-mix measurement")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving measurement")
     ap.add_argument("--serve-rate", type=float, default=1000.0)
+    ap.add_argument("--no-serial", action="store_true",
+                    help="batched mix: skip the one-graph-per-launch baseline (for profiling the batched launch)")
     ap.add_argument("--copy-calibrate", type=int, default=0, metavar="BYTES",
                     help="only run a plain device copy of BYTES (HBM counter calibration)")
     args = ap.parse_args()
@@ -195,7 +203,8 @@ def main():
                           "GBps_read_plus_write": 2 * args.copy_calibrate / ms / 1e6}), flush=True)
         return
     if args.workload in ("batch4096", "serve"):
-        part = bench_batched(steps=args.steps) if args.workload == "batch4096" else bench_serving(rate=args.serve_rate)
+        part = (bench_batched(steps=args.steps, serial_baseline=not args.no_serial) if args.workload == "batch4096"
+                else bench_serving(rate=args.serve_rate))
         print(json.dumps({
             "metric": "batched graphs/s and p99 vs one-graph-per-launch" if args.workload == "batch4096"
                       else "served graphs/s and p50/p99 latency under an HBM budget",
