@@ -180,8 +180,7 @@ __global__ __launch_bounds__(256) void trim1_kernel(const uint32_t *__restrict__
                                                     const uint32_t *__restrict__ Bc, uint32_t *cls,
                                                     uint32_t *lab, uint32_t *live, uint32_t V, uint32_t *ctr,
                                                     const uint32_t *in, uint32_t *out, uint32_t *stamp,
-                                                    uint32_t epoch, uint32_t pass, uint32_t sweep_share,
-                                                    bool degree_pass)
+                                                    uint32_t epoch, uint32_t pass, uint32_t sweep_share)
 {
     __shared__ uint32_t s_q[4][TRIM_WQ];
     __shared__ uint32_t s_rounds;
@@ -207,13 +206,7 @@ __global__ __launch_bounds__(256) void trim1_kernel(const uint32_t *__restrict__
             bool gone = false;
             if (c != DEAD) {
                 const uint32_t fb = Fr[v], fe = Fr[v + 1], bb = Br[v], be = Br[v + 1];
-                // pass 0 of a fixpoint from a state with no dead vertex: every
-                // neighbour is live, so only a missing in- or out-arc removes
-                // a vertex (the next pass tests the neighbours of what this
-                // one removed)
-                const bool keep = (BITS && degree_pass) ? (fb < fe && bb < be)
-                                                        : trim1_keep_at<BITS>(Fc, Bc, cls, live, c, fb, fe, bb, be);
-                if (!keep) {
+                if (!trim1_keep_at<BITS>(Fc, Bc, cls, live, c, fb, fe, bb, be)) {
                     cls[v] = DEAD;
                     lab[v] = v | LAB_FINAL;
                     gone = true;
@@ -270,6 +263,57 @@ __global__ __launch_bounds__(256) void trim1_kernel(const uint32_t *__restrict__
     wave_add32(&ctr[CT(C_TRIM1)], removed);
 }
 
+// Pass 0 of a single-class fixpoint from a state with no dead vertex, fused
+// with building the live bitmap: a vertex without an in- or out-arc is
+// removed (listed in `out`), every other live vertex gets its bit.  Pure
+// streaming of the class words and row offsets, so each wave keeps
+// TRIM_DEG_CHUNKS 64-vertex chunks' loads in flight.
+constexpr uint32_t TRIM_DEG_CHUNKS = 4;
+
+__global__ __launch_bounds__(256) void trim1_degree_kernel(const uint32_t *__restrict__ Fr,
+                                                           const uint32_t *__restrict__ Br, uint32_t *cls,
+                                                           uint32_t *lab, uint32_t *live, uint32_t V,
+                                                           uint32_t *ctr, uint32_t *out)
+{
+    __shared__ uint32_t s_q[4][TRIM_WQ];
+    const uint32_t wv = threadIdx.x >> 6;
+    uint32_t *wq = s_q[wv];
+    uint32_t nq = 0, removed = 0;
+    uint32_t *cnt_out = &ctr[CT(C_TLIST + 1)];
+    if (blockIdx.x == 0 && threadIdx.x == 0) ctr[CT(C_TLIST + 2)] = 0;
+    const uint32_t stride = gridDim.x * 256;
+    for (uint32_t base = (blockIdx.x * 4 + wv) * 64; base <= V; base += TRIM_DEG_CHUNKS * stride) {
+        if (nq + TRIM_DEG_CHUNKS * 64 > TRIM_WQ) wave_flush(wq, nq, cnt_out, out);
+        uint32_t v[TRIM_DEG_CHUNKS], c[TRIM_DEG_CHUNKS], f0[TRIM_DEG_CHUNKS], f1[TRIM_DEG_CHUNKS],
+            b0[TRIM_DEG_CHUNKS], b1[TRIM_DEG_CHUNKS];
+#pragma unroll
+        for (uint32_t h = 0; h < TRIM_DEG_CHUNKS; h++) {
+            v[h] = base + h * stride + lane();   // vertex 0, the dummy, is dead
+            const bool in = v[h] <= V;
+            c[h] = in ? cls[v[h]] : DEAD;
+            f0[h] = in ? Fr[v[h]] : 0u;
+            f1[h] = in ? Fr[v[h] + 1] : 0u;
+            b0[h] = in ? Br[v[h]] : 0u;
+            b1[h] = in ? Br[v[h] + 1] : 0u;
+        }
+#pragma unroll
+        for (uint32_t h = 0; h < TRIM_DEG_CHUNKS; h++) {
+            const bool alive = c[h] != DEAD;
+            const bool gone = alive && (f0[h] == f1[h] || b0[h] == b1[h]);
+            if (gone) {
+                cls[v[h]] = DEAD;
+                lab[v[h]] = v[h] | LAB_FINAL;
+            }
+            const uint64_t m = __ballot(alive && !gone);
+            if ((lane() & 31) == 0 && v[h] <= V) live[v[h] >> 5] = lane() < 32 ? (uint32_t)m : (uint32_t)(m >> 32);
+            removed += gone;
+            wave_push(wq, nq, gone, v[h]);
+        }
+    }
+    wave_flush_block(wq, nq, cnt_out, out);
+    wave_add32(&ctr[CT(C_TRIM1)], removed);
+}
+
 // The live bitmap of the single-class trim: bit v set for every vertex
 // whose class word is not DEAD (vertex 0 included, as dead).
 __global__ void live_bits_kernel(const uint32_t *__restrict__ cls, uint32_t *live, uint32_t V)
@@ -285,17 +329,19 @@ __global__ void live_bits_kernel(const uint32_t *__restrict__ cls, uint32_t *liv
 void k_trim1(hipStream_t s, const DevGraph &g, DevState &st, const uint32_t *in, uint32_t *out,
              uint32_t epoch, uint32_t pass, uint32_t sweep_share, uint32_t *live, bool degree_first)
 {
-    if (live) {
+    if (live && pass == 0 && degree_first) {
+        trim1_degree_kernel<<<sweep_blocks(g.V), SWEEP_BLOCK, 0, s>>>(g.Fr, g.Br, st.cls, st.lab, live, g.V,
+                                                                      st.ctr, out);
+    } else if (live) {
         if (pass == 0)
             live_bits_kernel<<<sweep_blocks(g.V), SWEEP_BLOCK, 0, s>>>(st.cls, live, g.V);
         trim1_kernel<true><<<sweep_blocks(g.V), SWEEP_BLOCK, 0, s>>>(g.Fr, g.Fc, g.Br, g.Bc, st.cls, st.lab,
                                                                      live, g.V, st.ctr, in, out, st.stamp,
-                                                                     epoch, pass, sweep_share,
-                                                                     degree_first && pass == 0);
+                                                                     epoch, pass, sweep_share);
     } else {
         trim1_kernel<false><<<sweep_blocks(g.V), SWEEP_BLOCK, 0, s>>>(g.Fr, g.Fc, g.Br, g.Bc, st.cls, st.lab,
                                                                       nullptr, g.V, st.ctr, in, out,
-                                                                      st.stamp, epoch, pass, sweep_share, false);
+                                                                      st.stamp, epoch, pass, sweep_share);
     }
 }
 
