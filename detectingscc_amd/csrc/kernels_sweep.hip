@@ -17,12 +17,18 @@ namespace sccamd {
 
 __device__ __forceinline__ uint32_t lane() { return threadIdx.x & 63u; }
 
+// pivot key of v from its out-degree and in-degree
+__device__ __forceinline__ uint64_t pivot_key_deg(uint32_t outdeg, uint32_t indeg, uint32_t v)
+{
+    uint64_t p = (uint64_t)outdeg * (uint64_t)indeg;
+    p = p > 0xFFFFFFFFull ? 0xFFFFFFFFull : p;
+    return (p << 32) | (uint64_t)(0xFFFFFFFFu - v);
+}
+
 __device__ __forceinline__ uint64_t pivot_key(const uint32_t *__restrict__ Fr,
                                               const uint32_t *__restrict__ Br, uint32_t v)
 {
-    uint64_t p = (uint64_t)(Fr[v + 1] - Fr[v]) * (uint64_t)(Br[v + 1] - Br[v]);
-    p = p > 0xFFFFFFFFull ? 0xFFFFFFFFull : p;
-    return (p << 32) | (uint64_t)(0xFFFFFFFFu - v);
+    return pivot_key_deg(Fr[v + 1] - Fr[v], Br[v + 1] - Br[v], v);
 }
 
 __device__ __forceinline__ uint32_t key_vertex(uint64_t k) { return 0xFFFFFFFFu - (uint32_t)k; }
@@ -429,6 +435,8 @@ __device__ __forceinline__ void zero_hub_counts(uint32_t *ctr)
 // its vertex, so no second sweep is needed to find it.  Each workgroup
 // leaves its count and key in `bred`; vis_finish_kernel reduces them with
 // one workgroup (thousands of same-address atomics would serialise).
+constexpr uint32_t VIS_CHUNKS = 4;
+
 template <bool ELECT>
 __global__ __launch_bounds__(256) void init_vis_kernel(const uint32_t *__restrict__ cls,
                                                        const uint32_t *__restrict__ Fr,
@@ -441,19 +449,36 @@ __global__ __launch_bounds__(256) void init_vis_kernel(const uint32_t *__restric
     const uint32_t nwaves = (gridDim.x * blockDim.x) >> 6;
     uint32_t live = 0;
     uint64_t best = 0;
-    for (uint32_t ch = wave; ch < nchunks; ch += nwaves) {
-        uint32_t v = ch * 64 + lane();
-        bool dead = v > V || cls[v] == DEAD;
-        live += !dead;
-        if (ELECT && !dead) {
-            const uint64_t k = pivot_key(Fr, Br, v);
-            best = k > best ? k : best;
+    // VIS_CHUNKS chunks per step keep that many loads in flight per wave
+    for (uint32_t ch0 = wave; ch0 < nchunks; ch0 += VIS_CHUNKS * nwaves) {
+        uint32_t c[VIS_CHUNKS], f0[VIS_CHUNKS], f1[VIS_CHUNKS], b0[VIS_CHUNKS], b1[VIS_CHUNKS];
+#pragma unroll
+        for (uint32_t h = 0; h < VIS_CHUNKS; h++) {
+            const uint32_t ch = ch0 + h * nwaves, v = ch * 64 + lane();
+            const bool in = ch < nchunks && v <= V;
+            c[h] = in ? cls[v] : DEAD;
+            if (ELECT) {
+                f0[h] = in ? Fr[v] : 0u;
+                f1[h] = in ? Fr[v + 1] : 0u;
+                b0[h] = in ? Br[v] : 0u;
+                b1[h] = in ? Br[v + 1] : 0u;
+            }
         }
-        uint64_t m = __ballot(dead);
-        // the chunk's two groups, each as (visited-or-finished, finished)
-        const uint32_t w = (lane() & 2) ? (uint32_t)(m >> 32) : (uint32_t)m;
-        uint32_t *dst = lane() < 4 ? fvis : bvis;
-        if (lane() < 8) dst[4 * ch + (lane() & 3)] = w;
+#pragma unroll
+        for (uint32_t h = 0; h < VIS_CHUNKS; h++) {
+            const uint32_t ch = ch0 + h * nwaves, v = ch * 64 + lane();
+            const bool dead = c[h] == DEAD;
+            live += !dead;
+            if (ELECT && !dead) {
+                const uint64_t k = pivot_key_deg(f1[h] - f0[h], b1[h] - b0[h], v);
+                best = k > best ? k : best;
+            }
+            const uint64_t m = __ballot(dead);
+            // the chunk's two groups, each as (visited-or-finished, finished)
+            const uint32_t w = (lane() & 2) ? (uint32_t)(m >> 32) : (uint32_t)m;
+            uint32_t *dst = lane() < 4 ? fvis : bvis;
+            if (lane() < 8 && ch < nchunks) dst[4 * ch + (lane() & 3)] = w;
+        }
     }
     if (ELECT) {
         for (int o = 32; o > 0; o >>= 1) {
@@ -639,29 +664,46 @@ void k_seed_single(hipStream_t s, DevState &st, uint32_t src, int dir)
 // update (scc_kernels.cu:124-142): FW and BW visited -> the pivot's SCC;
 // otherwise the class splits into 3*pivot + (fw + 2*bw), which never wraps
 // (defect D9) because each vertex is a pivot at most once.
-__global__ void update_kernel(uint32_t *cls, uint32_t *lab, const uint64_t *__restrict__ piv,
-                              const uint32_t *__restrict__ fvis, const uint32_t *__restrict__ bvis,
-                              uint32_t *ctr, uint32_t V)
+// Each lane takes UPD_CHUNKS vertices per step: their class words and
+// visited words load together, then their pivot keys.
+constexpr uint32_t UPD_CHUNKS = 2;
+
+__global__ __launch_bounds__(256) void update_kernel(uint32_t *cls, uint32_t *lab, const uint64_t *__restrict__ piv,
+                                                     const uint32_t *__restrict__ fvis,
+                                                     const uint32_t *__restrict__ bvis, uint32_t *ctr, uint32_t V)
 {
     uint32_t live = 0, found = 0;
-    WAVE_LOOP(v, valid, V)
-    {
-        if (!valid) continue;
-        const uint32_t c = cls[v];
-        if (c == DEAD) continue;
-        uint32_t p = key_vertex(piv[c]);
-        if (p < 1 || p > V) {          // no election for this class: cannot happen
-            atomicAdd(&ctr[CT(C_ANOM)], 1u);
-            p = v;
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t base = blockIdx.x * blockDim.x + (threadIdx.x & ~63u); base <= V; base += UPD_CHUNKS * stride) {
+        uint32_t v[UPD_CHUNKS], c[UPD_CHUNKS], fw[UPD_CHUNKS], bw[UPD_CHUNKS];
+        uint64_t key[UPD_CHUNKS];
+#pragma unroll
+        for (uint32_t h = 0; h < UPD_CHUNKS; h++) {
+            v[h] = base + h * stride + lane();   // vertex 0, the dummy, is dead
+            const bool in = v[h] <= V;
+            c[h] = in ? cls[v[h]] : DEAD;
+            fw[h] = in ? fvis[vword(v[h])] : 0u;
+            bw[h] = in ? bvis[vword(v[h])] : 0u;
         }
-        const uint32_t f = bit_of(fvis, v), b = bit_of(bvis, v);
-        if (f && b) {
-            lab[v] = p;
-            cls[v] = DEAD;
-            found += v == p;
-        } else {
-            cls[v] = 3u * p + f + 2u * b;
-            live++;
+#pragma unroll
+        for (uint32_t h = 0; h < UPD_CHUNKS; h++) key[h] = c[h] != DEAD ? piv[c[h]] : 0ull;
+#pragma unroll
+        for (uint32_t h = 0; h < UPD_CHUNKS; h++) {
+            if (c[h] == DEAD) continue;
+            uint32_t p = key_vertex(key[h]);
+            if (p < 1 || p > V) {          // no election for this class: cannot happen
+                atomicAdd(&ctr[CT(C_ANOM)], 1u);
+                p = v[h];
+            }
+            const uint32_t f = (fw[h] >> (v[h] & 31)) & 1u, b = (bw[h] >> (v[h] & 31)) & 1u;
+            if (f && b) {
+                lab[v[h]] = p;
+                cls[v[h]] = DEAD;
+                found += v[h] == p;
+            } else {
+                cls[v[h]] = 3u * p + f + 2u * b;
+                live++;
+            }
         }
     }
     wave_add32(&ctr[CT(C_LIVE)], live);
