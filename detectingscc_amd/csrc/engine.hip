@@ -3,10 +3,13 @@
 // for the device engine.
 //
 // The reference drives every phase from the host with a 1-byte D2H after
-// each kernel (scc_WCC.cu:683-686).  Here a BFS search queues `check_every`
-// levels back to back (queue counts live on the device) before the host
-// reads the counters once; sweeps that run to a fixpoint read one counter
-// word per pass.
+// each kernel (scc_WCC.cu:683-686).  Here queue counts and fixpoint flags
+// live on the device: a direction-optimised search decides each level on
+// the device and the host queues windows of levels per look at the
+// counters; the reference mappings queue `check_every` levels; trim-1
+// queues several passes per look; other fixpoints look once per sweep.
+// A look is a one-workgroup kernel that publishes the counters into
+// mapped host memory, on which the host spins.
 #include <chrono>
 #include <cstring>
 #include <memory>

@@ -136,7 +136,8 @@ typedef struct {
     int warp;         /* -w: 1,2,4,8,16,32,64 */
     int expand;       /* SCC_EXPAND_* */
     int block;        /* reserved, 0 */
-    int check_every;  /* BFS levels between host checks, 0 = auto */
+    int check_every;  /* BFS levels between host checks for the vertex, virtual-warp and LB
+                         mappings, 0 = 2 (direction-optimised searches decide on the device) */
     int sizes;        /* 1: also compute stats.largest and the first search's
                          vertex/arc counts (two more sweeps) */
 } scc_opts;
