@@ -445,13 +445,17 @@ scc_status Engine::reach(bool fw, bool bw, bool single, int expand, int W, uint3
 // Direction-optimised search.  Each level is one launch whose workgroups
 // choose top-down or bottom-up per direction on the device (dir_decide), so
 // the host queues a window of levels per look at the counters: kDoWindow
-// levels on the full grid, or kSpecLevels levels on a grid sized for a few
+// levels on the full grid (kFirstWindow for a single-class search, the
+// first of a decomposition: power-law graphs finish it within that many
+// levels, which saves the GPU a wait for the host in the middle), or
+// kSpecLevels levels on a grid sized for a few
 // times the current frontier while every direction is top-down, has no hubs
 // pending, grew at most kSpecRise-fold since the previous look and would
 // have to grow kSpecGrowth-fold more to reach the bottom-up switch point
 // (high-diameter graphs run thousands of small levels).  Levels queued past
 // the end of the search are no-ops.
 static constexpr int kDoWindow = 4;
+static constexpr int kFirstWindow = 8;
 static constexpr int kSpecLevels = 16;
 static constexpr int64_t kSpecGrowth = 8;
 static constexpr int64_t kSpecRise = 2;
@@ -475,7 +479,7 @@ scc_status Engine::reach_do(bool fw, bool bw, bool single, uint32_t *levels_out)
     }
     int64_t nf_prev[2] = {0, 0};
     uint32_t level = 0, blocks = full;
-    int window = kDoWindow;
+    int window = single ? kFirstWindow : kDoWindow;
     for (;;) {
         for (int k = 0; k < window; k++, level++) {
             for (int i = 0; i < nd; i++) a.d[i] = dir_args(dirs[i], level);
