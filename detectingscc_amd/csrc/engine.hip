@@ -688,19 +688,24 @@ scc_status Engine::run(const scc_opts &o, scc_stats *out)
     } else if (hong) {
         if (o.trim1 && (s = trim1(true, true, true)) != SCC_OK) return s;
         if ((s = pivot_round(true, first_expand, W, &stats, &live)) != SCC_OK) return s;
-        if (o.trim1 && (s = trim1(true)) != SCC_OK) return s;
-        if (o.trim2) {
-            mark(SCC_PHASE_TRIM2);
-            k_trim2(stream, g, st);
-        }
-        if (o.trim1 && (s = trim1(true)) != SCC_OK) return s;
-        if ((s = min_label()) != SCC_OK) return s;
-        for (;;) {
-            // a round with no live vertex elects nothing and reports live = 0
-            if ((s = pivot_round(false, loop_expand, W, &stats, &live)) != SCC_OK) return s;
-            if (!live) break;
-            // each round finishes at least its pivots' SCCs
-            if (++stats.rounds > limit()) return no_convergence("pivot rounds");
+        // with no live vertex left (a power-law graph's first search often
+        // takes everything trim-1 left) every later phase removes nothing,
+        // elects nothing and changes no label
+        if (live) {
+            if (o.trim1 && (s = trim1(true)) != SCC_OK) return s;
+            if (o.trim2) {
+                mark(SCC_PHASE_TRIM2);
+                k_trim2(stream, g, st);
+            }
+            if (o.trim1 && (s = trim1(true)) != SCC_OK) return s;
+            if ((s = min_label()) != SCC_OK) return s;
+            for (;;) {
+                // a round with no live vertex elects nothing and reports live = 0
+                if ((s = pivot_round(false, loop_expand, W, &stats, &live)) != SCC_OK) return s;
+                if (!live) break;
+                // each round finishes at least its pivots' SCCs
+                if (++stats.rounds > limit()) return no_convergence("pivot rounds");
+            }
         }
     } else {
         // vSlota/wSlota: one trim-1 pass, -q ignored (scc_Coloring.cu:507-508)
