@@ -678,8 +678,7 @@ scc_status Engine::run(const scc_opts &o, scc_stats *out)
     nmarks = 0;
     first_reach_marks[0] = first_reach_marks[1] = -1;
     mark(-1);
-    k_init_state(stream, g, st, nullptr);
-    zero_ctr(0, C_COUNT);
+    k_init_state(stream, g, st, nullptr);   // also zeroes the counters
     scc_status s;
     uint32_t live = g.V;
     const bool hong = (A == 'g' || A == 'x');
@@ -924,8 +923,7 @@ scc_status Engine::phase_begin(const uint8_t *dead)
     HIP_TRY(hipSetDevice(device));
     ran = false;
     if (dead) HIP_TRY(hipMemcpyAsync(d_tmp8, dead, (size_t)g.V + 1, hipMemcpyHostToDevice, stream));
-    k_init_state(stream, g, st, dead ? d_tmp8 : nullptr);
-    zero_ctr(0, C_COUNT);
+    k_init_state(stream, g, st, dead ? d_tmp8 : nullptr);   // also zeroes the counters
     return SCC_OK;
 }
 

@@ -96,13 +96,17 @@ void k_fill_u32(hipStream_t s, uint32_t *p, uint32_t val, uint64_t n)
 // All vertices start live in class 0 (range 0, scc_WCC.cu:630-631) unless
 // `dead` marks them finished; lab[v] = v.
 __global__ void init_state_kernel(uint32_t *cls, uint32_t *lab, uint64_t *piv, const uint8_t *dead,
-                                  uint32_t V)
+                                  uint32_t *ctr, uint32_t V)
 {
     uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
     if (tid == 0) {
         cls[0] = DEAD;
         lab[0] = 0;
         piv[0] = 0;
+    }
+    if (tid < C_COUNT) {   // both words of every counter (64-bit ones too)
+        ctr[CT(tid)] = 0;
+        ctr[CT(tid) + 1] = 0;
     }
     for (uint32_t v = tid + 1; v <= V; v += gridDim.x * blockDim.x) {
         cls[v] = (dead && dead[v]) ? DEAD : 0u;
@@ -112,7 +116,7 @@ __global__ void init_state_kernel(uint32_t *cls, uint32_t *lab, uint64_t *piv, c
 
 void k_init_state(hipStream_t s, const DevGraph &g, DevState &st, const uint8_t *dead_dev)
 {
-    init_state_kernel<<<sweep_blocks(g.V), SWEEP_BLOCK, 0, s>>>(st.cls, st.lab, st.piv, dead_dev, g.V);
+    init_state_kernel<<<sweep_blocks(g.V), SWEEP_BLOCK, 0, s>>>(st.cls, st.lab, st.piv, dead_dev, st.ctr, g.V);
 }
 
 // ---------------------------------------------------------- trim-1 ---
@@ -170,9 +174,11 @@ __device__ __forceinline__ bool trim1_keep_at(const uint32_t *__restrict__ Fc, c
 // ones find an empty list and only clear counters.  Removals go to
 // wave-private LDS queues (a sweep can remove millions of vertices).
 //
+// A removed vertex keeps its label: a live vertex's label is still the
+// v | LAB_FINAL the state was initialised with, which is its trim-1 label.
 // A sweep gives each wave 64-aligned chunks, so the wave owns its chunk's
-// class and label words and (BITS) its two live-bitmap words: removal is
-// plain stores.  A list pass may test one vertex from several lanes: the
+// class words and (BITS) its two live-bitmap words: removal is plain
+// stores.  A list pass may test one vertex from several lanes: the
 // first to stamp it with the pass's epoch tests it, removal is a CAS on its
 // class word and (BITS) an atomic clear of its live bit.
 constexpr uint32_t TRIM_WQ = 1024;
@@ -214,7 +220,6 @@ __global__ __launch_bounds__(256) void trim1_kernel(const uint32_t *__restrict__
                 const uint32_t fb = Fr[v], fe = Fr[v + 1], bb = Br[v], be = Br[v + 1];
                 if (!trim1_keep_at<BITS>(Fc, Bc, cls, live, c, fb, fe, bb, be)) {
                     cls[v] = DEAD;
-                    lab[v] = v | LAB_FINAL;
                     gone = true;
                 }
             }
@@ -254,7 +259,6 @@ __global__ __launch_bounds__(256) void trim1_kernel(const uint32_t *__restrict__
                     if (c != DEAD && stamp[w] != epoch && atomicExch(&stamp[w], epoch) != epoch &&
                         !trim1_keep_at<BITS>(Fc, Bc, cls, live, c, Fr[w], Fr[w + 1], Br[w], Br[w + 1]) &&
                         atomicCAS(&cls[w], c, DEAD) == c) {
-                        lab[w] = w | LAB_FINAL;
                         if (BITS) atomicAnd(&live[w >> 5], ~(1u << (w & 31)));
                         gone = true;
                     }
@@ -306,10 +310,7 @@ __global__ __launch_bounds__(256) void trim1_degree_kernel(const uint32_t *__res
         for (uint32_t h = 0; h < TRIM_DEG_CHUNKS; h++) {
             const bool alive = c[h] != DEAD;
             const bool gone = alive && (f0[h] == f1[h] || b0[h] == b1[h]);
-            if (gone) {
-                cls[v[h]] = DEAD;
-                lab[v[h]] = v[h] | LAB_FINAL;
-            }
+            if (gone) cls[v[h]] = DEAD;   // lab[v] already holds v | LAB_FINAL
             const uint64_t m = __ballot(alive && !gone);
             if ((lane() & 31) == 0 && v[h] <= V) live[v[h] >> 5] = lane() < 32 ? (uint32_t)m : (uint32_t)(m >> 32);
             removed += gone;

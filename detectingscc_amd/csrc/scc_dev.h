@@ -9,7 +9,8 @@
 //                   probe "live and in my class" is one compare.
 //   lab[v]  uint32  representative of v's SCC once finished (pivot id, own id
 //                   for trim-1, pair minimum for trim-2); canonicalised to the
-//                   minimum vertex id at the end.
+//                   minimum vertex id at the end.  A live vertex keeps the
+//                   v | LAB_FINAL it starts with, so trim-1 writes no label.
 //   aux[v]  uint32  scratch: colour during colouring, min id per label at the
 //                   end.
 //   fvis/bvis       one bit per vertex per direction; bits of finished
