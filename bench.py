@@ -144,6 +144,15 @@ def bench_serving(requests=400, rate=1000.0, budget_gib=32, slots=3):
     with scc.Context(0) as c:
         c.server_start(budget_bytes=budget_gib << 30, max_batch=256, slots=slots, solo_vertices=1 << 16,
                        lanes=16, block=1024)
+        # warm-up (untimed): the largest requests, a few times, so every
+        # slot's grow-only device buffers reach their size before the run
+        big = sorted(range(requests), key=lambda i: -graphs[i].E)[:slots]
+        for _ in range(slots):
+            warm_ids = [c.server_submit(graphs[i]) for i in big]
+            c.server_wait_all()
+            for rid in warm_ids:
+                c.server_poll(rid)
+        warm = c.server_stats()
         t = time.perf_counter()
         due = t
         ids = []
@@ -155,16 +164,18 @@ def bench_serving(requests=400, rate=1000.0, budget_gib=32, slots=3):
         c.server_wait_all()
         wall = time.perf_counter() - t
         st = c.server_stats()
-        for rid in ids:
-            c.server_poll(rid)
+        lat = np.array([c.server_poll(rid)[2] for rid in ids])
         c.server_stop()
-    log(f"served {st.completed} requests in {wall:.2f}s: p50 {st.p50_ms:.2f} ms, p99 {st.p99_ms:.2f} ms")
+    p50, p99 = (float(x) for x in np.percentile(lat, [50, 99]))
+    log(f"served {len(ids)} requests in {wall:.2f}s: p50 {p50:.2f} ms, p99 {p99:.2f} ms")
     return {
         "workload": f"{requests} requests, Poisson {rate:g}/s, V lognormal median 16384 sigma 1.5 cap 2^22, "
                     "degree 8, 50% ER / 50% R-MAT, seed 8",
         "arrival_rate_per_s": rate, "hbm_budget_bytes": budget_gib << 30, "slots": slots,
-        "graphs_per_s": st.graphs_per_s, "p50_ms": st.p50_ms, "p99_ms": st.p99_ms, "mean_ms": st.mean_ms,
-        "completed": st.completed, "batches": st.batches, "solo": st.solo, "peak_admitted_bytes": st.peak_bytes,
+        "graphs_per_s": len(ids) / wall, "p50_ms": p50, "p99_ms": p99, "mean_ms": float(lat.mean()),
+        "completed": st.completed - warm.completed, "batches": st.batches - warm.batches,
+        "solo": st.solo - warm.solo, "peak_admitted_bytes": st.peak_bytes,
+        "warmup": f"the {slots} largest requests submitted {slots} times before the timed arrivals",
     }
 
 

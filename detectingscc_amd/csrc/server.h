@@ -47,12 +47,14 @@ class Server {
     };
     void worker(int slot);
     bool can_admit_locked() const;
+    void take_locked(std::vector<Req *> &batch, uint64_t &taken);
 
     int dev;
     scc_batch_opts opts{};
     scc_opts solo_opts{};
     uint32_t solo_vertices = 1u << 16;
     uint64_t budget = 0, inflight = 0, peak = 0;
+    bool solo_busy = false;            // a slot runs a large request
     uint32_t max_batch = 64;
     std::mutex mu;
     std::condition_variable cv_work, cv_done;

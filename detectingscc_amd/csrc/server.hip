@@ -1,13 +1,16 @@
 // server.hip -- continuous-batching SCC request server.
 //
-// Requests wait in one FIFO.  Each slot thread owns a stream and a
-// BatchEngine (grow-only device buffers).  A free slot takes the longest
-// FIFO prefix that fits both `max_batch` and the HBM budget left by the
-// batches other slots have in flight, runs it as one persistent batched
-// launch, and publishes every request's labels.  Requests that arrive while
-// a batch runs join the next one, so the device never waits for a batch to
-// fill.  Admission is first-come first-served: the head of the queue is
-// never overtaken, which bounds the wait of a large request.
+// Requests wait in one queue in arrival order.  Each slot thread owns a
+// stream, a BatchEngine (grow-only device buffers) and, for large requests,
+// a device-wide Engine.  A free slot takes either the oldest large request
+// (V >= solo_vertices), if no other slot is running one, and runs it alone;
+// or the small requests in arrival order, skipping the large ones, up to
+// `max_batch` and the HBM budget left by what the other slots have in
+// flight, as one persistent batched launch.  Small requests thus never wait
+// behind large ones for a slot, while large requests keep their arrival
+// order among themselves, one at a time.  Requests that arrive while a
+// batch runs join the next one, so the device never waits for a batch to
+// fill.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -89,19 +92,33 @@ scc_status Server::submit(const scc_host_graph *g, uint64_t *id)
     r->E = g->E;
     r->bytes = bytes;
     r->solo = solo;
-    r->Fr.assign(g->Fr, g->Fr + (size_t)g->V + 2);
-    r->Br.assign(g->Br, g->Br + (size_t)g->V + 2);
-    r->Fc.assign(g->Fc, g->Fc + g->E);
-    r->Bc.assign(g->Bc, g->Bc + g->E);
+    // a bad request must not fail the batch it would share: the copy checks
+    // as it goes (one pass, branch-free within a block, so the caller's
+    // submit costs about one read and one write of the graph)
+    const size_t nr = (size_t)g->V + 2;
+    r->Fr.resize(nr);
+    r->Br.resize(nr);
+    r->Fc.resize(g->E);
+    r->Bc.resize(g->E);
+    uint32_t down = 0;
+    for (size_t v = 0; v < nr; v++) {
+        const uint32_t f = g->Fr[v], b = g->Br[v];
+        r->Fr[v] = f;
+        r->Br[v] = b;
+        if (v) down |= (f < g->Fr[v - 1]) | (b < g->Br[v - 1]);
+    }
     if (r->Fr[0] || r->Fr[1] || r->Br[0] || r->Br[1] || r->Fr[g->V + 1] != g->E || r->Br[g->V + 1] != g->E)
         return fail(SCC_ERR_INVALID, "row arrays are not in the reference layout");
-    // a bad request must not fail the batch it would share
-    for (uint32_t v = 0; v <= g->V; v++)
-        if (r->Fr[v] > r->Fr[v + 1] || r->Br[v] > r->Br[v + 1])
-            return fail(SCC_ERR_INVALID, "row offsets decrease at vertex %u", v);
-    for (uint32_t k = 0; k < g->E; k++)
-        if (r->Fc[k] < 1 || r->Fc[k] > g->V || r->Bc[k] < 1 || r->Bc[k] > g->V)
-            return fail(SCC_ERR_INVALID, "arc %u names a vertex outside 1..%u", k, g->V);
+    if (down) return fail(SCC_ERR_INVALID, "row offsets decrease");
+    uint32_t bad = 0;
+    const uint32_t hi = g->V - 1;   // column c is valid iff c - 1 <= V - 1 (unsigned)
+    for (uint32_t k = 0; k < g->E; k++) {
+        const uint32_t f = g->Fc[k], b = g->Bc[k];
+        r->Fc[k] = f;
+        r->Bc[k] = b;
+        bad |= (uint32_t)(f - 1u > hi) | (uint32_t)(b - 1u > hi);
+    }
+    if (bad) return fail(SCC_ERR_INVALID, "an arc names a vertex outside 1..%u", g->V);
     std::lock_guard<std::mutex> lk(mu);
     r->id = next_id++;
     r->arrival = clock::now();
@@ -114,9 +131,50 @@ scc_status Server::submit(const scc_host_graph *g, uint64_t *id)
     return SCC_OK;
 }
 
+// Whether take_locked would take anything: the oldest large request (if no
+// slot runs one) or the oldest small request fits in the budget left.
 bool Server::can_admit_locked() const
 {
-    return !pending.empty() && inflight + reqs.at(pending.front())->bytes <= budget;
+    bool large_seen = false, small_seen = false;
+    for (uint64_t id : pending) {
+        const Req *r = reqs.at(id).get();
+        if (r->solo ? (large_seen || solo_busy) : small_seen) continue;
+        if (inflight + r->bytes <= budget) return true;
+        (r->solo ? large_seen : small_seen) = true;
+        if (small_seen && (large_seen || solo_busy)) break;
+    }
+    return false;
+}
+
+// What a free slot takes (see the file comment).  The first small request
+// that does not fit ends the batch, so small requests keep their order.
+void Server::take_locked(std::vector<Req *> &batch, uint64_t &taken)
+{
+    if (!solo_busy) {
+        for (auto it = pending.begin(); it != pending.end(); ++it) {
+            Req *r = reqs.at(*it).get();
+            if (!r->solo) continue;
+            if (inflight + r->bytes <= budget) {
+                batch.push_back(r);
+                taken = r->bytes;
+                pending.erase(it);
+                solo_busy = true;
+                return;
+            }
+            break;
+        }
+    }
+    for (auto it = pending.begin(); it != pending.end() && batch.size() < max_batch;) {
+        Req *r = reqs.at(*it).get();
+        if (r->solo) {
+            ++it;
+            continue;
+        }
+        if (inflight + taken + r->bytes > budget) break;
+        taken += r->bytes;
+        batch.push_back(r);
+        it = pending.erase(it);
+    }
 }
 
 void Server::worker(int slot)
@@ -140,17 +198,8 @@ void Server::worker(int slot)
             // on stop the queue is drained first
             cv_work.wait(lk, [&] { return (stopping && pending.empty()) || can_admit_locked(); });
             if (pending.empty()) break;
-            // a large request runs alone on the device-wide engine; small ones
-            // are packed up to the next large one
-            while (!pending.empty() && batch.size() < max_batch) {
-                Req *r = reqs.at(pending.front()).get();
-                if (inflight + taken + r->bytes > budget) break;
-                if (r->solo && !batch.empty()) break;
-                taken += r->bytes;
-                batch.push_back(r);
-                pending.pop_front();
-                if (r->solo) break;
-            }
+            take_locked(batch, taken);
+            if (batch.empty()) continue;
             inflight += taken;
             peak = std::max(peak, inflight);
         }
@@ -232,7 +281,10 @@ void Server::worker(int slot)
             }
             last_done = now;
             batches++;
-            if (G == 1 && batch[0]->solo) solo_runs++;
+            if (G == 1 && batch[0]->solo) {
+                solo_runs++;
+                solo_busy = false;
+            }
             inflight -= taken;
         }
         cv_done.notify_all();
