@@ -292,15 +292,16 @@ scc_status scc_batch_labels(scc_ctx *ctx, uint32_t *lab, uint32_t *nscc);
 /* ----------------------------------------------------------- serving --- */
 /*
  * Continuous-batching request server (no reference counterpart).  Requests
- * (graphs, copied at submit) are admitted first-come first-served while the
- * device bytes of admitted requests stay within `hbm_budget`; each of
- * `slots` worker threads packs up to `max_batch` admitted requests into one
- * persistent batched launch on its own stream, so batches overlap on the
- * GPU and requests that arrive meanwhile join the next free slot.  A
- * request with V >= `solo_vertices` (0: 65536) runs alone on that slot's
- * device-wide engine instead of inside one workgroup.  Latency = submit ->
- * labels on the host.  A request larger than the whole budget is rejected
- * with SCC_ERR_BUDGET.
+ * (graphs, copied and checked at submit) are admitted while the device
+ * bytes of admitted requests stay within `hbm_budget`; each of `slots`
+ * worker threads packs up to `max_batch` small requests, in arrival order,
+ * into one persistent batched launch on its own stream, so batches overlap
+ * on the GPU and requests that arrive meanwhile join the next free slot.  A
+ * request with V >= `solo_vertices` (0: 65536) is large: it runs alone on a
+ * slot's device-wide engine instead of inside one workgroup, one large
+ * request at a time in arrival order, and small requests may overtake it.
+ * Latency = submit -> labels on the host.  A request larger than the whole
+ * budget is rejected with SCC_ERR_BUDGET.
  */
 typedef struct {
     uint64_t submitted, completed, rejected, failed, batches;
