@@ -484,12 +484,19 @@ template <int BS> __device__ uint32_t g_canon(Lds &L, const G1 &g)
 // larger graphs use their slots in global memory.  The views are generic
 // pointers, so the phase code is the same either way.
 constexpr uint32_t BVIS_LDS_WORDS = 2 * ((65536 + 64) / 64);
+// Class words of graphs of up to 2^15 vertices live in LDS as well (128 KB:
+// with the bitmaps about 145 KB of the 160 KB a workgroup may hold; the
+// kernel's registers allow one 1,024-lane workgroup per CU anyway), so the
+// neighbour-class tests of the trims, searches and colouring stay on the
+// CU.  Nothing outside the graph's pipeline reads them.
+constexpr uint32_t BCLS_LDS_WORDS = (1u << 15) + 2;
 
 template <int BS>
 __global__ __launch_bounds__(BS) void batch_scc_kernel(BatchDev B, uint32_t first, uint32_t count)
 {
     __shared__ Lds L;
     __shared__ uint32_t s_vis[2][BVIS_LDS_WORDS];
+    __shared__ uint32_t s_cls[BCLS_LDS_WORDS];
     for (;;) {
         __syncthreads();
         if (threadIdx.x == 0) L.graph = atomicAdd(B.next, 1u);
@@ -504,7 +511,7 @@ __global__ __launch_bounds__(BS) void batch_scc_kernel(BatchDev B, uint32_t firs
         g.Br = B.Br + vo;
         g.Fc = B.Fc + eo;
         g.Bc = B.Bc + eo;
-        g.cls = B.cls + vo;
+        g.cls = g.V + 1 <= BCLS_LDS_WORDS ? s_cls : B.cls + vo;
         g.lab = B.lab + vo;
         g.aux = B.aux + vo;
         // g_init_vis writes 2 * ((V + 64) / 64) words of each bitmap
