@@ -43,7 +43,9 @@ class Engine {
   private:
     void release();
     scc_status reserve(uint32_t V, uint32_t E);
-    scc_status read_ctr();
+    scc_status read_ctr();                 // publish_ctr + wait_ctr
+    uint32_t publish_ctr();                // queue a publication of the counters, returns its number
+    scc_status wait_ctr(uint32_t seq);     // wait for publication `seq`, copy it into h_ctr
     void zero_ctr(int first, int n);
     void mark(int phase);
     void collect_times(scc_stats *s);
@@ -72,7 +74,7 @@ class Engine {
 
     uint32_t capV = 0, capE = 0;
     uint32_t *h_ctr = nullptr;        // pinned mirror of st.ctr
-    uint32_t *h_pub = nullptr;        // mapped coherent host memory the GPU publishes counters into
+    uint32_t *h_pub = nullptr;        // mapped coherent host memory the GPU publishes counters into (2 buffers)
     uint32_t *d_pub = nullptr;        // its device address
     uint32_t pub_seq = 0;             // last publication requested
     uint8_t *d_tmp8 = nullptr;        // V+1 bytes for phase inputs/outputs

@@ -90,9 +90,9 @@ scc_status Engine::init(int dev)
     // counters published into mapped, coherent host memory: C_COUNT lines
     // plus one line for the sequence word (without it, read-backs copy and
     // synchronise the stream)
-    if (hipHostMalloc((void **)&h_pub, CT(C_COUNT + 1) * sizeof(uint32_t),
+    if (hipHostMalloc((void **)&h_pub, 2 * CT(C_COUNT + 1) * sizeof(uint32_t),
                       hipHostMallocMapped | hipHostMallocCoherent) == hipSuccess) {
-        std::memset(h_pub, 0, CT(C_COUNT + 1) * sizeof(uint32_t));
+        std::memset(h_pub, 0, 2 * CT(C_COUNT + 1) * sizeof(uint32_t));
         if (hipHostGetDevicePointer((void **)&d_pub, h_pub, 0) != hipSuccess) {
             (void)hipGetLastError();
             (void)hipHostFree(h_pub);
@@ -295,25 +295,34 @@ __global__ void publish_ctr_kernel(const uint32_t *ctr, uint32_t *pub, uint32_t 
     if (i == 0) __hip_atomic_store(pub + CT(C_COUNT), seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
-// The host waits for the counters of everything queued so far.  With the
-// published mirror it spins on the sequence word (a few microseconds after
-// the last kernel) instead of a copy plus a stream synchronisation (about
-// 25 us); after kPublishWaitMs it synchronises the stream anyway, which also
-// reports any device error.
+// The host reads the counters of everything queued before a publication.
+// With the published mirror it spins on the sequence word (a few
+// microseconds after the last kernel) instead of a copy plus a stream
+// synchronisation (about 25 us); after kPublishWaitMs it synchronises the
+// stream anyway, which also reports any device error.  The mirror has two
+// buffers, publication k going to buffer k & 1, so one publication may be
+// queued behind the one the host waits for (pipelined search windows): a
+// caller waits for publication k before it queues k + 2.
 static constexpr double kPublishWaitMs = 2000.0;
+static constexpr size_t kPubBuffer = CT(C_COUNT + 1);   // words of one buffer, sequence word last
 
-scc_status Engine::read_ctr()
+uint32_t Engine::publish_ctr()
+{
+    const uint32_t seq = ++pub_seq;
+    if (h_pub) publish_ctr_kernel<<<1, 64, 0, stream>>>(st.ctr, d_pub + (seq & 1) * kPubBuffer, seq);
+    return seq;
+}
+
+scc_status Engine::wait_ctr(uint32_t seq)
 {
     if (!h_pub) {
         HIP_TRY(hipMemcpyAsync(h_ctr, st.ctr, CT(C_COUNT) * sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
         HIP_TRY(hipStreamSynchronize(stream));
         return SCC_OK;
     }
-    if (++pub_seq == 0) pub_seq = 1;
-    const uint32_t seq = pub_seq;
-    publish_ctr_kernel<<<1, 64, 0, stream>>>(st.ctr, d_pub, seq);
     HIP_TRY(hipGetLastError());
-    const uint32_t *flag = h_pub + CT(C_COUNT);
+    const uint32_t *buf = h_pub + (seq & 1) * kPubBuffer;
+    const uint32_t *flag = buf + CT(C_COUNT);
     const auto t0 = std::chrono::steady_clock::now();
     for (uint64_t spin = 0; __atomic_load_n(flag, __ATOMIC_ACQUIRE) != seq; spin++) {
         if ((spin & 1023) == 1023 &&
@@ -326,9 +335,11 @@ scc_status Engine::read_ctr()
         __builtin_ia32_pause();
     }
     for (int i = 0; i < C_COUNT; i++)
-        for (int w = 0; w < kPubWords; w++) h_ctr[CT(i) + w] = h_pub[CT(i) + w];
+        for (int w = 0; w < kPubWords; w++) h_ctr[CT(i) + w] = buf[CT(i) + w];
     return SCC_OK;
 }
+
+scc_status Engine::read_ctr() { return wait_ctr(publish_ctr()); }
 
 // Zero counters [first, first+n), both words of each (64-bit counters too).
 __global__ void zero_ctr_kernel(uint32_t *ctr, uint32_t first, uint32_t n)
@@ -447,13 +458,17 @@ scc_status Engine::reach(bool fw, bool bw, bool single, int expand, int W, uint3
 // the host queues a window of levels per look at the counters: kDoWindow
 // levels on the full grid (kFirstWindow for a single-class search, the
 // first of a decomposition: power-law graphs finish it within that many
-// levels, which saves the GPU a wait for the host in the middle), or
-// kSpecLevels levels on a grid sized for a few
-// times the current frontier while every direction is top-down, has no hubs
-// pending, grew at most kSpecRise-fold since the previous look and would
-// have to grow kSpecGrowth-fold more to reach the bottom-up switch point
-// (high-diameter graphs run thousands of small levels).  Levels queued past
-// the end of the search are no-ops.
+// levels), or kSpecLevels levels on a grid sized for a few times the
+// current frontier while every direction is top-down, has no hubs pending,
+// grew at most kSpecRise-fold since the previous look and would have to
+// grow kSpecGrowth-fold more to reach the bottom-up switch point
+// (high-diameter graphs run thousands of small levels; such a window may
+// not switch to bottom-up).  From the second look on the windows are
+// pipelined: the next one is queued before the host reads the counters of
+// the last, so the GPU does not wait for the host (its parameters come from
+// the look before); a search that ends in its first window pays nothing
+// for that.
+// Levels queued past the end of the search are no-ops.
 static constexpr int kDoWindow = 4;
 static constexpr int kFirstWindow = 8;
 static constexpr int kSpecLevels = 16;
@@ -477,35 +492,49 @@ scc_status Engine::reach_do(bool fw, bool bw, bool single, uint32_t *levels_out)
         a.PR[i] = dirs[i] ? g.Fr : g.Br;
         a.PC[i] = dirs[i] ? g.Fc : g.Bc;
     }
-    int64_t nf_prev[2] = {0, 0};
-    uint32_t level = 0, blocks = full;
-    int window = single ? kFirstWindow : kDoWindow;
-    for (;;) {
+    uint32_t level = 0;
+    auto queue = [&](int window, uint32_t blocks) {
+        a.bu_ok = blocks == full;
         for (int k = 0; k < window; k++, level++) {
             for (int i = 0; i < nd; i++) a.d[i] = dir_args(dirs[i], level);
             a.level = (int)level;
             k_do_level(stream, a, single, blocks);
         }
-        scc_status s = read_ctr();
+    };
+    queue(single ? kFirstWindow : kDoWindow, full);
+    uint32_t seen = level, seq = publish_ctr();
+    int window = 0;   // nothing is queued ahead of the first look
+    uint32_t blocks = full;
+    int64_t nf_prev[2] = {0, 0};
+    for (;;) {
+        const bool ahead = window > 0;
+        uint32_t seen_next = seen, seq_next = seq;
+        if (ahead) {
+            queue(window, blocks);
+            seen_next = level;
+            seq_next = publish_ctr();
+        }
+        scc_status s = wait_ctr(seq);
         if (s != SCC_OK) return s;
-        // the state the next level would compute: levels 0..level-1 ran
+        // the state level `seen` would compute: levels 0..seen-1 ran
         const int64_t nlive = h_ctr[CT(C_NLIVE)];
         bool any = false, spec = true;
         int64_t nf_max = 0;
         uint32_t end = 0;
         for (int i = 0; i < nd; i++) {
             const int d = dirs[i];
-            const uint32_t last = level - 1;
+            const uint32_t last = seen - 1;
             const DirState prev = load_dir_state(h_ctr, last == 0 ? 0 : last & 1, i);
-            const uint32_t nf = h_ctr[CT((d ? C_BWCNT : C_FWCNT) + level % 3)];
+            const uint32_t nf = h_ctr[CT((d ? C_BWCNT : C_FWCNT) + seen % 3)];
             const uint32_t hubs = (uint32_t)(*reinterpret_cast<const unsigned long long *>(
                                                  h_ctr + CT((d ? C_BIGB : C_BIGF) + last % 3)) >> 32);
-            const DirState next = dir_decide(prev, nf, hubs, (uint32_t)nlive, level);
+            const DirState next = dir_decide(prev, nf, hubs, (uint32_t)nlive, seen, true);
             if (trace_levels)
                 std::fprintf(stderr, "[scc reach] after level %u: %s %s %u (hubs %u), visited %llu\n", last,
                              d ? "bw" : "fw",
-                             next.mode == DIR_OFF ? "done" : next.mode == DIR_BU ? "BU" : "TD", nf,
-                             prev.mode == DIR_BU ? 0u : hubs, (unsigned long long)next.visited);
+                             next.mode == DIR_OFF ? "done" : next.mode == DIR_BU ? "BU" : "TD",
+                             prev.mode == DIR_OFF ? 0u : nf, prev.mode == DIR_TD ? hubs : 0u,
+                             (unsigned long long)next.visited);
             if (next.mode == DIR_OFF) {
                 end = std::max(end, next.end);
                 continue;
@@ -519,13 +548,22 @@ scc_status Engine::reach_do(bool fw, bool bw, bool single, uint32_t *levels_out)
             nf_max = std::max<int64_t>(nf_max, nf);
         }
         if (!any) {
+            // a window queued meanwhile runs as no-ops; its counters are
+            // published into the other buffer, which nothing waits for
             if (levels_out) *levels_out += end;
             return SCC_OK;
         }
         // every level claims at least one vertex, so V+1 levels bound a search
-        if (level > limit()) return no_convergence("reachability");
+        if (seen > limit()) return no_convergence("reachability");
         window = spec ? kSpecLevels : kDoWindow;
         blocks = spec ? (uint32_t)std::min<int64_t>(full, std::max<int64_t>(32, (nf_max * 4 + 255) / 256)) : full;
+        if (!ahead) {   // after the first look: queue now, ahead from then on
+            queue(window, blocks);
+            seen_next = level;
+            seq_next = publish_ctr();
+        }
+        seen = seen_next;
+        seq = seq_next;
     }
 }
 

@@ -412,7 +412,8 @@ __global__ __launch_bounds__(256) void expand_do_kernel(DoArgs a, uint32_t bpd)
     const int L = a.level;
     DirState st = load_dir_state(a.ctr, L == 0 ? 0 : (L - 1) & 1, dir);
     if (L > 0) {
-        st = dir_decide(st, D.cnt[CT(L % 3)], (uint32_t)(*D.pbigc >> 32), a.ctr[CT(C_NLIVE)], (uint32_t)L);
+        st = dir_decide(st, D.cnt[CT(L % 3)], (uint32_t)(*D.pbigc >> 32), a.ctr[CT(C_NLIVE)], (uint32_t)L,
+                        a.bu_ok != 0);
         if (bid == 0 && threadIdx.x == 0) store_dir_state(a.ctr, L & 1, dir, st);
     }
     if (st.mode == DIR_TD)

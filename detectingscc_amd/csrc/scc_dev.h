@@ -120,6 +120,7 @@ struct DoArgs {
     uint32_t V;
     int ndirs;
     int level;
+    int bu_ok;                       // 0 on a grid too small for a bottom-up pass: stay top-down
 };
 
 // Direction-optimising switch points (Beamer et al.): go bottom-up when the
@@ -150,9 +151,10 @@ constexpr int kDirStateWords = 6;
 
 // The state of level L from that of level L-1, given the vertices level L-1
 // queued (nf), the hubs it deferred to level L (top-down only: a bottom-up
-// pass defers nothing) and the live vertices of the search.
+// pass defers nothing), the live vertices of the search and whether level
+// L may run bottom-up.
 __host__ __device__ __forceinline__ DirState dir_decide(DirState s, uint32_t nf, uint32_t hubs, uint32_t nlive,
-                                                        uint32_t level)
+                                                        uint32_t level, bool bu_ok)
 {
     if (s.mode == DIR_OFF) return s;
     if (s.mode == DIR_BU) hubs = 0;
@@ -165,7 +167,7 @@ __host__ __device__ __forceinline__ DirState dir_decide(DirState s, uint32_t nf,
     s.nf = nf;
     s.visited += nf;
     const int64_t remaining = (int64_t)nlive - (int64_t)s.visited;
-    if (s.mode == DIR_TD && (int64_t)nf * (grew ? kToBottomUpBurst : kToBottomUp) > remaining &&
+    if (s.mode == DIR_TD && bu_ok && (int64_t)nf * (grew ? kToBottomUpBurst : kToBottomUp) > remaining &&
         (int64_t)nf * kBottomUpFloor >= (int64_t)nlive)
         s.mode = DIR_BU;
     else if (s.mode == DIR_BU && (int64_t)nf * kToTopDown < (int64_t)nlive) s.mode = DIR_TD;
