@@ -102,11 +102,4 @@ __device__ __forceinline__ void wave_flush_block(const uint32_t *wq, uint32_t nq
     for (uint32_t i = lane; i < nq; i += 64) gout[off + i] = wq[i];
 }
 
-#define DECLARE_LDS_QUEUE(q)                                                                 \
-    __shared__ uint32_t q##_buf[QBUF];                                                       \
-    __shared__ uint32_t q##_n, q##_base;                                                     \
-    if (threadIdx.x == 0) q##_n = 0;                                                         \
-    __syncthreads();                                                                         \
-    const LdsQueue q{q##_buf, &q##_n, &q##_base}
-
 } // namespace sccamd
