@@ -260,10 +260,13 @@ static void launch_lb(hipStream_t s, const LevelArgs &a, bool single, uint32_t b
 // boundaries make every write visible.  The pass pushes the vertices it
 // visits, so a following top-down level can continue from them.
 //
-// Two tiers: each lane probes the first BU_SHORT parents of its vertex with
-// independent loads; vertices with longer parent lists and no hit yet are
-// listed in LDS and scanned by groups of BU_GROUP lanes, BU_GROUP parents
-// per step, with a ballot to stop each group at its first hit.
+// Two tiers: each lane probes the first BU_SHORT parents of its vertex;
+// vertices with longer parent lists and no hit yet are listed in LDS and
+// scanned by groups of BU_GROUP lanes, BU_GROUP parents per step, with a
+// ballot to stop each group at its first hit.  One parent up front and
+// groups of four were fastest on R-MAT scales 22 and 24 (probing more
+// parents before knowing whether the first one hits reads lines that are
+// mostly not needed).
 //
 // The pass is wave-centric: each wave streams its own 64-vertex chunks,
 // two at a time (each lane carries one vertex of each, so twice as many
@@ -278,8 +281,8 @@ static void launch_lb(hipStream_t s, const LevelArgs &a, bool single, uint32_t b
 // wave ORs its hits into each with one plain store (device-wide atomics
 // run at about 10 G/s on MI355X, which would bound a pass that visits
 // millions of vertices).
-constexpr uint32_t BU_SHORT = 4;
-constexpr uint32_t BU_GROUP = 8;
+constexpr uint32_t BU_SHORT = 1;
+constexpr uint32_t BU_GROUP = 4;
 constexpr uint32_t BU_WQ = 1024;
 constexpr uint32_t BU_CHUNKS = 2;   // 64-vertex chunks per wave iteration
 
