@@ -87,7 +87,7 @@ __global__ __launch_bounds__(256) void expand_vw_kernel(LevelArgs a, uint32_t bp
 // the frontier one level late, which leaves every reachable set unchanged
 // and makes a level a single launch.
 constexpr uint32_t LB_TILE = 256;
-constexpr uint32_t LB_CHUNK = 2048;
+constexpr uint32_t LB_CHUNK = 512;
 constexpr uint32_t LB_SLICES = 16;   // most workgroups sharing one tile
 constexpr uint32_t LB_DIRECT_N = 8192;   // frontier and mean degree of claims
 constexpr uint32_t LB_DIRECT_DEG = 4;    // without a visited-word read first
