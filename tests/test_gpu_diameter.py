@@ -34,8 +34,30 @@ def _ladder(n):
     return scc.from_arcs(2 * n, src, dst)
 
 
+def _barbell(na, nb, pairs, seed):
+    """Dense core A -> a chain of `pairs` 2-cycles -> denser core B.  The
+    backward search from B's hub is wide, then one pair wide for `pairs`
+    levels, then wide again in A, so the search switches direction on both
+    sides of a long run of narrow levels."""
+    rng = np.random.default_rng(seed)
+
+    def core(base, n, deg):
+        s = rng.integers(0, n, size=n * deg)
+        t = rng.integers(0, n, size=n * deg)
+        ring = np.arange(n)
+        return np.r_[s, ring] + base, np.r_[t, (ring + 1) % n] + base
+
+    sa, ta = core(1, na, 8)
+    sb, tb = core(1 + na + 2 * pairs, nb, 16)
+    a = 1 + na + 2 * np.arange(pairs)       # first vertex of each pair
+    src = np.r_[sa, sb, a, a + 1, a[:-1], [na], [a[-1]]]
+    dst = np.r_[ta, tb, a + 1, a, a[1:], [a[0]], [1 + na + 2 * pairs]]
+    return scc.from_arcs(na + 2 * pairs + nb, src, dst)
+
+
 GRAPHS = {
     "grid256": lambda: scc.gen_grid(256, 0.7, seed=4096),
+    "barbell": lambda: _barbell(20000, 30000, 3000, seed=11),
     "grid512": lambda: scc.gen_grid(512, 0.75, seed=7),
     "cycle20000": lambda: _cycle(20000),
     "ladder": lambda: _ladder(6000),
@@ -55,12 +77,12 @@ def test_high_diameter_matches_tarjan(ctx, name):
         assert st.num_sccs == n, (name, algo)
 
 
-@pytest.mark.parametrize("name", ["grid512", "ladder", "rmat14"])
+@pytest.mark.parametrize("name", ["grid512", "ladder", "rmat14", "barbell"])
 def test_high_diameter_reach_sets(ctx, name):
     g = GRAPHS[name]()
     ctx.upload(g)
     rng = np.random.default_rng(5)
-    srcs = rng.integers(1, g.V + 1, size=3)
+    srcs = np.r_[rng.integers(1, g.V + 1, size=3), g.V]
     dead = (np.arange(g.V + 1) % 13 == 5).astype(np.uint8)
     for src in srcs:
         dead[src] = 0
