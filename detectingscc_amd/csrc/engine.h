@@ -54,7 +54,8 @@ class Engine {
     scc_status reach_do(bool fw, bool bw, bool single, uint32_t *levels_out);
     DirArgs dir_args(int d, uint32_t level) const;
     // single: every live vertex has class 0 (the first trim of a
-    // decomposition); none_dead: besides, no vertex is dead yet
+    // decomposition); none_dead: besides, the state is fresh (no vertex dead,
+    // all in class 0, labels v | LAB_FINAL; its per-vertex words may be unwritten)
     scc_status trim1(bool fixpoint, bool single = false, bool none_dead = false);
     void init_vis(bool elect_first = false);
     uint64_t limit() const;

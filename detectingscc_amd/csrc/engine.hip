@@ -716,10 +716,13 @@ scc_status Engine::run(const scc_opts &o, scc_stats *out)
     nmarks = 0;
     first_reach_marks[0] = first_reach_marks[1] = -1;
     mark(-1);
-    k_init_state(stream, g, st, nullptr);   // also zeroes the counters
+    const bool hong = (A == 'g' || A == 'x');
+    // the first trim-1 pass of Hong and detectSCC starts from the fresh
+    // state and writes the per-vertex words itself
+    const bool fresh_trim = o.trim1 && (hong || A == 'd');
+    k_init_state(stream, g, st, nullptr, !fresh_trim);   // also zeroes the counters
     scc_status s;
     uint32_t live = g.V;
-    const bool hong = (A == 'g' || A == 'x');
     if (A == 'd') {
         if ((s = detect_phases(o, loop_expand, W, &stats, false)) != SCC_OK) return s;
     } else if (hong) {

@@ -94,9 +94,11 @@ void k_fill_u32(hipStream_t s, uint32_t *p, uint32_t val, uint64_t n)
 }
 
 // All vertices start live in class 0 (range 0, scc_WCC.cu:630-631) unless
-// `dead` marks them finished; lab[v] = v.
+// `dead` marks them finished; lab[v] = v.  With vertices == 0 only the
+// counters and the dummy vertex 0 are written: the first trim-1 pass of a
+// fresh state writes every other vertex's words itself.
 __global__ void init_state_kernel(uint32_t *cls, uint32_t *lab, uint64_t *piv, const uint8_t *dead,
-                                  uint32_t *ctr, uint32_t V)
+                                  uint32_t *ctr, uint32_t V, int vertices)
 {
     uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
     if (tid == 0) {
@@ -108,15 +110,17 @@ __global__ void init_state_kernel(uint32_t *cls, uint32_t *lab, uint64_t *piv, c
         ctr[CT(tid)] = 0;
         ctr[CT(tid) + 1] = 0;
     }
-    for (uint32_t v = tid + 1; v <= V; v += gridDim.x * blockDim.x) {
+    for (uint32_t v = tid + 1; vertices && v <= V; v += gridDim.x * blockDim.x) {
         cls[v] = (dead && dead[v]) ? DEAD : 0u;
         lab[v] = v | LAB_FINAL;
     }
 }
 
-void k_init_state(hipStream_t s, const DevGraph &g, DevState &st, const uint8_t *dead_dev)
+void k_init_state(hipStream_t s, const DevGraph &g, DevState &st, const uint8_t *dead_dev, bool vertices)
 {
-    init_state_kernel<<<sweep_blocks(g.V), SWEEP_BLOCK, 0, s>>>(st.cls, st.lab, st.piv, dead_dev, st.ctr, g.V);
+    static_assert(C_COUNT <= SWEEP_BLOCK, "one workgroup clears the counters");
+    init_state_kernel<<<vertices ? sweep_blocks(g.V) : 1u, SWEEP_BLOCK, 0, s>>>(st.cls, st.lab, st.piv, dead_dev,
+                                                                               st.ctr, g.V, vertices ? 1 : 0);
 }
 
 // ---------------------------------------------------------- trim-1 ---
@@ -273,11 +277,14 @@ __global__ __launch_bounds__(256) void trim1_kernel(const uint32_t *__restrict__
     wave_add32(&ctr[CT(C_TRIM1)], removed);
 }
 
-// Pass 0 of a single-class fixpoint from a state with no dead vertex, fused
-// with building the live bitmap: a vertex without an in- or out-arc is
-// removed (listed in `out`), every other live vertex gets its bit.  Pure
-// streaming of the class words and row offsets, so each wave keeps
-// TRIM_DEG_CHUNKS 64-vertex chunks' loads in flight.
+// Pass 0 of a single-class fixpoint from a fresh state (no dead vertex,
+// every vertex in class 0 with label v | LAB_FINAL), fused with writing that
+// state and building the live bitmap: a vertex without an in- or out-arc is
+// removed (class DEAD, listed in `out`), every other vertex gets class 0 and
+// its live bit, and every vertex gets label v | LAB_FINAL, so the state
+// initialisation before it may skip the per-vertex words.  Pure streaming
+// of the row offsets, so each wave keeps TRIM_DEG_CHUNKS 64-vertex chunks'
+// loads in flight.
 constexpr uint32_t TRIM_DEG_CHUNKS = 4;
 
 __global__ __launch_bounds__(256) void trim1_degree_kernel(const uint32_t *__restrict__ Fr,
@@ -294,13 +301,12 @@ __global__ __launch_bounds__(256) void trim1_degree_kernel(const uint32_t *__res
     const uint32_t stride = gridDim.x * 256;
     for (uint32_t base = (blockIdx.x * 4 + wv) * 64; base <= V; base += TRIM_DEG_CHUNKS * stride) {
         if (nq + TRIM_DEG_CHUNKS * 64 > TRIM_WQ) wave_flush(wq, nq, cnt_out, out);
-        uint32_t v[TRIM_DEG_CHUNKS], c[TRIM_DEG_CHUNKS], f0[TRIM_DEG_CHUNKS], f1[TRIM_DEG_CHUNKS],
-            b0[TRIM_DEG_CHUNKS], b1[TRIM_DEG_CHUNKS];
+        uint32_t v[TRIM_DEG_CHUNKS], f0[TRIM_DEG_CHUNKS], f1[TRIM_DEG_CHUNKS], b0[TRIM_DEG_CHUNKS],
+            b1[TRIM_DEG_CHUNKS];
 #pragma unroll
         for (uint32_t h = 0; h < TRIM_DEG_CHUNKS; h++) {
-            v[h] = base + h * stride + lane();   // vertex 0, the dummy, is dead
+            v[h] = base + h * stride + lane();
             const bool in = v[h] <= V;
-            c[h] = in ? cls[v[h]] : DEAD;
             f0[h] = in ? Fr[v[h]] : 0u;
             f1[h] = in ? Fr[v[h] + 1] : 0u;
             b0[h] = in ? Br[v[h]] : 0u;
@@ -308,9 +314,12 @@ __global__ __launch_bounds__(256) void trim1_degree_kernel(const uint32_t *__res
         }
 #pragma unroll
         for (uint32_t h = 0; h < TRIM_DEG_CHUNKS; h++) {
-            const bool alive = c[h] != DEAD;
+            const bool alive = v[h] >= 1 && v[h] <= V;   // vertex 0, the dummy, is dead
             const bool gone = alive && (f0[h] == f1[h] || b0[h] == b1[h]);
-            if (gone) cls[v[h]] = DEAD;   // lab[v] already holds v | LAB_FINAL
+            if (alive) {
+                cls[v[h]] = gone ? DEAD : 0u;
+                lab[v[h]] = v[h] | LAB_FINAL;
+            }
             const uint64_t m = __ballot(alive && !gone);
             if ((lane() & 31) == 0 && v[h] <= V) live[v[h] >> 5] = lane() < 32 ? (uint32_t)m : (uint32_t)(m >> 32);
             removed += gone;
@@ -946,45 +955,62 @@ __global__ void canon_reset_kernel(uint32_t *aux, uint32_t *cnt, uint32_t V)
 // instead of one per vertex.
 constexpr int MATCH_ROUNDS = 8;
 
+// Each lane takes CANON_CHUNKS vertices per step, one per 64-vertex chunk
+// of its wave, with all their label loads in flight together (then, in the
+// apply pass, all their representative loads): one load per step leaves
+// these passes waiting on memory latency.
+constexpr uint32_t CANON_CHUNKS = 4;
+
 // Like the size counts below, each wave keeps a running minimum for one
 // label several of its lanes shared and writes it once at the end.
-__global__ void canon_min_kernel(uint32_t *lab, uint32_t *aux, uint32_t *ctr, uint32_t V)
+__global__ __launch_bounds__(256) void canon_min_kernel(uint32_t *lab, uint32_t *aux, uint32_t *ctr, uint32_t V)
 {
     uint32_t heavy = DEAD, hmin = DEAD;
-    WAVE_LOOP(v, valid, V)
-    {
-        uint32_t r = valid ? lab[v] : DEAD;
-        const bool final_lab = valid && (r & LAB_FINAL);
-        const uint32_t rr = r & ~LAB_FINAL;
-        if (valid && (rr < 1 || rr > V || (final_lab && rr > v))) {
-            // every vertex must hold a representative (a final one is <= v)
-            atomicAdd(&ctr[CT(C_ANOM)], 1u);
-            lab[v] = r = v | LAB_FINAL;
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t base = blockIdx.x * blockDim.x + (threadIdx.x & ~63u); base < V; base += CANON_CHUNKS * stride) {
+        uint32_t vs[CANON_CHUNKS], rs[CANON_CHUNKS];
+#pragma unroll
+        for (uint32_t h = 0; h < CANON_CHUNKS; h++) {
+            vs[h] = base + h * stride + lane() + 1;
+            rs[h] = vs[h] <= V ? lab[vs[h]] : DEAD;
         }
-        bool pending = valid && !(r & LAB_FINAL);
-        for (int it = 0; it < MATCH_ROUNDS; it++) {
-            const uint64_t pm = __ballot(pending);
-            if (!pm) break;
-            const int L = __ffsll((unsigned long long)pm) - 1;
-            const uint32_t rl = __shfl(r, L);
-            const bool mine = pending && r == rl;
-            const uint64_t mm = __ballot(mine);
-            uint32_t m = mine ? v : DEAD;
-            for (int o = 32; o > 0; o >>= 1) {
-                uint32_t x = __shfl_xor(m, o);
-                m = x < m ? x : m;
+#pragma unroll
+        for (uint32_t h = 0; h < CANON_CHUNKS; h++) {
+            const uint32_t v = vs[h];
+            const bool valid = v <= V;
+            uint32_t r = rs[h];
+            const bool final_lab = valid && (r & LAB_FINAL);
+            const uint32_t rr = r & ~LAB_FINAL;
+            if (valid && (rr < 1 || rr > V || (final_lab && rr > v))) {
+                // every vertex must hold a representative (a final one is <= v)
+                atomicAdd(&ctr[CT(C_ANOM)], 1u);
+                lab[v] = r = v | LAB_FINAL;
             }
-            if (rl == heavy) {
-                hmin = m < hmin ? m : hmin;
-            } else if (heavy == DEAD && __popcll(mm) > 1) {
-                heavy = rl;
-                hmin = m;
-            } else if ((int)lane() == L && m < aux[rl]) {
-                atomicMin(&aux[rl], m);
+            bool pending = valid && !(r & LAB_FINAL);
+            for (int it = 0; it < MATCH_ROUNDS; it++) {
+                const uint64_t pm = __ballot(pending);
+                if (!pm) break;
+                const int L = __ffsll((unsigned long long)pm) - 1;
+                const uint32_t rl = __shfl(r, L);
+                const bool mine = pending && r == rl;
+                const uint64_t mm = __ballot(mine);
+                uint32_t m = mine ? v : DEAD;
+                for (int o = 32; o > 0; o >>= 1) {
+                    uint32_t x = __shfl_xor(m, o);
+                    m = x < m ? x : m;
+                }
+                if (rl == heavy) {
+                    hmin = m < hmin ? m : hmin;
+                } else if (heavy == DEAD && __popcll(mm) > 1) {
+                    heavy = rl;
+                    hmin = m;
+                } else if ((int)lane() == L && m < aux[rl]) {
+                    atomicMin(&aux[rl], m);
+                }
+                pending = pending && !mine;
             }
-            pending = pending && !mine;
+            if (pending && v < aux[r]) atomicMin(&aux[r], v);
         }
-        if (pending && v < aux[r]) atomicMin(&aux[r], v);
     }
     if (heavy != DEAD && lane() == 0 && hmin < aux[heavy]) atomicMin(&aux[heavy], hmin);
 }
@@ -992,38 +1018,51 @@ __global__ void canon_min_kernel(uint32_t *lab, uint32_t *aux, uint32_t *ctr, ui
 // With sizes, each wave also keeps a running count for one label that
 // several of its lanes shared (a giant SCC recurs in every wave), added to
 // memory once at the end instead of once per wave iteration.
-__global__ void canon_apply_kernel(uint32_t *lab, const uint32_t *__restrict__ aux, uint32_t *cnt,
-                                   uint32_t *ctr, uint32_t V, int sizes)
+__global__ __launch_bounds__(256) void canon_apply_kernel(uint32_t *lab, const uint32_t *__restrict__ aux,
+                                                          uint32_t *cnt, uint32_t *ctr, uint32_t V, int sizes)
 {
     uint32_t roots = 0, heavy = DEAD, hcount = 0;
-    WAVE_LOOP(v, valid, V)
-    {
-        uint32_t r = DEAD;
-        if (valid) {
-            const uint32_t l = lab[v];
-            r = (l & LAB_FINAL) ? (l & ~LAB_FINAL) : aux[l];
-            lab[v] = r;
-            roots += r == v;
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t base = blockIdx.x * blockDim.x + (threadIdx.x & ~63u); base < V; base += CANON_CHUNKS * stride) {
+        uint32_t vs[CANON_CHUNKS], rs[CANON_CHUNKS];
+#pragma unroll
+        for (uint32_t h = 0; h < CANON_CHUNKS; h++) {
+            vs[h] = base + h * stride + lane() + 1;
+            rs[h] = vs[h] <= V ? lab[vs[h]] : DEAD;
         }
-        if (!sizes) continue;
-        bool pending = valid;
-        for (int it = 0; it < MATCH_ROUNDS; it++) {
-            const uint64_t pm = __ballot(pending);
-            if (!pm) break;
-            const int L = __ffsll((unsigned long long)pm) - 1;
-            const uint32_t rl = __shfl(r, L);
-            const uint32_t n = (uint32_t)__popcll(__ballot(pending && r == rl));
-            if (rl == heavy) {
-                hcount += n;
-            } else if (heavy == DEAD && n > 1) {
-                heavy = rl;
-                hcount = n;
-            } else if ((int)lane() == L) {
-                atomicAdd(&cnt[rl], n);
+#pragma unroll
+        for (uint32_t h = 0; h < CANON_CHUNKS; h++) {
+            const uint32_t l = rs[h];
+            if (vs[h] <= V) rs[h] = (l & LAB_FINAL) ? (l & ~LAB_FINAL) : aux[l];
+        }
+#pragma unroll
+        for (uint32_t h = 0; h < CANON_CHUNKS; h++) {
+            const uint32_t v = vs[h], r = rs[h];
+            const bool valid = v <= V;
+            if (valid) {
+                lab[v] = r;
+                roots += r == v;
             }
-            pending = pending && r != rl;
+            if (!sizes) continue;
+            bool pending = valid;
+            for (int it = 0; it < MATCH_ROUNDS; it++) {
+                const uint64_t pm = __ballot(pending);
+                if (!pm) break;
+                const int L = __ffsll((unsigned long long)pm) - 1;
+                const uint32_t rl = __shfl(r, L);
+                const uint32_t n = (uint32_t)__popcll(__ballot(pending && r == rl));
+                if (rl == heavy) {
+                    hcount += n;
+                } else if (heavy == DEAD && n > 1) {
+                    heavy = rl;
+                    hcount = n;
+                } else if ((int)lane() == L) {
+                    atomicAdd(&cnt[rl], n);
+                }
+                pending = pending && r != rl;
+            }
+            if (pending) atomicAdd(&cnt[r], 1u);
         }
-        if (pending) atomicAdd(&cnt[r], 1u);
     }
     if (hcount && lane() == 0) atomicAdd(&cnt[heavy], hcount);
     wave_add32(&ctr[CT(C_NSCC)], roots);

@@ -228,7 +228,8 @@ __host__ __forceinline__ uint32_t dir_blocks(uint32_t bpd, int ndirs)
 
 // ---- sweep kernels (kernels_sweep.hip) ----
 void k_fill_u32(hipStream_t s, uint32_t *p, uint32_t val, uint64_t n);
-void k_init_state(hipStream_t s, const DevGraph &g, DevState &st, const uint8_t *dead_dev);
+void k_init_state(hipStream_t s, const DevGraph &g, DevState &st, const uint8_t *dead_dev,
+                  bool vertices = true);
 // trim-1 pass `pass` of a fixpoint: pass 0 (and a pass whose input list,
 // counted at C_TLIST + pass%3, holds more than V / sweep_share vertices)
 // sweeps every vertex, the others test the live neighbours of the vertices
@@ -237,8 +238,10 @@ void k_init_state(hipStream_t s, const DevGraph &g, DevState &st, const uint8_t 
 // (st.stamp never holds it).  With `live` (only while every live vertex has
 // class 0), neighbour tests read a live bitmap there instead of classes;
 // pass 0 builds it from the class words.  With degree_first (a fixpoint
-// from a state where no vertex is dead yet) pass 0 removes only vertices
-// without an in- or out-arc: every neighbour is live then.
+// from a fresh state: no vertex dead, all in class 0, labels v | LAB_FINAL)
+// pass 0 removes only vertices without an in- or out-arc, since every
+// neighbour is live then, and writes every vertex's class word and label
+// itself, so the fresh state's per-vertex words need not be written before.
 void k_trim1(hipStream_t s, const DevGraph &g, DevState &st, const uint32_t *in, uint32_t *out,
              uint32_t epoch, uint32_t pass, uint32_t sweep_share, uint32_t *live, bool degree_first);
 void k_trim2(hipStream_t s, const DevGraph &g, DevState &st);
