@@ -84,9 +84,9 @@ scc_status Engine::init(int dev)
     device = dev;
     HIP_TRY(hipSetDevice(dev));
     HIP_TRY(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
-    HIP_TRY(hipHostMalloc((void **)&h_ctr, CT(C_COUNT) * sizeof(uint32_t), hipHostMallocDefault));
-    HIP_TRY(hipMalloc((void **)&st.ctr, CT(C_COUNT) * sizeof(uint32_t)));
-    HIP_TRY(hipMemset(st.ctr, 0, CT(C_COUNT) * sizeof(uint32_t)));
+    HIP_TRY(hipHostMalloc((void **)&h_ctr, CT(kCtrLines) * sizeof(uint32_t), hipHostMallocDefault));
+    HIP_TRY(hipMalloc((void **)&st.ctr, CT(kCtrLines) * sizeof(uint32_t)));
+    HIP_TRY(hipMemset(st.ctr, 0, CT(kCtrLines) * sizeof(uint32_t)));
     // counters published into mapped, coherent host memory: C_COUNT lines
     // plus one line for the sequence word (without it, read-backs copy and
     // synchronise the stream)
@@ -149,8 +149,8 @@ scc_status Engine::reserve(uint32_t V, uint32_t E)
     A(dalloc(&st.cls, nv)); A(dalloc(&st.lab, nv)); A(dalloc(&st.aux, nv));
     A(dalloc(&st.fvis, 2 * nwords)); A(dalloc(&st.bvis, 2 * nwords));
     A(dalloc(&st.piv, 3 * nv + 3));
-    for (int i = 0; i < 2; i++) {
-        A(dalloc(&st.qf[i], nv)); A(dalloc(&st.qb[i], nv));
+    for (int i = 0; i < 2; i++) {   // the FW queues also hold the trim-1 lists
+        A(dalloc(&st.qf[i], std::max(nv, trim_list_words(nV)))); A(dalloc(&st.qb[i], nv));
     }
     // a hub list holds vertices of degree >= LB_BIG only
     const size_t nhub = std::min<size_t>(nv, (size_t)nE / LB_BIG + 1);
@@ -285,11 +285,18 @@ template <class T> scc_status Engine::scratch(DevBuf &b, size_t n, T **out)
 // words 0..1, the search state (DirState) words 0..kDirStateWords-1.
 constexpr int kPubWords = kDirStateWords;
 
+// A spread counter is published as its own word plus its slots.
 __global__ void publish_ctr_kernel(const uint32_t *ctr, uint32_t *pub, uint32_t seq)
 {
     const uint32_t i = threadIdx.x;
-    if (i < C_COUNT)
+    if (i < C_COUNT) {
         for (int w = 0; w < kPubWords; w++) pub[CT(i) + w] = ctr[CT(i) + w];
+        if (spread_index((int)i) >= 0) {
+            uint32_t t = ctr[CT(i)];
+            for (int j = 0; j < kSpread; j++) t += ctr[CSLOT((int)i, j)];
+            pub[CT(i)] = t;
+        }
+    }
     __threadfence_system();
     __syncthreads();
     if (i == 0) __hip_atomic_store(pub + CT(C_COUNT), seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
@@ -316,8 +323,11 @@ uint32_t Engine::publish_ctr()
 scc_status Engine::wait_ctr(uint32_t seq)
 {
     if (!h_pub) {
-        HIP_TRY(hipMemcpyAsync(h_ctr, st.ctr, CT(C_COUNT) * sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
+        HIP_TRY(hipMemcpyAsync(h_ctr, st.ctr, CT(kCtrLines) * sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
         HIP_TRY(hipStreamSynchronize(stream));
+        for (int i = 0; i < C_COUNT; i++)
+            if (spread_index(i) >= 0)
+                for (int j = 0; j < kSpread; j++) h_ctr[CT(i)] += h_ctr[CSLOT(i, j)];
         return SCC_OK;
     }
     HIP_TRY(hipGetLastError());
@@ -341,12 +351,16 @@ scc_status Engine::wait_ctr(uint32_t seq)
 
 scc_status Engine::read_ctr() { return wait_ctr(publish_ctr()); }
 
-// Zero counters [first, first+n), both words of each (64-bit counters too).
+// Zero counters [first, first+n), both words of each (64-bit counters too)
+// and the slots of spread counters.
 __global__ void zero_ctr_kernel(uint32_t *ctr, uint32_t first, uint32_t n)
 {
     if (threadIdx.x < n) {
-        ctr[CT(first + threadIdx.x)] = 0;
-        ctr[CT(first + threadIdx.x) + 1] = 0;
+        const int c = (int)(first + threadIdx.x);
+        ctr[CT(c)] = 0;
+        ctr[CT(c) + 1] = 0;
+        if (spread_index(c) >= 0)
+            for (int j = 0; j < kSpread; j++) ctr[CSLOT(c, j)] = 0;
     }
 }
 

@@ -193,7 +193,9 @@ __global__ void q_map_kernel(uint32_t *lab, const uint32_t *__restrict__ tbits,
         roots += l == v;
     }
     for (int o = 32; o > 0; o >>= 1) roots += __shfl_xor(roots, o);
-    if ((threadIdx.x & 63u) == 0 && roots) atomicAdd(&ctr[CT(C_NSCC)], roots);
+    // one add per wave, spread over the counter's slots
+    if ((threadIdx.x & 63u) == 0 && roots)
+        atomicAdd(&ctr[CSLOT(C_NSCC, (blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) % kSpread)], roots);
 }
 
 static uint32_t grid_for(uint64_t n)

@@ -4,6 +4,8 @@
 // consecutive vertices per step, so ballots and cross-lane reductions always
 // see a full wave.  Class and state updates that other lanes may race on use
 // atomics (the reference's byte read-modify-writes, scc.h:28-42, are not).
+#include <algorithm>
+
 #include "lds_queue.h"
 #include "scc_dev.h"
 
@@ -54,27 +56,30 @@ __device__ __forceinline__ uint64_t block_sum(uint64_t x)
     return t;   // valid in thread 0
 }
 
-__device__ __forceinline__ void wave_add32(uint32_t *p, uint32_t x)
-{
-    const uint64_t t = block_sum(x);
-    if (threadIdx.x == 0 && t) atomicAdd(p, (uint32_t)t);
-}
-
 __device__ __forceinline__ void wave_add64(uint32_t *p32, uint64_t x)
 {
     const uint64_t t = block_sum(x);
     if (threadIdx.x == 0 && t) atomicAdd(reinterpret_cast<unsigned long long *>(p32), (unsigned long long)t);
 }
 
+// Workgroup sum of x added to slot blockIdx.x % kSpread of spread counter
+// c: the workgroups of a sweep end together, and one address taking an
+// atomic from each of them would serialise the sweep's tail.
+__device__ __forceinline__ void spread_add32(uint32_t *ctr, int c, uint32_t x)
+{
+    const uint64_t t = block_sum(x);
+    if (threadIdx.x == 0 && t) atomicAdd(&ctr[CSLOT(c, blockIdx.x % kSpread)], (uint32_t)t);
+}
+
 uint32_t sweep_blocks(uint32_t V)
 {
     uint64_t b = ((uint64_t)V + SWEEP_BLOCK) / SWEEP_BLOCK;
-    // each workgroup ends with one atomic per counter, and same-address
-    // atomics serialise: 1,024 workgroups (half the chip's wave slots) up to
-    // 8M vertices, 2,048 above, where the sweeps' dependent loads need every
-    // wave slot (measured: R-MAT scale 24 gains 9%, scale 22 loses 3% with
-    // 2,048)
-    const uint64_t cap = V > (8u << 20) ? 2048 : 1024;
+    // 2,048 workgroups (every wave slot of the chip) above 1M vertices: the
+    // sweeps' dependent loads need them all (a plain streaming sweep runs at
+    // 3.3 TB/s with 1,024 and 5.2 TB/s with 2,048).  The workgroups' closing
+    // atomics go to spread counters, so more workgroups cost no serialised
+    // tail (measured: R-MAT scale 22 0.409 -> 0.389 ms per decomposition)
+    const uint64_t cap = V > (1u << 20) ? 2048 : 1024;
     return (uint32_t)(b < 1 ? 1 : (b > cap ? cap : b));
 }
 
@@ -106,9 +111,9 @@ __global__ void init_state_kernel(uint32_t *cls, uint32_t *lab, uint64_t *piv, c
         lab[0] = 0;
         piv[0] = 0;
     }
-    if (tid < C_COUNT) {   // both words of every counter (64-bit ones too)
-        ctr[CT(tid)] = 0;
-        ctr[CT(tid) + 1] = 0;
+    for (uint32_t i = tid; i < (uint32_t)kCtrLines; i += gridDim.x * blockDim.x) {
+        ctr[CT(i)] = 0;   // both words of every counter line (64-bit counters too)
+        ctr[CT(i) + 1] = 0;
     }
     for (uint32_t v = tid + 1; vertices && v <= V; v += gridDim.x * blockDim.x) {
         cls[v] = (dead && dead[v]) ? DEAD : 0u;
@@ -118,7 +123,6 @@ __global__ void init_state_kernel(uint32_t *cls, uint32_t *lab, uint64_t *piv, c
 
 void k_init_state(hipStream_t s, const DevGraph &g, DevState &st, const uint8_t *dead_dev, bool vertices)
 {
-    static_assert(C_COUNT <= SWEEP_BLOCK, "one workgroup clears the counters");
     init_state_kernel<<<vertices ? sweep_blocks(g.V) : 1u, SWEEP_BLOCK, 0, s>>>(st.cls, st.lab, st.piv, dead_dev,
                                                                                st.ctr, g.V, vertices ? 1 : 0);
 }
@@ -178,6 +182,14 @@ __device__ __forceinline__ bool trim1_keep_at(const uint32_t *__restrict__ Fc, c
 // ones find an empty list and only clear counters.  Removals go to
 // wave-private LDS queues (a sweep can remove millions of vertices).
 //
+// A list count is a spread counter: a sweep reserves list space from slot
+// blockIdx.x % kSpread and writes into that slot's region of the list (the
+// slot's workgroups own a fixed share of the 64-vertex chunks, which bounds
+// the region: trim_slot_cap), a list pass reserves from the counter's own
+// word and writes from index 0.  A pass only ever writes one way, and the
+// next pass reads the segments (own word first, then the slots) through
+// their prefix sums.
+//
 // A removed vertex keeps its label: a live vertex's label is still the
 // v | LAB_FINAL the state was initialised with, which is its trim-1 label.
 // A sweep gives each wave 64-aligned chunks, so the wave owns its chunk's
@@ -189,6 +201,61 @@ constexpr uint32_t TRIM_WQ = 1024;
 constexpr uint32_t TRIM_W = 8;
 constexpr uint32_t TRIM_PER_BLOCK = 256 / TRIM_W;
 
+// Largest share of a sweep's removals one slot can take: its workgroups
+// (every kSpread-th of sweep_blocks(V)) hold 4 waves each, and a wave owns
+// at most ceil((V + 1) / (256 * grid)) chunks of 64 vertices.
+uint32_t trim_slot_cap(uint32_t V)
+{
+    const uint64_t grid = sweep_blocks(V);
+    const uint64_t per_wave = ((uint64_t)V + 1 + 256 * grid - 1) / (256 * grid);
+    return (uint32_t)(((grid + kSpread - 1) / kSpread) * 256 * per_wave);
+}
+
+size_t trim_list_words(uint32_t V)
+{
+    const size_t slotted = (size_t)kSpread * trim_slot_cap(V);
+    return std::max<size_t>(slotted, (size_t)V + 1);
+}
+
+// Clears list count c (its own word and its slots); workgroup 0 calls it.
+__device__ __forceinline__ void clear_list_count(uint32_t *ctr, int c)
+{
+    if (threadIdx.x <= (uint32_t)kSpread) ctr[threadIdx.x == 0 ? CT(c) : CSLOT(c, threadIdx.x - 1)] = 0;
+}
+
+// pre[k] (k = 0..kSpread+1): entries of list count c in the segments before
+// segment k (segment 0: the counter's own word, segment j+1: slot j, at most
+// cap each).  Called by the whole workgroup; pre is in LDS.
+__device__ __forceinline__ void list_prefix(const uint32_t *ctr, int c, uint32_t cap, uint32_t *pre)
+{
+    if (threadIdx.x < 64) {
+        const uint32_t l = threadIdx.x;
+        uint32_t x = 0;
+        if (l == 0) x = ctr[CT(c)];
+        else if (l <= (uint32_t)kSpread) x = min(ctr[CSLOT(c, l - 1)], cap);
+        for (int o = 1; o < 64; o <<= 1) {
+            const uint32_t y = __shfl_up(x, o);
+            if ((int)l >= o) x += y;
+        }
+        if (l <= (uint32_t)kSpread) pre[l + 1] = x;
+        if (l == 0) pre[0] = 0;
+    }
+    __syncthreads();
+}
+
+// Position in the list of logical entry i < pre[kSpread + 1]: the last
+// segment k whose prefix is <= i holds it.
+__device__ __forceinline__ uint32_t list_pos(const uint32_t *pre, uint32_t i, uint32_t cap)
+{
+    uint32_t lo = 0, hi = kSpread;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi + 1) >> 1;
+        if (pre[mid] <= i) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo == 0 ? i : (lo - 1) * cap + (i - pre[lo]);
+}
+
 template <bool BITS>
 __global__ __launch_bounds__(256) void trim1_kernel(const uint32_t *__restrict__ Fr,
                                                     const uint32_t *__restrict__ Fc,
@@ -196,19 +263,26 @@ __global__ __launch_bounds__(256) void trim1_kernel(const uint32_t *__restrict__
                                                     const uint32_t *__restrict__ Bc, uint32_t *cls,
                                                     uint32_t *lab, uint32_t *live, uint32_t V, uint32_t *ctr,
                                                     const uint32_t *in, uint32_t *out, uint32_t *stamp,
-                                                    uint32_t epoch, uint32_t pass, uint32_t sweep_share)
+                                                    uint32_t epoch, uint32_t pass, uint32_t sweep_share,
+                                                    uint32_t cap)
 {
     __shared__ uint32_t s_q[4][TRIM_WQ];
     __shared__ uint32_t s_rounds;
+    __shared__ uint32_t s_pre[kSpread + 2];
     const uint32_t wv = threadIdx.x >> 6;
     uint32_t *wq = s_q[wv];
     uint32_t nq = 0, removed = 0;
-    const uint32_t n = pass ? ctr[CT(C_TLIST + pass % 3)] : 0u;
-    uint32_t *cnt_out = &ctr[CT(C_TLIST + (pass + 1) % 3)];
-    if (blockIdx.x == 0 && threadIdx.x == 0) ctr[CT(C_TLIST + (pass + 2) % 3)] = 0;
-    if (pass == 0 || n > V / sweep_share) {
+    const int c_in = C_TLIST + pass % 3, c_out = C_TLIST + (pass + 1) % 3;
+    if (pass) list_prefix(ctr, c_in, cap, s_pre);
+    const uint32_t n = pass ? s_pre[kSpread + 1] : 0u;
+    if (blockIdx.x == 0) clear_list_count(ctr, C_TLIST + (pass + 2) % 3);
+    const bool sweep = pass == 0 || n > V / sweep_share;
+    const uint32_t slot = blockIdx.x % kSpread;
+    uint32_t *cnt_out = sweep ? &ctr[CSLOT(c_out, slot)] : &ctr[CT(c_out)];
+    uint32_t *qout = sweep ? out + (size_t)slot * cap : out;
+    if (sweep) {
         for (uint32_t base = (blockIdx.x * 4 + wv) * 64; base <= V; base += gridDim.x * 256) {
-            if (nq + 64 > TRIM_WQ) wave_flush(wq, nq, cnt_out, out);
+            if (nq + 64 > TRIM_WQ) wave_flush(wq, nq, cnt_out, qout);
             const uint32_t v = base + lane();   // vertex 0, the dummy, is dead
             uint32_t word = 0, c = DEAD;
             if (v <= V) {
@@ -241,7 +315,7 @@ __global__ __launch_bounds__(256) void trim1_kernel(const uint32_t *__restrict__
             const uint32_t i = base + threadIdx.x / TRIM_W;
             uint32_t fb = 0, nf = 0, bb = 0, total = 0;
             if (i < n) {
-                const uint32_t v = in[i];
+                const uint32_t v = in[list_pos(s_pre, i, cap)];
                 fb = Fr[v];
                 nf = Fr[v + 1] - fb;
                 bb = Br[v];
@@ -253,7 +327,7 @@ __global__ __launch_bounds__(256) void trim1_kernel(const uint32_t *__restrict__
             __syncthreads();
             const uint32_t rounds = s_rounds;
             for (uint32_t r = 0; r < rounds; r++) {
-                if (nq + 64 > TRIM_WQ) wave_flush(wq, nq, cnt_out, out);
+                if (nq + 64 > TRIM_WQ) wave_flush(wq, nq, cnt_out, qout);
                 const uint32_t j = r * TRIM_W + sub;
                 bool gone = false;
                 uint32_t w = 0;
@@ -273,8 +347,8 @@ __global__ __launch_bounds__(256) void trim1_kernel(const uint32_t *__restrict__
             __syncthreads();   // s_rounds reuse
         }
     }
-    wave_flush_block(wq, nq, cnt_out, out);
-    wave_add32(&ctr[CT(C_TRIM1)], removed);
+    wave_flush_block(wq, nq, cnt_out, qout);
+    spread_add32(ctr, C_TRIM1, removed);
 }
 
 // Pass 0 of a single-class fixpoint from a fresh state (no dead vertex,
@@ -290,17 +364,19 @@ constexpr uint32_t TRIM_DEG_CHUNKS = 4;
 __global__ __launch_bounds__(256) void trim1_degree_kernel(const uint32_t *__restrict__ Fr,
                                                            const uint32_t *__restrict__ Br, uint32_t *cls,
                                                            uint32_t *lab, uint32_t *live, uint32_t V,
-                                                           uint32_t *ctr, uint32_t *out)
+                                                           uint32_t *ctr, uint32_t *out, uint32_t cap)
 {
     __shared__ uint32_t s_q[4][TRIM_WQ];
     const uint32_t wv = threadIdx.x >> 6;
     uint32_t *wq = s_q[wv];
     uint32_t nq = 0, removed = 0;
-    uint32_t *cnt_out = &ctr[CT(C_TLIST + 1)];
-    if (blockIdx.x == 0 && threadIdx.x == 0) ctr[CT(C_TLIST + 2)] = 0;
+    const uint32_t slot = blockIdx.x % kSpread;
+    uint32_t *cnt_out = &ctr[CSLOT(C_TLIST + 1, slot)];
+    uint32_t *qout = out + (size_t)slot * cap;
+    if (blockIdx.x == 0) clear_list_count(ctr, C_TLIST + 2);
     const uint32_t stride = gridDim.x * 256;
     for (uint32_t base = (blockIdx.x * 4 + wv) * 64; base <= V; base += TRIM_DEG_CHUNKS * stride) {
-        if (nq + TRIM_DEG_CHUNKS * 64 > TRIM_WQ) wave_flush(wq, nq, cnt_out, out);
+        if (nq + TRIM_DEG_CHUNKS * 64 > TRIM_WQ) wave_flush(wq, nq, cnt_out, qout);
         uint32_t v[TRIM_DEG_CHUNKS], f0[TRIM_DEG_CHUNKS], f1[TRIM_DEG_CHUNKS], b0[TRIM_DEG_CHUNKS],
             b1[TRIM_DEG_CHUNKS];
 #pragma unroll
@@ -326,8 +402,8 @@ __global__ __launch_bounds__(256) void trim1_degree_kernel(const uint32_t *__res
             wave_push(wq, nq, gone, v[h]);
         }
     }
-    wave_flush_block(wq, nq, cnt_out, out);
-    wave_add32(&ctr[CT(C_TRIM1)], removed);
+    wave_flush_block(wq, nq, cnt_out, qout);
+    spread_add32(ctr, C_TRIM1, removed);
 }
 
 // The live bitmap of the single-class trim: bit v set for every vertex
@@ -345,19 +421,20 @@ __global__ void live_bits_kernel(const uint32_t *__restrict__ cls, uint32_t *liv
 void k_trim1(hipStream_t s, const DevGraph &g, DevState &st, const uint32_t *in, uint32_t *out,
              uint32_t epoch, uint32_t pass, uint32_t sweep_share, uint32_t *live, bool degree_first)
 {
+    const uint32_t cap = trim_slot_cap(g.V);
     if (live && pass == 0 && degree_first) {
         trim1_degree_kernel<<<sweep_blocks(g.V), SWEEP_BLOCK, 0, s>>>(g.Fr, g.Br, st.cls, st.lab, live, g.V,
-                                                                      st.ctr, out);
+                                                                      st.ctr, out, cap);
     } else if (live) {
         if (pass == 0)
             live_bits_kernel<<<sweep_blocks(g.V), SWEEP_BLOCK, 0, s>>>(st.cls, live, g.V);
         trim1_kernel<true><<<sweep_blocks(g.V), SWEEP_BLOCK, 0, s>>>(g.Fr, g.Fc, g.Br, g.Bc, st.cls, st.lab,
                                                                      live, g.V, st.ctr, in, out, st.stamp,
-                                                                     epoch, pass, sweep_share);
+                                                                     epoch, pass, sweep_share, cap);
     } else {
         trim1_kernel<false><<<sweep_blocks(g.V), SWEEP_BLOCK, 0, s>>>(g.Fr, g.Fc, g.Br, g.Bc, st.cls, st.lab,
                                                                       nullptr, g.V, st.ctr, in, out,
-                                                                      st.stamp, epoch, pass, sweep_share);
+                                                                      st.stamp, epoch, pass, sweep_share, cap);
     }
 }
 
@@ -417,7 +494,7 @@ __global__ void trim2_kernel(const uint32_t *__restrict__ Fr, const uint32_t *__
             }
         }
     }
-    wave_add32(&ctr[CT(C_TRIM2)], pairs);
+    spread_add32(ctr, C_TRIM2, pairs);
 }
 
 void k_trim2(hipStream_t s, const DevGraph &g, DevState &st)
@@ -716,8 +793,8 @@ __global__ __launch_bounds__(256) void update_kernel(uint32_t *cls, uint32_t *la
             }
         }
     }
-    wave_add32(&ctr[CT(C_LIVE)], live);
-    wave_add32(&ctr[CT(C_PIVOT)], found);
+    spread_add32(ctr, C_LIVE, live);
+    spread_add32(ctr, C_PIVOT, found);
 }
 
 void k_update(hipStream_t s, const DevGraph &g, DevState &st)
@@ -811,7 +888,7 @@ __global__ void minlabel_prop_kernel(const uint32_t *__restrict__ Fr,
             changed = 1;
         }
     }
-    wave_add32(&ctr[CT(C_CHANGED)], changed);
+    spread_add32(ctr, C_CHANGED, changed);
 }
 
 void k_minlabel_prop(hipStream_t s, const DevGraph &g, DevState &st)
@@ -873,7 +950,7 @@ __global__ void color_prop_kernel(const uint32_t *__restrict__ Fr, const uint32_
             changed = 1;
         }
     }
-    wave_add32(&ctr[CT(C_CHANGED)], changed);
+    spread_add32(ctr, C_CHANGED, changed);
 }
 
 void k_color_prop(hipStream_t s, const DevGraph &g, DevState &st)
@@ -926,8 +1003,8 @@ __global__ void color_finish_kernel(uint32_t *cls, uint32_t *lab, const uint32_t
             live++;
         }
     }
-    wave_add32(&ctr[CT(C_LIVE)], live);
-    wave_add32(&ctr[CT(C_PIVOT)], found);
+    spread_add32(ctr, C_LIVE, live);
+    spread_add32(ctr, C_PIVOT, found);
 }
 
 void k_color_finish(hipStream_t s, const DevGraph &g, DevState &st)
@@ -1065,7 +1142,7 @@ __global__ __launch_bounds__(256) void canon_apply_kernel(uint32_t *lab, const u
         }
     }
     if (hcount && lane() == 0) atomicAdd(&cnt[heavy], hcount);
-    wave_add32(&ctr[CT(C_NSCC)], roots);
+    spread_add32(ctr, C_NSCC, roots);
 }
 
 __global__ void canon_largest_kernel(const uint32_t *__restrict__ cnt, uint32_t *ctr, uint32_t V)

@@ -71,6 +71,25 @@ enum : int {
     C_COUNT = 48
 };
 
+// Spread counters.  Same-address atomics serialise (about 12 ns each), so a
+// counter that every workgroup of a sweep adds to, or reserves list space
+// from, has kSpread slots besides its own word, each on a line of its own
+// after the C_COUNT counter lines: workgroup b uses slot b % kSpread.  Its
+// value is its own word plus its slots (the counter publication folds them
+// for the host).  The spread counters are C_CHANGED..C_NSCC and the trim-1
+// list counts C_TLIST..C_TLIST+2.
+constexpr int kSpread = 32;
+__host__ __device__ constexpr int spread_index(int c)
+{
+    return (c >= C_CHANGED && c <= C_NSCC) ? c - C_CHANGED
+           : (c >= C_TLIST && c < C_TLIST + 3) ? C_NSCC - C_CHANGED + 1 + (c - C_TLIST)
+                                                : -1;
+}
+constexpr int kSpreadCounters = C_NSCC - C_CHANGED + 1 + 3;
+constexpr int kCtrLines = C_COUNT + kSpreadCounters * kSpread;   // lines of the counters block
+// word of slot j of spread counter c
+__host__ __device__ constexpr int CSLOT(int c, int j) { return CT(C_COUNT + spread_index(c) * kSpread + j); }
+
 struct DevGraph {
     uint32_t V = 0, E = 0;
     uint32_t *Fr = nullptr, *Fc = nullptr, *Br = nullptr, *Bc = nullptr;
@@ -86,7 +105,7 @@ struct DevState {
     uint32_t *bigf[3] = {}, *bigb[3] = {};           // hub lists, rotating like the counts
     uint32_t *bigoff_f[3] = {}, *bigoff_b[3] = {};
     uint32_t *part = nullptr;      // partition ids (detectSCC)
-    uint32_t *ctr = nullptr;       // C_COUNT words
+    uint32_t *ctr = nullptr;       // kCtrLines lines (CT(kCtrLines) words)
     uint64_t *bred = nullptr;      // per-workgroup (live count, best key) of the bitmap initialisation
     uint32_t nwords = 0;           // words of a one-bit-per-vertex bitmap (fvis/bvis hold twice that)
 };
@@ -234,7 +253,11 @@ void k_init_state(hipStream_t s, const DevGraph &g, DevState &st, const uint8_t 
 // counted at C_TLIST + pass%3, holds more than V / sweep_share vertices)
 // sweeps every vertex, the others test the live neighbours of the vertices
 // listed in `in`; removals are listed in `out`, counted at
-// C_TLIST + (pass+1)%3.  `epoch` is new for every pass of the context
+// C_TLIST + (pass+1)%3.  A sweep lists workgroup b's removals in the region
+// of slot b % kSpread (slot j's region starts at j * trim_slot_cap(V)) and
+// counts them in that slot; a list pass lists from index 0 and counts in
+// the counter's own word.  `in` and `out` hold trim_list_words(V) words.
+// `epoch` is new for every pass of the context
 // (st.stamp never holds it).  With `live` (only while every live vertex has
 // class 0), neighbour tests read a live bitmap there instead of classes;
 // pass 0 builds it from the class words.  With degree_first (a fixpoint
@@ -244,6 +267,8 @@ void k_init_state(hipStream_t s, const DevGraph &g, DevState &st, const uint8_t 
 // itself, so the fresh state's per-vertex words need not be written before.
 void k_trim1(hipStream_t s, const DevGraph &g, DevState &st, const uint32_t *in, uint32_t *out,
              uint32_t epoch, uint32_t pass, uint32_t sweep_share, uint32_t *live, bool degree_first);
+uint32_t trim_slot_cap(uint32_t V);
+size_t trim_list_words(uint32_t V);
 void k_trim2(hipStream_t s, const DevGraph &g, DevState &st);
 // Visited bitmaps of a search and C_NLIVE; with elect_first (every live
 // vertex in class 0) also the first pivot, seeded in both directions.
