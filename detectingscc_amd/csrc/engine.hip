@@ -150,7 +150,7 @@ scc_status Engine::reserve(uint32_t V, uint32_t E)
     A(dalloc(&st.fvis, 2 * nwords)); A(dalloc(&st.bvis, 2 * nwords));
     A(dalloc(&st.piv, 3 * nv + 3));
     for (int i = 0; i < 2; i++) {   // the FW queues also hold the trim-1 lists
-        A(dalloc(&st.qf[i], std::max(nv, trim_list_words(nV)))); A(dalloc(&st.qb[i], nv));
+        A(dalloc(&st.qf[i], std::max(queue_words(nV), trim_list_words(nV)))); A(dalloc(&st.qb[i], queue_words(nV)));
     }
     // a hub list holds vertices of degree >= LB_BIG only
     const size_t nhub = std::min<size_t>(nv, (size_t)nE / LB_BIG + 1);
@@ -420,6 +420,7 @@ DirArgs Engine::dir_args(int d, uint32_t level) const
     D.qin = d ? st.qb[in] : st.qf[in];
     D.qout = d ? st.qb[out] : st.qf[out];
     D.cnt = st.ctr + CT(d ? C_BWCNT : C_FWCNT);
+    D.cslot = st.ctr + CSLOT(d ? C_BWCNT : C_FWCNT, 0);
     D.big = d ? st.bigb[cur] : st.bigf[cur];
     D.bigoff = d ? st.bigoff_b[cur] : st.bigoff_f[cur];
     D.bigc = reinterpret_cast<unsigned long long *>(st.ctr + CT(hub + cur));
@@ -502,6 +503,7 @@ scc_status Engine::reach_do(bool fw, bool bw, bool single, uint32_t *levels_out)
     a.ctr = st.ctr;
     a.V = g.V;
     a.ndirs = nd;
+    a.qcap = slot_cap(g.V, full);   // bottom-up levels run on the full grid only
     for (int i = 0; i < nd; i++) {
         a.PR[i] = dirs[i] ? g.Fr : g.Br;
         a.PC[i] = dirs[i] ? g.Fc : g.Bc;

@@ -14,6 +14,8 @@
 // kernels (LB, bottom-up) stage pushes in LDS and reserve queue space
 // with one global atomic per ~1,800 pushes; the reference-mapping kernels
 // (vertex, virtual warp) aggregate per wave.
+#include <algorithm>
+
 #include "lds_queue.h"
 #include "scc_dev.h"
 #include "scc_amd.h"
@@ -54,9 +56,12 @@ __global__ __launch_bounds__(256) void expand_vw_kernel(LevelArgs a, uint32_t bp
     split_dirs(a.ndirs, bpd, dir, bid);
     const DirArgs &D = a.d[dir];
     const int L = a.level;
+    // only bottom-up levels (direction-optimised searches) write the counts'
+    // slots, and a search starts with them clear: this mapping's counts are
+    // the counters' own words
     const uint32_t n = D.cnt[CT(L % 3)];
     uint32_t *cnt_out = &D.cnt[CT((L + 1) % 3)];
-    if (bid == 0 && threadIdx.x == 0) D.cnt[CT((L + 2) % 3)] = 0;
+    if (bid == 0) seg_clear(&D.cnt[CT((L + 2) % 3)], &D.cslot[CT(((L + 2) % 3) * kSpread)]);
     if (n == 0) return;
     const uint32_t g = (bid * blockDim.x + threadIdx.x) / W;
     const uint32_t ng = bpd * blockDim.x / W;
@@ -119,18 +124,21 @@ __device__ __forceinline__ uint32_t block_exscan256(uint32_t x, uint32_t *wsum, 
 constexpr uint32_t LB_LDS = QBUF + 2 + (LB_TILE + 1) + 3 * LB_TILE + 4;
 
 // One top-down level of direction D for workgroup `bid` of `bpd`; `lds`
-// holds LB_LDS words.  Called by the whole workgroup.
+// holds LB_LDS words.  The input queue holds n entries: from index 0, or,
+// after a bottom-up level (pre not null), in segments read through seg_pos
+// with the prefixes `pre` and slot regions of qcap.  Called by the whole
+// workgroup.
 template <bool SINGLE>
 __device__ __forceinline__ void lb_level(const DirArgs &D, const uint32_t *cls, int L, uint32_t bid,
-                                         uint32_t bpd, uint32_t *lds)
+                                         uint32_t bpd, uint32_t *lds, uint32_t n, const uint32_t *pre,
+                                         uint32_t qcap)
 {
-    const uint32_t n = D.cnt[CT(L % 3)];
     const unsigned long long packed = *D.pbigc;
     const uint32_t items = (uint32_t)(packed >> 32), chunks = (uint32_t)packed;
     uint32_t *cnt_out = &D.cnt[CT((L + 1) % 3)];
-    if (bid == 0 && threadIdx.x == 0) {
-        D.cnt[CT((L + 2) % 3)] = 0;
-        *D.bigc_zero = 0ull;
+    if (bid == 0) {
+        seg_clear(&D.cnt[CT((L + 2) % 3)], &D.cslot[CT(((L + 2) % 3) * kSpread)]);
+        if (threadIdx.x == 0) *D.bigc_zero = 0ull;
     }
     if (n == 0 && items == 0) return;
     const LdsQueue q{lds, lds + QBUF, lds + QBUF + 1};
@@ -180,7 +188,7 @@ __device__ __forceinline__ void lb_level(const DirArgs &D, const uint32_t *cls, 
         const uint32_t i = base + threadIdx.x;
         uint32_t v = 0, beg = 0, deg = 0, c = 0;
         if (i < n) {
-            v = D.qin[i];
+            v = D.qin[pre ? seg_pos(pre, i, qcap) : i];
             beg = R[v];
             deg = R[v + 1] - beg;
             if (!SINGLE) c = cls[v];
@@ -239,7 +247,9 @@ __global__ __launch_bounds__(256) void expand_lb_kernel(LevelArgs a, uint32_t bp
     int dir;
     uint32_t bid;
     split_dirs(a.ndirs, bpd, dir, bid);
-    lb_level<SINGLE>(a.d[dir], a.cls, a.level, bid, bpd, s_lds);
+    const DirArgs &D = a.d[dir];
+    // no bottom-up level runs in this mapping: the queues start at index 0
+    lb_level<SINGLE>(D, a.cls, a.level, bid, bpd, s_lds, D.cnt[CT(a.level % 3)], nullptr, 0u);
 }
 
 static void launch_lb(hipStream_t s, const LevelArgs &a, bool single, uint32_t bpd)
@@ -301,15 +311,23 @@ constexpr uint32_t BU_LDS = 4 * BU_WQ + 4 * BU_CHUNKS * 64 + 4 * BU_CHUNKS * 2;
 
 // One bottom-up level of direction D (parents in PR/PC) for workgroup `bid`
 // of `bpd`; `lds` holds BU_LDS words.  Called by the whole workgroup.
+//
+// The vertices a pass visits go to the region of slot bid % kSpread of the
+// output queue (slot_cap(V, bpd) = qcap entries: the slot's workgroups own
+// that many vertices at most) and are counted in that slot of the count, so
+// the flushes of the pass's workgroups are spread over kSpread addresses.
 template <bool SINGLE>
 __device__ __forceinline__ void bu_level(const DirArgs &D, const uint32_t *__restrict__ PR,
                                          const uint32_t *__restrict__ PC, const uint32_t *cls, uint32_t V,
-                                         int L, uint32_t bid, uint32_t bpd, uint32_t *lds)
+                                         int L, uint32_t bid, uint32_t bpd, uint32_t *lds, uint32_t qcap,
+                                         uint32_t *anom)
 {
-    uint32_t *cnt_out = &D.cnt[CT((L + 1) % 3)];
-    if (bid == 0 && threadIdx.x == 0) {
-        D.cnt[CT((L + 2) % 3)] = 0;
-        *D.bigc_zero = 0ull;
+    const uint32_t slot = bid % kSpread;
+    uint32_t *cnt_out = &D.cslot[CT(((L + 1) % 3) * kSpread + slot)];
+    uint32_t *qout = D.qout + (size_t)slot * qcap;
+    if (bid == 0) {
+        seg_clear(&D.cnt[CT((L + 2) % 3)], &D.cslot[CT(((L + 2) % 3) * kSpread)]);
+        if (threadIdx.x == 0) *D.bigc_zero = 0ull;
     }
     const uint32_t wv = threadIdx.x >> 6, lane = wlane();
     uint32_t *wq = lds + wv * BU_WQ;
@@ -320,7 +338,7 @@ __device__ __forceinline__ void bu_level(const DirArgs &D, const uint32_t *__res
     const uint32_t gshift = lane & ~(BU_GROUP - 1);
     const uint32_t stride = bpd * 256;   // one chunk of every wave of the grid
     for (uint32_t base = (bid * 4 + wv) * 64; base <= V; base += BU_CHUNKS * stride) {
-        if (nq + 2 * BU_CHUNKS * 64 > BU_WQ) wave_flush(wq, nq, cnt_out, D.qout);
+        if (nq + 2 * BU_CHUNKS * 64 > BU_WQ) wave_flush(wq, nq, cnt_out, qout, qcap, anom);
         uint32_t u[BU_CHUNKS], word[BU_CHUNKS], c[BU_CHUNKS], b[BU_CHUNKS], e[BU_CHUNKS];
         bool cand[BU_CHUNKS], found[BU_CHUNKS], need_long[BU_CHUNKS];
         uint64_t hits[BU_CHUNKS];
@@ -393,7 +411,7 @@ __device__ __forceinline__ void bu_level(const DirArgs &D, const uint32_t *__res
             if ((lane & 31) == 0 && m) D.vis[vword(u[h])] = word[h] | m;
         }
     }
-    wave_flush_block(wq, nq, cnt_out, D.qout);
+    wave_flush_block(wq, nq, cnt_out, qout, qcap, anom);
 }
 
 // ---------------------------------------------------------------------
@@ -408,21 +426,33 @@ template <bool SINGLE>
 __global__ __launch_bounds__(256) void expand_do_kernel(DoArgs a, uint32_t bpd)
 {
     __shared__ uint32_t s_lds[BU_LDS > LB_LDS ? BU_LDS : LB_LDS];
+    __shared__ uint32_t s_pre[kSpread + 2];
     int dir;
     uint32_t bid;
     split_dirs(a.ndirs, bpd, dir, bid);
     const DirArgs &D = a.d[dir];
     const int L = a.level;
     DirState st = load_dir_state(a.ctr, L == 0 ? 0 : (L - 1) & 1, dir);
+    // the input queue: after a bottom-up level (the mode level L-1 ran in)
+    // its count is the own word plus the slots, one load per lane of wave 0;
+    // after a top-down level or the seeds the own word alone, with no load
+    // added to the level's first dependent step
+    const bool slotted = L > 0 && st.mode == DIR_BU;
+    uint32_t n;
+    if (slotted) {
+        seg_prefix(&D.cnt[CT(L % 3)], &D.cslot[CT((L % 3) * kSpread)], a.qcap, s_pre);
+        n = s_pre[kSpread + 1];
+    } else {
+        n = D.cnt[CT(L % 3)];
+    }
     if (L > 0) {
-        st = dir_decide(st, D.cnt[CT(L % 3)], (uint32_t)(*D.pbigc >> 32), a.ctr[CT(C_NLIVE)], (uint32_t)L,
-                        a.bu_ok != 0);
+        st = dir_decide(st, n, (uint32_t)(*D.pbigc >> 32), a.ctr[CT(C_NLIVE)], (uint32_t)L, a.bu_ok != 0);
         if (bid == 0 && threadIdx.x == 0) store_dir_state(a.ctr, L & 1, dir, st);
     }
     if (st.mode == DIR_TD)
-        lb_level<SINGLE>(D, a.cls, L, bid, bpd, s_lds);
+        lb_level<SINGLE>(D, a.cls, L, bid, bpd, s_lds, n, slotted ? s_pre : nullptr, a.qcap);
     else if (st.mode == DIR_BU)
-        bu_level<SINGLE>(D, a.PR[dir], a.PC[dir], a.cls, a.V, L, bid, bpd, s_lds);
+        bu_level<SINGLE>(D, a.PR[dir], a.PC[dir], a.cls, a.V, L, bid, bpd, s_lds, a.qcap, a.ctr + CT(C_ANOM));
 }
 
 __global__ void do_begin_kernel(uint32_t *ctr, int ndirs)
@@ -442,6 +472,13 @@ uint32_t do_blocks(uint32_t V, int ndirs)
 {
     const uint64_t b = ((uint64_t)V + 256) / 256;
     return dir_blocks((uint32_t)(b > 2048 ? 2048 : b), ndirs);
+}
+
+size_t queue_words(uint32_t V)
+{
+    size_t w = (size_t)V + 1;
+    for (int nd = 1; nd <= 2; nd++) w = std::max(w, (size_t)kSpread * slot_cap(V, do_blocks(V, nd)));
+    return w;
 }
 
 void k_do_begin(hipStream_t s, uint32_t *ctr, int ndirs) { do_begin_kernel<<<1, 64, 0, s>>>(ctr, ndirs); }

@@ -204,56 +204,19 @@ constexpr uint32_t TRIM_PER_BLOCK = 256 / TRIM_W;
 // Largest share of a sweep's removals one slot can take: its workgroups
 // (every kSpread-th of sweep_blocks(V)) hold 4 waves each, and a wave owns
 // at most ceil((V + 1) / (256 * grid)) chunks of 64 vertices.
-uint32_t trim_slot_cap(uint32_t V)
+uint32_t slot_cap(uint32_t V, uint32_t grid_)
 {
-    const uint64_t grid = sweep_blocks(V);
+    const uint64_t grid = grid_ < 1 ? 1 : grid_;
     const uint64_t per_wave = ((uint64_t)V + 1 + 256 * grid - 1) / (256 * grid);
     return (uint32_t)(((grid + kSpread - 1) / kSpread) * 256 * per_wave);
 }
+
+uint32_t trim_slot_cap(uint32_t V) { return slot_cap(V, sweep_blocks(V)); }
 
 size_t trim_list_words(uint32_t V)
 {
     const size_t slotted = (size_t)kSpread * trim_slot_cap(V);
     return std::max<size_t>(slotted, (size_t)V + 1);
-}
-
-// Clears list count c (its own word and its slots); workgroup 0 calls it.
-__device__ __forceinline__ void clear_list_count(uint32_t *ctr, int c)
-{
-    if (threadIdx.x <= (uint32_t)kSpread) ctr[threadIdx.x == 0 ? CT(c) : CSLOT(c, threadIdx.x - 1)] = 0;
-}
-
-// pre[k] (k = 0..kSpread+1): entries of list count c in the segments before
-// segment k (segment 0: the counter's own word, segment j+1: slot j, at most
-// cap each).  Called by the whole workgroup; pre is in LDS.
-__device__ __forceinline__ void list_prefix(const uint32_t *ctr, int c, uint32_t cap, uint32_t *pre)
-{
-    if (threadIdx.x < 64) {
-        const uint32_t l = threadIdx.x;
-        uint32_t x = 0;
-        if (l == 0) x = ctr[CT(c)];
-        else if (l <= (uint32_t)kSpread) x = min(ctr[CSLOT(c, l - 1)], cap);
-        for (int o = 1; o < 64; o <<= 1) {
-            const uint32_t y = __shfl_up(x, o);
-            if ((int)l >= o) x += y;
-        }
-        if (l <= (uint32_t)kSpread) pre[l + 1] = x;
-        if (l == 0) pre[0] = 0;
-    }
-    __syncthreads();
-}
-
-// Position in the list of logical entry i < pre[kSpread + 1]: the last
-// segment k whose prefix is <= i holds it.
-__device__ __forceinline__ uint32_t list_pos(const uint32_t *pre, uint32_t i, uint32_t cap)
-{
-    uint32_t lo = 0, hi = kSpread;
-    while (lo < hi) {
-        const uint32_t mid = (lo + hi + 1) >> 1;
-        if (pre[mid] <= i) lo = mid;
-        else hi = mid - 1;
-    }
-    return lo == 0 ? i : (lo - 1) * cap + (i - pre[lo]);
 }
 
 template <bool BITS>
@@ -273,16 +236,18 @@ __global__ __launch_bounds__(256) void trim1_kernel(const uint32_t *__restrict__
     uint32_t *wq = s_q[wv];
     uint32_t nq = 0, removed = 0;
     const int c_in = C_TLIST + pass % 3, c_out = C_TLIST + (pass + 1) % 3;
-    if (pass) list_prefix(ctr, c_in, cap, s_pre);
+    if (pass) seg_prefix(&ctr[CT(c_in)], &ctr[CSLOT(c_in, 0)], cap, s_pre);
     const uint32_t n = pass ? s_pre[kSpread + 1] : 0u;
-    if (blockIdx.x == 0) clear_list_count(ctr, C_TLIST + (pass + 2) % 3);
+    if (blockIdx.x == 0) seg_clear(&ctr[CT(C_TLIST + (pass + 2) % 3)], &ctr[CSLOT(C_TLIST + (pass + 2) % 3, 0)]);
     const bool sweep = pass == 0 || n > V / sweep_share;
     const uint32_t slot = blockIdx.x % kSpread;
     uint32_t *cnt_out = sweep ? &ctr[CSLOT(c_out, slot)] : &ctr[CT(c_out)];
     uint32_t *qout = sweep ? out + (size_t)slot * cap : out;
+    const uint32_t qcap = sweep ? cap : ~0u;   // a list pass removes each vertex once: at most V entries
+    uint32_t *anom = &ctr[CT(C_ANOM)];
     if (sweep) {
         for (uint32_t base = (blockIdx.x * 4 + wv) * 64; base <= V; base += gridDim.x * 256) {
-            if (nq + 64 > TRIM_WQ) wave_flush(wq, nq, cnt_out, qout);
+            if (nq + 64 > TRIM_WQ) wave_flush(wq, nq, cnt_out, qout, qcap, anom);
             const uint32_t v = base + lane();   // vertex 0, the dummy, is dead
             uint32_t word = 0, c = DEAD;
             if (v <= V) {
@@ -315,7 +280,7 @@ __global__ __launch_bounds__(256) void trim1_kernel(const uint32_t *__restrict__
             const uint32_t i = base + threadIdx.x / TRIM_W;
             uint32_t fb = 0, nf = 0, bb = 0, total = 0;
             if (i < n) {
-                const uint32_t v = in[list_pos(s_pre, i, cap)];
+                const uint32_t v = in[seg_pos(s_pre, i, cap)];
                 fb = Fr[v];
                 nf = Fr[v + 1] - fb;
                 bb = Br[v];
@@ -327,7 +292,7 @@ __global__ __launch_bounds__(256) void trim1_kernel(const uint32_t *__restrict__
             __syncthreads();
             const uint32_t rounds = s_rounds;
             for (uint32_t r = 0; r < rounds; r++) {
-                if (nq + 64 > TRIM_WQ) wave_flush(wq, nq, cnt_out, qout);
+                if (nq + 64 > TRIM_WQ) wave_flush(wq, nq, cnt_out, qout, qcap, anom);
                 const uint32_t j = r * TRIM_W + sub;
                 bool gone = false;
                 uint32_t w = 0;
@@ -347,7 +312,7 @@ __global__ __launch_bounds__(256) void trim1_kernel(const uint32_t *__restrict__
             __syncthreads();   // s_rounds reuse
         }
     }
-    wave_flush_block(wq, nq, cnt_out, qout);
+    wave_flush_block(wq, nq, cnt_out, qout, qcap, anom);
     spread_add32(ctr, C_TRIM1, removed);
 }
 
@@ -373,10 +338,11 @@ __global__ __launch_bounds__(256) void trim1_degree_kernel(const uint32_t *__res
     const uint32_t slot = blockIdx.x % kSpread;
     uint32_t *cnt_out = &ctr[CSLOT(C_TLIST + 1, slot)];
     uint32_t *qout = out + (size_t)slot * cap;
-    if (blockIdx.x == 0) clear_list_count(ctr, C_TLIST + 2);
+    uint32_t *anom = &ctr[CT(C_ANOM)];
+    if (blockIdx.x == 0) seg_clear(&ctr[CT(C_TLIST + 2)], &ctr[CSLOT(C_TLIST + 2, 0)]);
     const uint32_t stride = gridDim.x * 256;
     for (uint32_t base = (blockIdx.x * 4 + wv) * 64; base <= V; base += TRIM_DEG_CHUNKS * stride) {
-        if (nq + TRIM_DEG_CHUNKS * 64 > TRIM_WQ) wave_flush(wq, nq, cnt_out, qout);
+        if (nq + TRIM_DEG_CHUNKS * 64 > TRIM_WQ) wave_flush(wq, nq, cnt_out, qout, cap, anom);
         uint32_t v[TRIM_DEG_CHUNKS], f0[TRIM_DEG_CHUNKS], f1[TRIM_DEG_CHUNKS], b0[TRIM_DEG_CHUNKS],
             b1[TRIM_DEG_CHUNKS];
 #pragma unroll
@@ -402,7 +368,7 @@ __global__ __launch_bounds__(256) void trim1_degree_kernel(const uint32_t *__res
             wave_push(wq, nq, gone, v[h]);
         }
     }
-    wave_flush_block(wq, nq, cnt_out, qout);
+    wave_flush_block(wq, nq, cnt_out, qout, cap, anom);
     spread_add32(ctr, C_TRIM1, removed);
 }
 
@@ -503,12 +469,15 @@ void k_trim2(hipStream_t s, const DevGraph &g, DevState &st)
                                                            g.V, st.ctr);
 }
 
+// A search starts with no hub pending and the slots of its queue counts
+// clear (bottom-up levels of an earlier search may have left them set).
 __device__ __forceinline__ void zero_hub_counts(uint32_t *ctr)
 {
     for (int i = 0; i < 3; i++) {
         *reinterpret_cast<unsigned long long *>(ctr + CT(C_BIGF + i)) = 0ull;
         *reinterpret_cast<unsigned long long *>(ctr + CT(C_BIGB + i)) = 0ull;
     }
+    for (int j = 0; j < 6 * kSpread; j++) ctr[CSLOT(C_FWCNT, j)] = 0;
 }
 
 // ------------------------------------------------- visited bitmaps ---

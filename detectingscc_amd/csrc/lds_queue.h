@@ -71,22 +71,31 @@ __device__ __forceinline__ void wave_push(uint32_t *wq, uint32_t &nq, bool p, ui
     nq += (uint32_t)__popcll(m);
 }
 
-__device__ __forceinline__ void wave_flush(uint32_t *wq, uint32_t &nq, uint32_t *gcnt, uint32_t *gout)
+// Entries at or past `cap` are not written, and a reservation that passes
+// it counts one in *anom (a caller whose bound says that cannot happen
+// passes its region's size; the region's reader clamps its count to cap).
+__device__ __forceinline__ void wave_flush(uint32_t *wq, uint32_t &nq, uint32_t *gcnt, uint32_t *gout,
+                                           uint32_t cap = ~0u, uint32_t *anom = nullptr)
 {
     if (!nq) return;
     const uint32_t lane = threadIdx.x & 63u;
     wave_sync();
     uint32_t b = 0;
-    if (lane == 0) b = atomicAdd(gcnt, nq);
+    if (lane == 0) {
+        b = atomicAdd(gcnt, nq);
+        if (anom && b + nq > cap) atomicAdd(anom, 1u);
+    }
     b = __shfl(b, 0);
-    for (uint32_t i = lane; i < nq; i += 64) gout[b + i] = wq[i];
+    for (uint32_t i = lane; i < nq; i += 64)
+        if (b + i < cap) gout[b + i] = wq[i];
     nq = 0;
     wave_sync();
 }
 
 // End of kernel: the four waves' remainders with one atomic (every thread
-// of the 256-lane workgroup calls).
-__device__ __forceinline__ void wave_flush_block(const uint32_t *wq, uint32_t nq, uint32_t *gcnt, uint32_t *gout)
+// of the 256-lane workgroup calls).  cap and anom as for wave_flush.
+__device__ __forceinline__ void wave_flush_block(const uint32_t *wq, uint32_t nq, uint32_t *gcnt, uint32_t *gout,
+                                                 uint32_t cap = ~0u, uint32_t *anom = nullptr)
 {
     __shared__ uint32_t s_n[4], s_base;
     const uint32_t lane = threadIdx.x & 63u, wv = threadIdx.x >> 6;
@@ -95,11 +104,13 @@ __device__ __forceinline__ void wave_flush_block(const uint32_t *wq, uint32_t nq
     if (threadIdx.x == 0) {
         const uint32_t tot = s_n[0] + s_n[1] + s_n[2] + s_n[3];
         s_base = tot ? atomicAdd(gcnt, tot) : 0u;
+        if (anom && tot && s_base + tot > cap) atomicAdd(anom, 1u);
     }
     __syncthreads();
     uint32_t off = s_base;
     for (uint32_t w = 0; w < wv; w++) off += s_n[w];
-    for (uint32_t i = lane; i < nq; i += 64) gout[off + i] = wq[i];
+    for (uint32_t i = lane; i < nq; i += 64)
+        if (off + i < cap) gout[off + i] = wq[i];
 }
 
 } // namespace sccamd

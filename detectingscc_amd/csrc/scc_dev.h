@@ -76,19 +76,71 @@ enum : int {
 // from, has kSpread slots besides its own word, each on a line of its own
 // after the C_COUNT counter lines: workgroup b uses slot b % kSpread.  Its
 // value is its own word plus its slots (the counter publication folds them
-// for the host).  The spread counters are C_CHANGED..C_NSCC and the trim-1
-// list counts C_TLIST..C_TLIST+2.
+// for the host).  The spread counters are C_CHANGED..C_NSCC, the trim-1
+// list counts C_TLIST..C_TLIST+2 and the search queue counts
+// C_FWCNT..C_BWCNT+2 (the slots of each direction's three counts are
+// consecutive: count k's slot j is CSLOT(first, k * kSpread + j)).
 constexpr int kSpread = 32;
 __host__ __device__ constexpr int spread_index(int c)
 {
     return (c >= C_CHANGED && c <= C_NSCC) ? c - C_CHANGED
            : (c >= C_TLIST && c < C_TLIST + 3) ? C_NSCC - C_CHANGED + 1 + (c - C_TLIST)
+           : (c >= C_FWCNT && c < C_BWCNT + 3) ? C_NSCC - C_CHANGED + 4 + (c - C_FWCNT)
                                                 : -1;
 }
-constexpr int kSpreadCounters = C_NSCC - C_CHANGED + 1 + 3;
+constexpr int kSpreadCounters = C_NSCC - C_CHANGED + 1 + 3 + 6;
 constexpr int kCtrLines = C_COUNT + kSpreadCounters * kSpread;   // lines of the counters block
 // word of slot j of spread counter c
 __host__ __device__ constexpr int CSLOT(int c, int j) { return CT(C_COUNT + spread_index(c) * kSpread + j); }
+
+// A list whose count is a spread counter has kSpread + 1 segments: segment
+// 0 from index 0 (entries reserved from the counter's own word), segment
+// j + 1 from index j * cap (entries reserved from slot j, at most cap).
+// One pass writes a list one way only (a trim-1 sweep or a bottom-up level
+// through the slots, a trim-1 list pass, a top-down level or the seeding
+// through the own word), so segment 0 and slot 0's region never both hold
+// entries.
+// seg_prefix writes pre[k] = the entries of the segments before k
+// (k = 0..kSpread+1, so pre[kSpread + 1] is the list's length), given the
+// counter's own word and its slot 0 (slot j is CT(j) words further on).
+// Called by the whole workgroup; pre is in LDS.
+__device__ __forceinline__ void seg_prefix(const uint32_t *own, const uint32_t *slot0, uint32_t cap, uint32_t *pre)
+{
+    if (threadIdx.x < 64) {
+        const uint32_t l = threadIdx.x;
+        uint32_t x = 0;
+        if (l == 0) x = *own;
+        else if (l <= (uint32_t)kSpread) x = min(slot0[CT((int)l - 1)], cap);
+        for (int o = 1; o < 64; o <<= 1) {
+            const uint32_t y = __shfl_up(x, o);
+            if ((int)l >= o) x += y;
+        }
+        if (l <= (uint32_t)kSpread) pre[l + 1] = x;
+        if (l == 0) pre[0] = 0;
+    }
+    __syncthreads();
+}
+
+// Index in the list of logical entry i < pre[kSpread + 1]: the last
+// segment k whose prefix is <= i holds it.
+__device__ __forceinline__ uint32_t seg_pos(const uint32_t *pre, uint32_t i, uint32_t cap)
+{
+    uint32_t lo = 0, hi = kSpread;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi + 1) >> 1;
+        if (pre[mid] <= i) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo == 0 ? i : (lo - 1) * cap + (i - pre[lo]);
+}
+
+// Clears a spread counter (its own word and slots); threads 0..kSpread of
+// one workgroup call it.
+__device__ __forceinline__ void seg_clear(uint32_t *own, uint32_t *slot0)
+{
+    if (threadIdx.x == 0) *own = 0;
+    else if (threadIdx.x <= (uint32_t)kSpread) slot0[CT((int)threadIdx.x - 1)] = 0;
+}
 
 struct DevGraph {
     uint32_t V = 0, E = 0;
@@ -116,7 +168,8 @@ struct DirArgs {
     uint32_t *vis;               // visited bitmap
     const uint32_t *qin;
     uint32_t *qout;
-    uint32_t *cnt;               // the 3 rotating counts
+    uint32_t *cnt;               // the 3 rotating counts (spread counters)
+    uint32_t *cslot;             // their slots: count k's slot j at cslot[CT(k * kSpread + j)]
     // LB mode defers each hub (degree >= LB_BIG) of level L to a list whose
     // chunks the level L+1 kernel expands before its own tiles, so a level
     // is one launch; three lists rotate like the counts
@@ -140,6 +193,7 @@ struct DoArgs {
     int ndirs;
     int level;
     int bu_ok;                       // 0 on a grid too small for a bottom-up pass: stay top-down
+    uint32_t qcap;                   // slot region of a bottom-up level's queue: slot_cap(V, full grid)
 };
 
 // Direction-optimising switch points (Beamer et al.): go bottom-up when the
@@ -267,6 +321,10 @@ void k_init_state(hipStream_t s, const DevGraph &g, DevState &st, const uint8_t 
 // itself, so the fresh state's per-vertex words need not be written before.
 void k_trim1(hipStream_t s, const DevGraph &g, DevState &st, const uint32_t *in, uint32_t *out,
              uint32_t epoch, uint32_t pass, uint32_t sweep_share, uint32_t *live, bool degree_first);
+// Largest number of vertices the workgroups b with b % kSpread == j own in
+// a pass over 64-vertex chunks where wave w of `grid` 4-wave workgroups
+// takes chunks w, w + 4 * grid, ...: the size of slot j's list region.
+uint32_t slot_cap(uint32_t V, uint32_t grid);
 uint32_t trim_slot_cap(uint32_t V);
 size_t trim_list_words(uint32_t V);
 void k_trim2(hipStream_t s, const DevGraph &g, DevState &st);
@@ -302,6 +360,8 @@ void k_expand_level(hipStream_t s, const LevelArgs &a, int expand, int warp, boo
 void k_do_begin(hipStream_t s, uint32_t *ctr, int ndirs);
 void k_do_level(hipStream_t s, const DoArgs &a, bool single_class, uint32_t blocks_per_dir);
 uint32_t do_blocks(uint32_t V, int ndirs);
+// Words of a search queue: bottom-up levels write kSpread slot regions.
+size_t queue_words(uint32_t V);
 
 uint32_t sweep_blocks(uint32_t V);
 
