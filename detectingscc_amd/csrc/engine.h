@@ -56,7 +56,8 @@ class Engine {
     // single: every live vertex has class 0 (the first trim of a
     // decomposition); none_dead: besides, the state is fresh (no vertex dead,
     // all in class 0, labels v | LAB_FINAL; its per-vertex words may be unwritten)
-    scc_status trim1(bool fixpoint, bool single = false, bool none_dead = false);
+    // wait = false: one queued batch of passes, no look at the counters
+    scc_status trim1(bool fixpoint, bool single = false, bool none_dead = false, bool wait = true);
     void init_vis(bool elect_first = false);
     uint64_t limit() const;
     scc_status no_convergence(const char *what);

@@ -593,7 +593,7 @@ scc_status Engine::reach_do(bool fw, bool bw, bool single, uint32_t *levels_out)
 static constexpr uint32_t kTrimSweepShare = 32;
 static constexpr uint32_t kTrimQueued = 4;
 
-scc_status Engine::trim1(bool fixpoint, bool single, bool none_dead)
+scc_status Engine::trim1(bool fixpoint, bool single, bool none_dead, bool wait)
 {
     mark(SCC_PHASE_TRIM1);
     zero_ctr(C_TLIST, 3);
@@ -602,6 +602,21 @@ scc_status Engine::trim1(bool fixpoint, bool single, bool none_dead)
     uint32_t *live = single ? st.fvis : nullptr;
     k_trim1(stream, g, st, st.qf[1], st.qf[0], 0u, 0, kTrimSweepShare, live, fixpoint && single && none_dead);
     uint32_t pass = 1;
+    if (!wait) {
+        // one batch of passes and no look at the counters: the SCCs do not
+        // depend on how far trimming got (what a fixpoint would still remove,
+        // the phases after the first search remove), and the GPU goes on to
+        // the next phase without waiting for the host
+        for (uint32_t k = 0; fixpoint && k < kTrimQueued; k++, pass++) {
+            if (++trim_epoch == 0) {
+                k_fill_u32(stream, st.stamp, 0u, (uint64_t)g.V + 1);
+                trim_epoch = 1;
+            }
+            k_trim1(stream, g, st, st.qf[(pass + 1) & 1], st.qf[pass & 1], trim_epoch, pass, kTrimSweepShare, live,
+                    true);
+        }
+        return SCC_OK;
+    }
     while (fixpoint) {
         for (uint32_t k = 0; k < kTrimQueued; k++, pass++) {
             if (++trim_epoch == 0) {   // wrapped: no stale stamp may match a new epoch
@@ -742,7 +757,7 @@ scc_status Engine::run(const scc_opts &o, scc_stats *out)
     if (A == 'd') {
         if ((s = detect_phases(o, loop_expand, W, &stats, false)) != SCC_OK) return s;
     } else if (hong) {
-        if (o.trim1 && (s = trim1(true, true, true)) != SCC_OK) return s;
+        if (o.trim1 && (s = trim1(true, true, true, false)) != SCC_OK) return s;
         if ((s = pivot_round(true, first_expand, W, &stats, &live)) != SCC_OK) return s;
         // with no live vertex left (a power-law graph's first search often
         // takes everything trim-1 left) every later phase removes nothing,
@@ -765,7 +780,7 @@ scc_status Engine::run(const scc_opts &o, scc_stats *out)
         }
     } else {
         // vSlota/wSlota: one trim-1 pass, -q ignored (scc_Coloring.cu:507-508)
-        if (o.trim1 && (s = trim1(false, true)) != SCC_OK) return s;
+        if (o.trim1 && (s = trim1(false, true, false, false)) != SCC_OK) return s;
         if ((s = pivot_round(true, first_expand, W, &stats, &live)) != SCC_OK) return s;
         if ((s = color_loop(loop_expand, W, &stats)) != SCC_OK) return s;
     }
