@@ -665,9 +665,11 @@ scc_status Engine::pivot_round(bool first, int expand, int W, scc_stats *stats, 
         first_reach_marks[1] = (int)nmarks - 1;
         if (search_stats) k_reach_stats(stream, g, st);
     }
-    zero_ctr(C_LIVE, 1);
+    // the first round of a decomposition follows the state initialisation,
+    // which zeroed C_LIVE, and has one pivot at most
+    if (!first) zero_ctr(C_LIVE, 1);
     k_update(stream, g, st);
-    k_clear_piv(stream, st, g.V + 1);
+    k_clear_piv(stream, st, first ? 1u : g.V + 1);
     if ((s = read_ctr()) != SCC_OK) return s;
     *live = h_ctr[CT(C_LIVE)];
     if (first) stats->first_pivot = h_ctr[CT(C_PIV0)];

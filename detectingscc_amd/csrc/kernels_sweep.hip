@@ -990,7 +990,7 @@ __global__ void canon_reset_kernel(uint32_t *aux, uint32_t *cnt, uint32_t V)
 {
     for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v <= V; v += gridDim.x * blockDim.x) {
         aux[v] = DEAD;
-        cnt[v] = 0;
+        if (cnt) cnt[v] = 0;
     }
 }
 
@@ -1058,7 +1058,29 @@ __global__ __launch_bounds__(256) void canon_min_kernel(uint32_t *lab, uint32_t 
             if (pending && v < aux[r]) atomicMin(&aux[r], v);
         }
     }
-    if (heavy != DEAD && lane() == 0 && hmin < aux[heavy]) atomicMin(&aux[heavy], hmin);
+    // The waves' running minima (every wave of a giant SCC's decomposition
+    // holds one for the same label) are combined per workgroup, and one
+    // lane adds it only if a coherent read shows it still lowers the value:
+    // a plain read sees a stale cached copy, and thousands of waves ending
+    // together with an atomic each on one address serialise.
+    __shared__ uint32_t s_heavy[4], s_hmin[4];
+    const uint32_t wv = threadIdx.x >> 6;
+    if (lane() == 0) {
+        s_heavy[wv] = heavy;
+        s_hmin[wv] = hmin;
+    }
+    __syncthreads();
+    if (lane() == 0 && heavy != DEAD) {
+        uint32_t m = hmin;
+        bool lead = true;   // the lowest wave holding this label adds for all of them
+        for (uint32_t w = 0; w < (blockDim.x >> 6); w++) {
+            if (w == wv || s_heavy[w] != heavy) continue;
+            if (w < wv) lead = false;
+            m = s_hmin[w] < m ? s_hmin[w] : m;
+        }
+        if (lead && m < __hip_atomic_load(&aux[heavy], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+            atomicMin(&aux[heavy], m);
+    }
 }
 
 // With sizes, each wave also keeps a running count for one label that
@@ -1130,7 +1152,7 @@ void k_canon(hipStream_t s, const DevGraph &g, DevState &st, bool sizes)
 {
     const uint32_t b = sweep_blocks(g.V);
     // cls is all DEAD at this point; it holds the SCC sizes from here on
-    canon_reset_kernel<<<b, SWEEP_BLOCK, 0, s>>>(st.aux, st.cls, g.V);
+    canon_reset_kernel<<<b, SWEEP_BLOCK, 0, s>>>(st.aux, sizes ? st.cls : nullptr, g.V);
     canon_min_kernel<<<b, SWEEP_BLOCK, 0, s>>>(st.lab, st.aux, st.ctr, g.V);
     canon_apply_kernel<<<b, SWEEP_BLOCK, 0, s>>>(st.lab, st.aux, st.cls, st.ctr, g.V, sizes ? 1 : 0);
     if (sizes) canon_largest_kernel<<<b, SWEEP_BLOCK, 0, s>>>(st.cls, st.ctr, g.V);
