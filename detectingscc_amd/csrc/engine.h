@@ -61,7 +61,9 @@ class Engine {
     void init_vis(bool elect_first = false);
     uint64_t limit() const;
     scc_status no_convergence(const char *what);
-    scc_status pivot_round(bool first, int expand, int W, scc_stats *stats, uint32_t *live);
+    // canon >= 0: queue the canonical labels (with sizes if canon == 1) gated
+    // on no live vertex being left, before the look at the counters
+    scc_status pivot_round(bool first, int expand, int W, scc_stats *stats, uint32_t *live, int canon = -1);
     scc_status min_label();
     scc_status color_loop(int expand, int W, scc_stats *stats);
     scc_status detect_phases(const scc_opts &o, int expand, int W, scc_stats *stats, bool local_only);
@@ -82,6 +84,7 @@ class Engine {
     uint8_t *d_tmp8 = nullptr;        // V+1 bytes for phase inputs/outputs
     int check_every = 0;
     bool search_stats = false;        // count |R| and |E_R| of the first search
+    bool do_seeded = false;           // init_vis(true) wrote the level-0 state of a two-direction search
     uint32_t trim_epoch = 0;          // last epoch stamped by a trim-1 list pass
     bool trace_levels = false;        // print each reachability decision to stderr
     std::vector<hipEvent_t> events;

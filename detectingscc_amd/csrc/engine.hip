@@ -404,7 +404,11 @@ uint32_t Engine::bpd(int W) const
     return (uint32_t)(b > 2048 ? 2048 : b);
 }
 
-void Engine::init_vis(bool elect_first) { k_init_vis(stream, g, st, elect_first); }
+void Engine::init_vis(bool elect_first)
+{
+    k_init_vis(stream, g, st, elect_first);
+    do_seeded = elect_first;
+}
 
 // The level-L arguments of direction d (0 = FW, 1 = BW): queues alternate
 // with the level, counts and hub lists rotate through three slots.
@@ -437,6 +441,7 @@ scc_status Engine::reach(bool fw, bool bw, bool single, int expand, int W, uint3
 {
     if (!fw && !bw) return SCC_OK;
     if (expand == SCC_EXPAND_DO) return reach_do(fw, bw, single, levels_out);
+    do_seeded = false;   // the seeded state is for a direction-optimised search only
     const int K = check_every > 0 ? check_every : 2;
     const uint32_t blocks = bpd(expand == SCC_EXPAND_VWARP ? W : 1);
     uint32_t level = 0;
@@ -496,7 +501,8 @@ scc_status Engine::reach_do(bool fw, bool bw, bool single, uint32_t *levels_out)
     if (fw) dirs[nd++] = 0;
     if (bw) dirs[nd++] = 1;
     const uint32_t full = do_blocks(g.V, nd);
-    k_do_begin(stream, st.ctr, nd);
+    if (!(do_seeded && nd == 2)) k_do_begin(stream, st.ctr, nd);
+    do_seeded = false;
     DoArgs a;
     std::memset(&a, 0, sizeof a);
     a.cls = st.cls;
@@ -646,7 +652,7 @@ scc_status Engine::no_convergence(const char *what)
 }
 
 // One pivot round: elect, seed, FW||BW, update.  Returns live count.
-scc_status Engine::pivot_round(bool first, int expand, int W, scc_stats *stats, uint32_t *live)
+scc_status Engine::pivot_round(bool first, int expand, int W, scc_stats *stats, uint32_t *live, int canon)
 {
     mark(SCC_PHASE_PIVOT);
     init_vis(first);
@@ -670,6 +676,13 @@ scc_status Engine::pivot_round(bool first, int expand, int W, scc_stats *stats, 
     if (!first) zero_ctr(C_LIVE, 1);
     k_update(stream, g, st);
     k_clear_piv(stream, st, first ? 1u : g.V + 1);
+    if (canon >= 0) {
+        // the labels of a decomposition this round finished, in the same
+        // look at the counters (a no-op on the device if a vertex is live)
+        mark(SCC_PHASE_LABEL);
+        k_canon(stream, g, st, canon != 0, true);
+        mark(-1);
+    }
     if ((s = read_ctr()) != SCC_OK) return s;
     *live = h_ctr[CT(C_LIVE)];
     if (first) stats->first_pivot = h_ctr[CT(C_PIV0)];
@@ -756,11 +769,13 @@ scc_status Engine::run(const scc_opts &o, scc_stats *out)
     k_init_state(stream, g, st, nullptr, !fresh_trim);   // also zeroes the counters
     scc_status s;
     uint32_t live = g.V;
+    bool labelled = false;   // the first pivot round finished the decomposition and its labels
     if (A == 'd') {
         if ((s = detect_phases(o, loop_expand, W, &stats, false)) != SCC_OK) return s;
     } else if (hong) {
         if (o.trim1 && (s = trim1(true, true, true, false)) != SCC_OK) return s;
-        if ((s = pivot_round(true, first_expand, W, &stats, &live)) != SCC_OK) return s;
+        if ((s = pivot_round(true, first_expand, W, &stats, &live, o.sizes != 0 ? 1 : 0)) != SCC_OK) return s;
+        labelled = live == 0;
         // with no live vertex left (a power-law graph's first search often
         // takes everything trim-1 left) every later phase removes nothing,
         // elects nothing and changes no label
@@ -786,12 +801,14 @@ scc_status Engine::run(const scc_opts &o, scc_stats *out)
         if ((s = pivot_round(true, first_expand, W, &stats, &live)) != SCC_OK) return s;
         if ((s = color_loop(loop_expand, W, &stats)) != SCC_OK) return s;
     }
-    if (A != 'd') {   // detectSCC labels come out of the merge already canonical
-        mark(SCC_PHASE_LABEL);
-        k_canon(stream, g, st, o.sizes != 0);
+    if (!labelled) {
+        if (A != 'd') {   // detectSCC labels come out of the merge already canonical
+            mark(SCC_PHASE_LABEL);
+            k_canon(stream, g, st, o.sizes != 0);
+        }
+        mark(-1);
+        if ((s = read_ctr()) != SCC_OK) return s;
     }
-    mark(-1);
-    if ((s = read_ctr()) != SCC_OK) return s;
     if (h_ctr[CT(C_ANOM)])
         return fail(SCC_ERR_DEVICE, "internal invariant violated %u times", h_ctr[CT(C_ANOM)]);
     collect_times(&stats);

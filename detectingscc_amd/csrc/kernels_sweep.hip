@@ -555,7 +555,9 @@ __global__ __launch_bounds__(256) void init_vis_kernel(const uint32_t *__restric
 
 // Sums the live counts into C_NLIVE; with ELECT also stores the winning key
 // in piv[0] and seeds its vertex (if any) in both directions, like
-// select_kernel + seed_counts_kernel.
+// select_kernel + seed_counts_kernel, and writes the level-0 state of a
+// direction-optimised search in both directions (what do_begin_kernel
+// writes), so that search needs no start-up launch.
 template <bool ELECT>
 __global__ __launch_bounds__(1024) void vis_finish_kernel(const uint64_t *__restrict__ bred, uint32_t nblocks,
                                                           uint32_t *ctr, uint64_t *piv, uint32_t *fvis,
@@ -597,6 +599,14 @@ __global__ __launch_bounds__(1024) void vis_finish_kernel(const uint64_t *__rest
         ctr[CT(C_BWCNT + i)] = i == 0 ? n : 0u;
     }
     zero_hub_counts(ctr);
+    DirState ds;
+    ds.mode = DIR_TD;
+    ds.end = 0;
+    ds.nf = n;
+    ds.pad = 0;
+    ds.visited = n;
+    store_dir_state(ctr, 0, 0, ds);
+    store_dir_state(ctr, 0, 1, ds);
 }
 
 void k_init_vis(hipStream_t s, const DevGraph &g, DevState &st, bool elect_first)
@@ -986,8 +996,21 @@ void k_color_finish(hipStream_t s, const DevGraph &g, DevState &st)
 // lab[v] -> min vertex id with the same representative; count roots and
 // the largest SCC.  A wave's lanes that share the first lane's label
 // combine into one atomic (the giant SCC would otherwise serialise).
-__global__ void canon_reset_kernel(uint32_t *aux, uint32_t *cnt, uint32_t V)
+// With a gate (the counters), a label pass queued before the host knows
+// whether any vertex is still live is a no-op unless C_LIVE (its own word
+// plus its slots, summed by each wave) is 0.
+__device__ __forceinline__ bool gate_closed(const uint32_t *gate)
 {
+    if (!gate) return false;
+    const uint32_t l = lane();
+    uint32_t x = l == 0 ? gate[CT(C_LIVE)] : (l <= (uint32_t)kSpread ? gate[CSLOT(C_LIVE, l - 1)] : 0u);
+    for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o);
+    return x != 0;
+}
+
+__global__ void canon_reset_kernel(uint32_t *aux, uint32_t *cnt, uint32_t V, const uint32_t *gate)
+{
+    if (gate_closed(gate)) return;
     for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v <= V; v += gridDim.x * blockDim.x) {
         aux[v] = DEAD;
         if (cnt) cnt[v] = 0;
@@ -1009,8 +1032,10 @@ constexpr uint32_t CANON_CHUNKS = 4;
 
 // Like the size counts below, each wave keeps a running minimum for one
 // label several of its lanes shared and writes it once at the end.
-__global__ __launch_bounds__(256) void canon_min_kernel(uint32_t *lab, uint32_t *aux, uint32_t *ctr, uint32_t V)
+__global__ __launch_bounds__(256) void canon_min_kernel(uint32_t *lab, uint32_t *aux, uint32_t *ctr, uint32_t V,
+                                                        const uint32_t *gate)
 {
+    if (gate_closed(gate)) return;
     uint32_t heavy = DEAD, hmin = DEAD;
     const uint32_t stride = gridDim.x * blockDim.x;
     for (uint32_t base = blockIdx.x * blockDim.x + (threadIdx.x & ~63u); base < V; base += CANON_CHUNKS * stride) {
@@ -1087,8 +1112,10 @@ __global__ __launch_bounds__(256) void canon_min_kernel(uint32_t *lab, uint32_t 
 // several of its lanes shared (a giant SCC recurs in every wave), added to
 // memory once at the end instead of once per wave iteration.
 __global__ __launch_bounds__(256) void canon_apply_kernel(uint32_t *lab, const uint32_t *__restrict__ aux,
-                                                          uint32_t *cnt, uint32_t *ctr, uint32_t V, int sizes)
+                                                          uint32_t *cnt, uint32_t *ctr, uint32_t V, int sizes,
+                                                          const uint32_t *gate)
 {
+    if (gate_closed(gate)) return;
     uint32_t roots = 0, heavy = DEAD, hcount = 0;
     const uint32_t stride = gridDim.x * blockDim.x;
     for (uint32_t base = blockIdx.x * blockDim.x + (threadIdx.x & ~63u); base < V; base += CANON_CHUNKS * stride) {
@@ -1136,8 +1163,10 @@ __global__ __launch_bounds__(256) void canon_apply_kernel(uint32_t *lab, const u
     spread_add32(ctr, C_NSCC, roots);
 }
 
-__global__ void canon_largest_kernel(const uint32_t *__restrict__ cnt, uint32_t *ctr, uint32_t V)
+__global__ void canon_largest_kernel(const uint32_t *__restrict__ cnt, uint32_t *ctr, uint32_t V,
+                                     const uint32_t *gate)
 {
+    if (gate_closed(gate)) return;
     uint32_t m = 0;
     for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x + 1; v <= V; v += gridDim.x * blockDim.x)
         m = cnt[v] > m ? cnt[v] : m;
@@ -1148,14 +1177,15 @@ __global__ void canon_largest_kernel(const uint32_t *__restrict__ cnt, uint32_t 
     if (lane() == 0 && m) atomicMax(&ctr[CT(C_LARGEST)], m);
 }
 
-void k_canon(hipStream_t s, const DevGraph &g, DevState &st, bool sizes)
+void k_canon(hipStream_t s, const DevGraph &g, DevState &st, bool sizes, bool gated)
 {
+    const uint32_t *gate = gated ? st.ctr : nullptr;
     const uint32_t b = sweep_blocks(g.V);
     // cls is all DEAD at this point; it holds the SCC sizes from here on
-    canon_reset_kernel<<<b, SWEEP_BLOCK, 0, s>>>(st.aux, sizes ? st.cls : nullptr, g.V);
-    canon_min_kernel<<<b, SWEEP_BLOCK, 0, s>>>(st.lab, st.aux, st.ctr, g.V);
-    canon_apply_kernel<<<b, SWEEP_BLOCK, 0, s>>>(st.lab, st.aux, st.cls, st.ctr, g.V, sizes ? 1 : 0);
-    if (sizes) canon_largest_kernel<<<b, SWEEP_BLOCK, 0, s>>>(st.cls, st.ctr, g.V);
+    canon_reset_kernel<<<b, SWEEP_BLOCK, 0, s>>>(st.aux, sizes ? st.cls : nullptr, g.V, gate);
+    canon_min_kernel<<<b, SWEEP_BLOCK, 0, s>>>(st.lab, st.aux, st.ctr, g.V, gate);
+    canon_apply_kernel<<<b, SWEEP_BLOCK, 0, s>>>(st.lab, st.aux, st.cls, st.ctr, g.V, sizes ? 1 : 0, gate);
+    if (sizes) canon_largest_kernel<<<b, SWEEP_BLOCK, 0, s>>>(st.cls, st.ctr, g.V, gate);
 }
 
 // ------------------------------------------------ phase exports ---

@@ -344,7 +344,9 @@ void k_color_init(hipStream_t s, const DevGraph &g, DevState &st);
 void k_color_prop(hipStream_t s, const DevGraph &g, DevState &st);
 void k_color_roots(hipStream_t s, const DevGraph &g, DevState &st);
 void k_color_finish(hipStream_t s, const DevGraph &g, DevState &st);
-void k_canon(hipStream_t s, const DevGraph &g, DevState &st, bool sizes);
+// Canonical labels; gated: each pass is a no-op unless C_LIVE is 0 (queued
+// before the host has read the live count of the round before).
+void k_canon(hipStream_t s, const DevGraph &g, DevState &st, bool sizes, bool gated = false);
 void k_seed_single(hipStream_t s, DevState &st, uint32_t src, int dir);
 void k_export_visited(hipStream_t s, const DevGraph &g, DevState &st, int dir, uint8_t *out_dev);
 void k_export_removed(hipStream_t s, const DevGraph &g, DevState &st, uint8_t *out_dev);
