@@ -674,13 +674,17 @@ scc_status Engine::pivot_round(bool first, int expand, int W, scc_stats *stats, 
     // the first round of a decomposition follows the state initialisation,
     // which zeroed C_LIVE, and has one pivot at most
     if (!first) zero_ctr(C_LIVE, 1);
-    k_update(stream, g, st);
+    // the first round's update also takes its SCC's minimum (one pivot)
+    k_update(stream, g, st, first);
     k_clear_piv(stream, st, first ? 1u : g.V + 1);
     if (canon >= 0) {
         // the labels of a decomposition this round finished, in the same
-        // look at the counters (a no-op on the device if a vertex is live)
+        // look at the counters (a no-op on the device if a vertex is live);
+        // in the first round the only non-final label is the pivot's, whose
+        // minimum the update left in aux
         mark(SCC_PHASE_LABEL);
-        k_canon(stream, g, st, canon != 0, true);
+        if (first && canon == 0) k_canon_apply(stream, g, st, true);
+        else k_canon(stream, g, st, canon != 0, true);
         mark(-1);
     }
     if ((s = read_ctr()) != SCC_OK) return s;

@@ -562,7 +562,7 @@ template <bool ELECT>
 __global__ __launch_bounds__(1024) void vis_finish_kernel(const uint64_t *__restrict__ bred, uint32_t nblocks,
                                                           uint32_t *ctr, uint64_t *piv, uint32_t *fvis,
                                                           uint32_t *bvis, uint32_t *qf, uint32_t *qb,
-                                                          uint32_t *pivlist)
+                                                          uint32_t *pivlist, uint32_t *aux)
 {
     __shared__ uint64_t s_best[16];
     uint64_t live = 0, best = 0;
@@ -591,6 +591,7 @@ __global__ __launch_bounds__(1024) void vis_finish_kernel(const uint64_t *__rest
         qb[0] = v;
         fvis[vword(v)] |= 1u << (v & 31);
         bvis[vword(v)] |= 1u << (v & 31);
+        aux[v] = DEAD;   // the update takes the minimum of the pivot's SCC here
     }
     ctr[CT(C_NPIV)] = n;
     ctr[CT(C_PIV0)] = n ? v : 0u;
@@ -618,12 +619,12 @@ void k_init_vis(hipStream_t s, const DevGraph &g, DevState &st, bool elect_first
         init_vis_kernel<true><<<blocks, SWEEP_BLOCK, 0, s>>>(st.cls, g.Fr, g.Br, st.fvis, st.bvis, st.bred, g.V,
                                                              nchunks);
         vis_finish_kernel<true><<<1, 1024, 0, s>>>(st.bred, blocks, st.ctr, st.piv, st.fvis, st.bvis, st.qf[0],
-                                                   st.qb[0], st.pivlist);
+                                                   st.qb[0], st.pivlist, st.aux);
     } else {
         init_vis_kernel<false><<<blocks, SWEEP_BLOCK, 0, s>>>(st.cls, g.Fr, g.Br, st.fvis, st.bvis, st.bred, g.V,
                                                               nchunks);
         vis_finish_kernel<false><<<1, 1024, 0, s>>>(st.bred, blocks, st.ctr, st.piv, st.fvis, st.bvis, st.qf[0],
-                                                    st.qb[0], st.pivlist);
+                                                    st.qb[0], st.pivlist, st.aux);
     }
 }
 
@@ -732,13 +733,21 @@ void k_seed_single(hipStream_t s, DevState &st, uint32_t src, int dir)
 // (defect D9) because each vertex is a pivot at most once.
 // Each lane takes UPD_CHUNKS vertices per step: their class words and
 // visited words load together, then their pivot keys.
+//
+// With scc_min (a round with one pivot, the first of a decomposition, whose
+// vertex is C_PIV0), the smallest vertex of the pivot's SCC also goes to
+// scc_min[pivot] (cleared when the pivot was seeded): the canonical label
+// of that SCC, so labelling a decomposition this round finishes needs no
+// minimum pass.  The minima are combined per workgroup and added behind a
+// coherent read, like the label pass's.
 constexpr uint32_t UPD_CHUNKS = 2;
 
 __global__ __launch_bounds__(256) void update_kernel(uint32_t *cls, uint32_t *lab, const uint64_t *__restrict__ piv,
                                                      const uint32_t *__restrict__ fvis,
-                                                     const uint32_t *__restrict__ bvis, uint32_t *ctr, uint32_t V)
+                                                     const uint32_t *__restrict__ bvis, uint32_t *ctr, uint32_t V,
+                                                     uint32_t *scc_min)
 {
-    uint32_t live = 0, found = 0;
+    uint32_t live = 0, found = 0, mmin = DEAD;
     const uint32_t stride = gridDim.x * blockDim.x;
     for (uint32_t base = blockIdx.x * blockDim.x + (threadIdx.x & ~63u); base <= V; base += UPD_CHUNKS * stride) {
         uint32_t v[UPD_CHUNKS], c[UPD_CHUNKS], fw[UPD_CHUNKS], bw[UPD_CHUNKS];
@@ -766,6 +775,7 @@ __global__ __launch_bounds__(256) void update_kernel(uint32_t *cls, uint32_t *la
                 lab[v[h]] = p;
                 cls[v[h]] = DEAD;
                 found += v[h] == p;
+                mmin = v[h] < mmin ? v[h] : mmin;
             } else {
                 cls[v[h]] = 3u * p + f + 2u * b;
                 live++;
@@ -774,12 +784,28 @@ __global__ __launch_bounds__(256) void update_kernel(uint32_t *cls, uint32_t *la
     }
     spread_add32(ctr, C_LIVE, live);
     spread_add32(ctr, C_PIVOT, found);
+    if (!scc_min) return;
+    __shared__ uint32_t s_min[4];
+    for (int o = 32; o > 0; o >>= 1) {
+        const uint32_t x = __shfl_xor(mmin, o);
+        mmin = x < mmin ? x : mmin;
+    }
+    if (lane() == 0) s_min[threadIdx.x >> 6] = mmin;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t m = DEAD;
+        for (uint32_t w = 0; w < (blockDim.x >> 6); w++) m = s_min[w] < m ? s_min[w] : m;
+        const uint32_t p0 = ctr[CT(C_PIV0)];
+        if (m != DEAD && p0 >= 1 && p0 <= V &&
+            m < __hip_atomic_load(&scc_min[p0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+            atomicMin(&scc_min[p0], m);
+    }
 }
 
-void k_update(hipStream_t s, const DevGraph &g, DevState &st)
+void k_update(hipStream_t s, const DevGraph &g, DevState &st, bool scc_min)
 {
     update_kernel<<<sweep_blocks(g.V), SWEEP_BLOCK, 0, s>>>(st.cls, st.lab, st.piv, st.fvis,
-                                                            st.bvis, st.ctr, g.V);
+                                                            st.bvis, st.ctr, g.V, scc_min ? st.aux : nullptr);
 }
 
 // The next round's classes are 3p+k for this round's pivots p: clear them.
@@ -1175,6 +1201,12 @@ __global__ void canon_largest_kernel(const uint32_t *__restrict__ cnt, uint32_t 
         m = x > m ? x : m;
     }
     if (lane() == 0 && m) atomicMax(&ctr[CT(C_LARGEST)], m);
+}
+
+void k_canon_apply(hipStream_t s, const DevGraph &g, DevState &st, bool gated)
+{
+    canon_apply_kernel<<<sweep_blocks(g.V), SWEEP_BLOCK, 0, s>>>(st.lab, st.aux, st.cls, st.ctr, g.V, 0,
+                                                                 gated ? st.ctr : nullptr);
 }
 
 void k_canon(hipStream_t s, const DevGraph &g, DevState &st, bool sizes, bool gated)

@@ -334,7 +334,9 @@ void k_init_vis(hipStream_t s, const DevGraph &g, DevState &st, bool elect_first
 // single_class: every live vertex is in class 0 (the first pivot)
 void k_elect(hipStream_t s, const DevGraph &g, DevState &st);
 void k_select(hipStream_t s, const DevGraph &g, DevState &st, int seed_bw);
-void k_update(hipStream_t s, const DevGraph &g, DevState &st);
+// scc_min: a round with one pivot (C_PIV0) also leaves its SCC's smallest
+// vertex in aux[pivot]
+void k_update(hipStream_t s, const DevGraph &g, DevState &st, bool scc_min = false);
 void k_clear_piv(hipStream_t s, DevState &st, uint32_t max_pivots);
 void k_reach_stats(hipStream_t s, const DevGraph &g, DevState &st);
 void k_minlabel_init(hipStream_t s, const DevGraph &g, DevState &st);
@@ -347,6 +349,9 @@ void k_color_finish(hipStream_t s, const DevGraph &g, DevState &st);
 // Canonical labels; gated: each pass is a no-op unless C_LIVE is 0 (queued
 // before the host has read the live count of the round before).
 void k_canon(hipStream_t s, const DevGraph &g, DevState &st, bool sizes, bool gated = false);
+// The apply pass alone: every non-final label's minimum is already in aux
+// (k_update with scc_min in a round that left no live vertex).
+void k_canon_apply(hipStream_t s, const DevGraph &g, DevState &st, bool gated);
 void k_seed_single(hipStream_t s, DevState &st, uint32_t src, int dir);
 void k_export_visited(hipStream_t s, const DevGraph &g, DevState &st, int dir, uint8_t *out_dev);
 void k_export_removed(hipStream_t s, const DevGraph &g, DevState &st, uint8_t *out_dev);
