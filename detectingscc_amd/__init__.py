@@ -50,7 +50,7 @@ class _HostGraph(ctypes.Structure):
 class _Opts(ctypes.Structure):
     _fields_ = [("algo", ctypes.c_char), ("trim1", ctypes.c_int), ("trim2", ctypes.c_int),
                 ("warp", ctypes.c_int), ("expand", ctypes.c_int), ("block", ctypes.c_int),
-                ("check_every", ctypes.c_int), ("sizes", ctypes.c_int)]
+                ("check_every", ctypes.c_int), ("sizes", ctypes.c_int), ("phase_times", ctypes.c_int)]
 
 
 class _Stats(ctypes.Structure):
@@ -281,12 +281,14 @@ def _stats(s: _Stats) -> Stats:
 
 
 def make_opts(algo: str = "g", trim1: bool = True, trim2: bool = True, warp: int = 1,
-              expand: int = EXPAND_AUTO, check_every: int = 0, sizes: bool = True) -> _Opts:
+              expand: int = EXPAND_AUTO, check_every: int = 0, sizes: bool = True,
+              phase_times: bool = False) -> _Opts:
     o = _Opts()
     lib().scc_opts_default(ctypes.byref(o))
     o.algo = algo.encode()[:1]
     o.trim1, o.trim2, o.warp, o.expand, o.check_every = int(trim1), int(trim2), int(warp), int(expand), int(check_every)
     o.sizes = int(sizes)
+    o.phase_times = int(phase_times)
     return o
 
 
@@ -447,8 +449,9 @@ class Context:
         _check(lib().scc_partition_upload(self._ctx, _p32(part)))
 
     def run(self, algo: str = "g", trim1: bool = True, trim2: bool = True, warp: int = 1,
-            expand: int = EXPAND_AUTO, check_every: int = 0, sizes: bool = True) -> Stats:
-        o = make_opts(algo, trim1, trim2, warp, expand, check_every, sizes)
+            expand: int = EXPAND_AUTO, check_every: int = 0, sizes: bool = True,
+            phase_times: bool = False) -> Stats:
+        o = make_opts(algo, trim1, trim2, warp, expand, check_every, sizes, phase_times)
         s = _Stats()
         _check(lib().scc_run(self._ctx, ctypes.byref(o), ctypes.byref(s)))
         return _stats(s)

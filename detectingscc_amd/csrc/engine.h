@@ -47,7 +47,10 @@ class Engine {
     uint32_t publish_ctr();                // queue a publication of the counters, returns its number
     scc_status wait_ctr(uint32_t seq);     // wait for publication `seq`, copy it into h_ctr
     void zero_ctr(int first, int n);
-    void mark(int phase);
+    // A device timestamp opening `phase` (-1: untimed, also the start and
+    // end of a decomposition).  Phase marks are recorded only with
+    // phase_marks (each costs the GPU a few microseconds) or `force`.
+    void mark(int phase, bool force = false);
     void collect_times(scc_stats *s);
     uint32_t bpd(int W) const;
     scc_status reach(bool fw, bool bw, bool single, int expand, int W, uint32_t *levels_out);
@@ -84,7 +87,8 @@ class Engine {
     uint8_t *d_tmp8 = nullptr;        // V+1 bytes for phase inputs/outputs
     int check_every = 0;
     bool search_stats = false;        // count |R| and |E_R| of the first search
-    bool do_seeded = false;           // init_vis(true) wrote the level-0 state of a two-direction search
+    bool do_seeded = false;
+    bool phase_marks = true;          // scc_opts.phase_times of the current run (phase APIs: on)           // init_vis(true) wrote the level-0 state of a two-direction search
     uint32_t trim_epoch = 0;          // last epoch stamped by a trim-1 list pass
     bool trace_levels = false;        // print each reachability decision to stderr
     std::vector<hipEvent_t> events;

@@ -369,8 +369,9 @@ void Engine::zero_ctr(int first, int n)
     zero_ctr_kernel<<<1, 64, 0, stream>>>(st.ctr, (uint32_t)first, (uint32_t)n);
 }
 
-void Engine::mark(int phase)
+void Engine::mark(int phase, bool force)
 {
+    if (phase >= 0 && !phase_marks && !force) return;
     if (nmarks == events.size()) {
         hipEvent_t e;
         if (hipEventCreate(&e) != hipSuccess) return;
@@ -393,7 +394,7 @@ void Engine::collect_times(scc_stats *s)
     for (size_t i = 0; i + 1 < nmarks; i++) {
         float ms = 0;
         (void)hipEventElapsedTime(&ms, events[i], events[i + 1]);
-        if (mark_phase[i] >= 0) s->ms_phase[mark_phase[i]] += ms;
+        if (mark_phase[i] >= 0 && phase_marks) s->ms_phase[mark_phase[i]] += ms;
         s->ms_scc += ms;
     }
 }
@@ -661,11 +662,11 @@ scc_status Engine::pivot_round(bool first, int expand, int W, scc_stats *stats, 
         k_elect(stream, g, st);
         k_select(stream, g, st, 1);
     }
-    mark(SCC_PHASE_REACH);
+    mark(SCC_PHASE_REACH, first);   // the first search is always timed
     const int first_mark = (int)nmarks - 1;
     scc_status s = reach(true, true, first, expand, W, &stats->levels);
     if (s != SCC_OK) return s;
-    mark(SCC_PHASE_UPDATE);
+    mark(SCC_PHASE_UPDATE, first);
     if (first) {
         first_reach_marks[0] = first_mark;
         first_reach_marks[1] = (int)nmarks - 1;
@@ -760,6 +761,7 @@ scc_status Engine::run(const scc_opts &o, scc_stats *out)
     const int loop_expand = (expand == SCC_EXPAND_VWARP) ? SCC_EXPAND_VERTEX : expand;
     check_every = o.check_every;
     search_stats = o.sizes != 0;
+    phase_marks = o.phase_times != 0;
 
     scc_stats stats;
     std::memset(&stats, 0, sizeof stats);
@@ -1017,6 +1019,7 @@ scc_status Engine::phase_begin(const uint8_t *dead)
     if (!has_graph) return fail(SCC_ERR_NOTREADY, "no graph uploaded");
     HIP_TRY(hipSetDevice(device));
     ran = false;
+    phase_marks = true;   // the phase entry points time their phase
     if (dead) HIP_TRY(hipMemcpyAsync(d_tmp8, dead, (size_t)g.V + 1, hipMemcpyHostToDevice, stream));
     k_init_state(stream, g, st, dead ? d_tmp8 : nullptr);   // also zeroes the counters
     return SCC_OK;

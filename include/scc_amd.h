@@ -140,6 +140,9 @@ typedef struct {
                          mappings, 0 = 2 (direction-optimised searches decide on the device) */
     int sizes;        /* 1: also compute stats.largest and the first search's
                          vertex/arc counts (two more sweeps) */
+    int phase_times;  /* 1: fill stats.ms_phase (a device timestamp at every phase
+                         boundary, each a few microseconds of idle GPU); 0: only
+                         ms_scc and ms_first_reach are timed */
 } scc_opts;
 
 void scc_opts_default(scc_opts *o);
@@ -169,7 +172,7 @@ typedef struct {
     uint64_t first_bw_vertices, first_bw_arcs; /* |R|, |E_R| of the first BW (opts.sizes) */
     double ms_scc;           /* CSR resident -> labels resident (device clock) */
     double ms_first_reach;   /* first-pivot FW||BW (device clock) */
-    double ms_phase[SCC_PHASE_COUNT];
+    double ms_phase[SCC_PHASE_COUNT];   /* with opts.phase_times, else 0 */
 } scc_stats;
 
 /* Create a context on HIP device `device` (the reference hard-codes
