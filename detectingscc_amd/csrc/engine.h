@@ -61,12 +61,17 @@ class Engine {
     // all in class 0, labels v | LAB_FINAL; its per-vertex words may be unwritten)
     // wait = false: one queued batch of passes, no look at the counters
     scc_status trim1(bool fixpoint, bool single = false, bool none_dead = false, bool wait = true);
-    void init_vis(bool elect_first = false);
+    void init_vis(bool elect_first = false, bool use_live_bits = false);
+    // the single-class trim-1's live bitmap (in d_tmp8)
+    uint32_t *live_bits() const { return reinterpret_cast<uint32_t *>(d_tmp8); }
     uint64_t limit() const;
     scc_status no_convergence(const char *what);
     // canon >= 0: queue the canonical labels (with sizes if canon == 1) gated
     // on no live vertex being left, before the look at the counters
-    scc_status pivot_round(bool first, int expand, int W, scc_stats *stats, uint32_t *live, int canon = -1);
+    // use_live_bits: the first round reads live_bits() (exact, every live vertex
+    // in class 0: a single-class trim-1 just ran) instead of the class words
+    scc_status pivot_round(bool first, int expand, int W, scc_stats *stats, uint32_t *live, int canon = -1,
+                           bool use_live_bits = false);
     scc_status min_label();
     scc_status color_loop(int expand, int W, scc_stats *stats);
     scc_status detect_phases(const scc_opts &o, int expand, int W, scc_stats *stats, bool local_only);
@@ -84,7 +89,7 @@ class Engine {
     uint32_t *h_pub = nullptr;        // mapped coherent host memory the GPU publishes counters into (2 buffers)
     uint32_t *d_pub = nullptr;        // its device address
     uint32_t pub_seq = 0;             // last publication requested
-    uint8_t *d_tmp8 = nullptr;        // V+1 bytes for phase inputs/outputs
+    uint8_t *d_tmp8 = nullptr;        // V+1 bytes for phase inputs/outputs; during a decomposition the live bitmap
     int check_every = 0;
     bool search_stats = false;        // count |R| and |E_R| of the first search
     bool do_seeded = false;

@@ -158,7 +158,9 @@ scc_status Engine::reserve(uint32_t V, uint32_t E)
         A(dalloc(&st.bigf[i], nhub)); A(dalloc(&st.bigb[i], nhub));
         A(dalloc(&st.bigoff_f[i], nhub)); A(dalloc(&st.bigoff_b[i], nhub));
     }
-    A(dalloc(&st.pivlist, nv)); A(dalloc(&st.stamp, nv)); A(dalloc(&st.part, nv + 1)); A(dalloc(&d_tmp8, nv));
+    // d_tmp8 also holds the single-class trim's live bitmap (live_bits())
+    A(dalloc(&st.pivlist, nv)); A(dalloc(&st.stamp, nv)); A(dalloc(&st.part, nv + 1));
+    A(dalloc(&d_tmp8, std::max(nv, 4 * (nwords + 2))));
     A(dalloc(&st.bred, 2 * (size_t)VIS_BLOCKS));
     if (e != hipSuccess) {
         (void)hipGetLastError();
@@ -405,9 +407,9 @@ uint32_t Engine::bpd(int W) const
     return (uint32_t)(b > 2048 ? 2048 : b);
 }
 
-void Engine::init_vis(bool elect_first)
+void Engine::init_vis(bool elect_first, bool use_live_bits)
 {
-    k_init_vis(stream, g, st, elect_first);
+    k_init_vis(stream, g, st, elect_first, use_live_bits ? live_bits() : nullptr);
     do_seeded = elect_first;
 }
 
@@ -604,9 +606,9 @@ scc_status Engine::trim1(bool fixpoint, bool single, bool none_dead, bool wait)
 {
     mark(SCC_PHASE_TRIM1);
     zero_ctr(C_TLIST, 3);
-    // the single-class live bitmap lives in the FW visited bitmap, which is
-    // free until the next search
-    uint32_t *live = single ? st.fvis : nullptr;
+    // the single-class live bitmap lives in the byte scratch array, free
+    // during a decomposition; the first pivot round reads it
+    uint32_t *live = single ? live_bits() : nullptr;
     k_trim1(stream, g, st, st.qf[1], st.qf[0], 0u, 0, kTrimSweepShare, live, fixpoint && single && none_dead);
     uint32_t pass = 1;
     if (!wait) {
@@ -653,10 +655,11 @@ scc_status Engine::no_convergence(const char *what)
 }
 
 // One pivot round: elect, seed, FW||BW, update.  Returns live count.
-scc_status Engine::pivot_round(bool first, int expand, int W, scc_stats *stats, uint32_t *live, int canon)
+scc_status Engine::pivot_round(bool first, int expand, int W, scc_stats *stats, uint32_t *live, int canon,
+                               bool use_live_bits)
 {
     mark(SCC_PHASE_PIVOT);
-    init_vis(first);
+    init_vis(first, first && use_live_bits);
     if (!first) {
         zero_ctr(C_NPIV, 1);
         k_elect(stream, g, st);
@@ -676,7 +679,7 @@ scc_status Engine::pivot_round(bool first, int expand, int W, scc_stats *stats, 
     // which zeroed C_LIVE, and has one pivot at most
     if (!first) zero_ctr(C_LIVE, 1);
     // the first round's update also takes its SCC's minimum (one pivot)
-    k_update(stream, g, st, first);
+    k_update(stream, g, st, first, first && use_live_bits ? live_bits() : nullptr);
     k_clear_piv(stream, st, first ? 1u : g.V + 1);
     if (canon >= 0) {
         // the labels of a decomposition this round finished, in the same
@@ -780,7 +783,9 @@ scc_status Engine::run(const scc_opts &o, scc_stats *out)
         if ((s = detect_phases(o, loop_expand, W, &stats, false)) != SCC_OK) return s;
     } else if (hong) {
         if (o.trim1 && (s = trim1(true, true, true, false)) != SCC_OK) return s;
-        if ((s = pivot_round(true, first_expand, W, &stats, &live, o.sizes != 0 ? 1 : 0)) != SCC_OK) return s;
+        // the single-class trim just left an exact live bitmap
+        if ((s = pivot_round(true, first_expand, W, &stats, &live, o.sizes != 0 ? 1 : 0, o.trim1 != 0)) != SCC_OK)
+            return s;
         labelled = live == 0;
         // with no live vertex left (a power-law graph's first search often
         // takes everything trim-1 left) every later phase removes nothing,
@@ -804,7 +809,7 @@ scc_status Engine::run(const scc_opts &o, scc_stats *out)
     } else {
         // vSlota/wSlota: one trim-1 pass, -q ignored (scc_Coloring.cu:507-508)
         if (o.trim1 && (s = trim1(false, true, false, false)) != SCC_OK) return s;
-        if ((s = pivot_round(true, first_expand, W, &stats, &live)) != SCC_OK) return s;
+        if ((s = pivot_round(true, first_expand, W, &stats, &live, -1, o.trim1 != 0)) != SCC_OK) return s;
         if ((s = color_loop(loop_expand, W, &stats)) != SCC_OK) return s;
     }
     if (!labelled) {

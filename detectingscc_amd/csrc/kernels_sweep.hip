@@ -495,6 +495,7 @@ constexpr uint32_t VIS_CHUNKS = 4;
 
 template <bool ELECT>
 __global__ __launch_bounds__(256) void init_vis_kernel(const uint32_t *__restrict__ cls,
+                                                       const uint32_t *__restrict__ live_bits,
                                                        const uint32_t *__restrict__ Fr,
                                                        const uint32_t *__restrict__ Br, uint32_t *fvis,
                                                        uint32_t *bvis, uint64_t *bred, uint32_t V,
@@ -512,7 +513,10 @@ __global__ __launch_bounds__(256) void init_vis_kernel(const uint32_t *__restric
         for (uint32_t h = 0; h < VIS_CHUNKS; h++) {
             const uint32_t ch = ch0 + h * nwaves, v = ch * 64 + lane();
             const bool in = ch < nchunks && v <= V;
-            c[h] = in ? cls[v] : DEAD;
+            if (live_bits)   // one class: the live bitmap (2 words a chunk instead of 64)
+                c[h] = in && ((live_bits[v >> 5] >> (v & 31)) & 1u) ? 0u : DEAD;
+            else
+                c[h] = in ? cls[v] : DEAD;
             if (ELECT) {
                 f0[h] = in ? Fr[v] : 0u;
                 f1[h] = in ? Fr[v + 1] : 0u;
@@ -610,19 +614,19 @@ __global__ __launch_bounds__(1024) void vis_finish_kernel(const uint64_t *__rest
     store_dir_state(ctr, 0, 1, ds);
 }
 
-void k_init_vis(hipStream_t s, const DevGraph &g, DevState &st, bool elect_first)
+void k_init_vis(hipStream_t s, const DevGraph &g, DevState &st, bool elect_first, const uint32_t *live_bits)
 {
     uint32_t nchunks = st.nwords / 2;
     uint32_t blocks = (nchunks + 3) / 4;
     blocks = blocks < 1 ? 1 : (blocks > VIS_BLOCKS ? VIS_BLOCKS : blocks);
     if (elect_first) {
-        init_vis_kernel<true><<<blocks, SWEEP_BLOCK, 0, s>>>(st.cls, g.Fr, g.Br, st.fvis, st.bvis, st.bred, g.V,
-                                                             nchunks);
+        init_vis_kernel<true><<<blocks, SWEEP_BLOCK, 0, s>>>(st.cls, live_bits, g.Fr, g.Br, st.fvis, st.bvis,
+                                                             st.bred, g.V, nchunks);
         vis_finish_kernel<true><<<1, 1024, 0, s>>>(st.bred, blocks, st.ctr, st.piv, st.fvis, st.bvis, st.qf[0],
                                                    st.qb[0], st.pivlist, st.aux);
     } else {
-        init_vis_kernel<false><<<blocks, SWEEP_BLOCK, 0, s>>>(st.cls, g.Fr, g.Br, st.fvis, st.bvis, st.bred, g.V,
-                                                              nchunks);
+        init_vis_kernel<false><<<blocks, SWEEP_BLOCK, 0, s>>>(st.cls, live_bits, g.Fr, g.Br, st.fvis, st.bvis,
+                                                              st.bred, g.V, nchunks);
         vis_finish_kernel<false><<<1, 1024, 0, s>>>(st.bred, blocks, st.ctr, st.piv, st.fvis, st.bvis, st.qf[0],
                                                     st.qb[0], st.pivlist, st.aux);
     }
@@ -745,7 +749,7 @@ constexpr uint32_t UPD_CHUNKS = 2;
 __global__ __launch_bounds__(256) void update_kernel(uint32_t *cls, uint32_t *lab, const uint64_t *__restrict__ piv,
                                                      const uint32_t *__restrict__ fvis,
                                                      const uint32_t *__restrict__ bvis, uint32_t *ctr, uint32_t V,
-                                                     uint32_t *scc_min)
+                                                     uint32_t *scc_min, const uint32_t *__restrict__ live_bits)
 {
     uint32_t live = 0, found = 0, mmin = DEAD;
     const uint32_t stride = gridDim.x * blockDim.x;
@@ -756,7 +760,10 @@ __global__ __launch_bounds__(256) void update_kernel(uint32_t *cls, uint32_t *la
         for (uint32_t h = 0; h < UPD_CHUNKS; h++) {
             v[h] = base + h * stride + lane();   // vertex 0, the dummy, is dead
             const bool in = v[h] <= V;
-            c[h] = in ? cls[v[h]] : DEAD;
+            if (live_bits)   // one class: the live bitmap instead of the class words
+                c[h] = in && ((live_bits[v[h] >> 5] >> (v[h] & 31)) & 1u) ? 0u : DEAD;
+            else
+                c[h] = in ? cls[v[h]] : DEAD;
             fw[h] = in ? fvis[vword(v[h])] : 0u;
             bw[h] = in ? bvis[vword(v[h])] : 0u;
         }
@@ -802,10 +809,11 @@ __global__ __launch_bounds__(256) void update_kernel(uint32_t *cls, uint32_t *la
     }
 }
 
-void k_update(hipStream_t s, const DevGraph &g, DevState &st, bool scc_min)
+void k_update(hipStream_t s, const DevGraph &g, DevState &st, bool scc_min, const uint32_t *live_bits)
 {
     update_kernel<<<sweep_blocks(g.V), SWEEP_BLOCK, 0, s>>>(st.cls, st.lab, st.piv, st.fvis,
-                                                            st.bvis, st.ctr, g.V, scc_min ? st.aux : nullptr);
+                                                            st.bvis, st.ctr, g.V, scc_min ? st.aux : nullptr,
+                                                            live_bits);
 }
 
 // The next round's classes are 3p+k for this round's pivots p: clear them.

@@ -330,13 +330,18 @@ size_t trim_list_words(uint32_t V);
 void k_trim2(hipStream_t s, const DevGraph &g, DevState &st);
 // Visited bitmaps of a search and C_NLIVE; with elect_first (every live
 // vertex in class 0) also the first pivot, seeded in both directions.
-void k_init_vis(hipStream_t s, const DevGraph &g, DevState &st, bool elect_first);
+// live_bits: every live vertex is in class 0 and the bitmap (1 bit per
+// vertex, as the single-class trim-1 leaves it) is exact; read instead of
+// the class words.
+void k_init_vis(hipStream_t s, const DevGraph &g, DevState &st, bool elect_first,
+                const uint32_t *live_bits = nullptr);
 // single_class: every live vertex is in class 0 (the first pivot)
 void k_elect(hipStream_t s, const DevGraph &g, DevState &st);
 void k_select(hipStream_t s, const DevGraph &g, DevState &st, int seed_bw);
 // scc_min: a round with one pivot (C_PIV0) also leaves its SCC's smallest
 // vertex in aux[pivot]
-void k_update(hipStream_t s, const DevGraph &g, DevState &st, bool scc_min = false);
+void k_update(hipStream_t s, const DevGraph &g, DevState &st, bool scc_min = false,
+              const uint32_t *live_bits = nullptr);   // live_bits as for k_init_vis
 void k_clear_piv(hipStream_t s, DevState &st, uint32_t max_pivots);
 void k_reach_stats(hipStream_t s, const DevGraph &g, DevState &st);
 void k_minlabel_init(hipStream_t s, const DevGraph &g, DevState &st);
