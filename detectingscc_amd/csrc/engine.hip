@@ -46,7 +46,7 @@ void Engine::release()
     auto f = [](void *p) {
         if (p) (void)hipFree(p);
     };
-    f(g.Fr); f(g.Fc); f(g.Br); f(g.Bc);
+    f(g.Fr); f(g.Fc); f(g.Br); f(g.Bc); f(g.F1); f(g.B1);
     f(st.cls); f(st.lab); f(st.aux); f(st.fvis); f(st.bvis); f(st.piv);
     for (int i = 0; i < 2; i++) {
         f(st.qf[i]); f(st.qb[i]);
@@ -124,7 +124,7 @@ scc_status Engine::reserve(uint32_t V, uint32_t E)
     auto f = [](void *p) {
         if (p) (void)hipFree(p);
     };
-    f(g.Fr); f(g.Fc); f(g.Br); f(g.Bc);
+    f(g.Fr); f(g.Fc); f(g.Br); f(g.Bc); f(g.F1); f(g.B1);
     f(st.cls); f(st.lab); f(st.aux); f(st.fvis); f(st.bvis); f(st.piv);
     for (int i = 0; i < 2; i++) {
         f(st.qf[i]); f(st.qb[i]);
@@ -146,6 +146,7 @@ scc_status Engine::reserve(uint32_t V, uint32_t E)
     };
     A(dalloc(&g.Fr, nv + 1)); A(dalloc(&g.Br, nv + 1));
     A(dalloc(&g.Fc, nE)); A(dalloc(&g.Bc, nE));
+    A(dalloc(&g.F1, nv)); A(dalloc(&g.B1, nv));
     A(dalloc(&st.cls, nv)); A(dalloc(&st.lab, nv)); A(dalloc(&st.aux, nv));
     A(dalloc(&st.fvis, 2 * nwords)); A(dalloc(&st.bvis, 2 * nwords));
     A(dalloc(&st.piv, 3 * nv + 3));
@@ -213,6 +214,7 @@ scc_status Engine::upload(uint32_t V, uint32_t E, const uint32_t *Fr, const uint
         HIP_TRY(hipMemcpyAsync(g.Fc, Fc, sizeof(uint32_t) * E, hipMemcpyHostToDevice, stream));
         HIP_TRY(hipMemcpyAsync(g.Bc, Bc, sizeof(uint32_t) * E, hipMemcpyHostToDevice, stream));
     }
+    k_first_arcs(stream, g);
     HIP_TRY(hipStreamSynchronize(stream));
     has_graph = true;
     has_part = false;
@@ -257,6 +259,7 @@ scc_status Engine::upload_device(uint32_t V, uint32_t E, const uint32_t *dFr, co
         HIP_TRY(hipMemcpyAsync(g.Fc, dFc, sizeof(uint32_t) * E, hipMemcpyDeviceToDevice, stream));
         HIP_TRY(hipMemcpyAsync(g.Bc, dBc, sizeof(uint32_t) * E, hipMemcpyDeviceToDevice, stream));
     }
+    k_first_arcs(stream, g);
     has_graph = true;
     has_part = false;
     ran = false;
@@ -516,6 +519,7 @@ scc_status Engine::reach_do(bool fw, bool bw, bool single, uint32_t *levels_out)
     for (int i = 0; i < nd; i++) {
         a.PR[i] = dirs[i] ? g.Fr : g.Br;
         a.PC[i] = dirs[i] ? g.Fc : g.Bc;
+        a.PF[i] = dirs[i] ? g.F1 : g.B1;
     }
     uint32_t level = 0;
     auto queue = [&](int window, uint32_t blocks) {

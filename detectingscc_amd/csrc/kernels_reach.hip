@@ -270,13 +270,14 @@ static void launch_lb(hipStream_t s, const LevelArgs &a, bool single, uint32_t b
 // boundaries make every write visible.  The pass pushes the vertices it
 // visits, so a following top-down level can continue from them.
 //
-// Two tiers: each lane probes the first BU_SHORT parents of its vertex;
-// vertices with longer parent lists and no hit yet are listed in LDS and
-// scanned by groups of BU_GROUP lanes, BU_GROUP parents per step, with a
-// ballot to stop each group at its first hit.  One parent up front and
-// groups of four were fastest on R-MAT scales 22 and 24 (probing more
-// parents before knowing whether the first one hits reads lines that are
-// mostly not needed).
+// Two tiers: each lane probes the first parent of its vertex, read from
+// the first-parent array (one coalesced word per vertex, loaded with the
+// row offsets); vertices with longer parent lists and no hit yet are listed
+// in LDS and scanned by groups of BU_GROUP lanes, BU_GROUP parents per
+// step, with a ballot to stop each group at its first hit.  One parent up
+// front and groups of four were fastest on R-MAT scales 22 and 24 (probing
+// more parents before knowing whether the first one hits reads lines that
+// are mostly not needed).
 //
 // The pass is wave-centric: each wave streams its own 64-vertex chunks,
 // two at a time (each lane carries one vertex of each, so twice as many
@@ -291,7 +292,7 @@ static void launch_lb(hipStream_t s, const LevelArgs &a, bool single, uint32_t b
 // wave ORs its hits into each with one plain store (device-wide atomics
 // run at about 10 G/s on MI355X, which would bound a pass that visits
 // millions of vertices).
-constexpr uint32_t BU_SHORT = 1;
+constexpr uint32_t BU_SHORT = 1;   // the first parent, read from the first-parent array
 constexpr uint32_t BU_GROUP = 4;
 constexpr uint32_t BU_WQ = 1024;
 constexpr uint32_t BU_CHUNKS = 2;   // 64-vertex chunks per wave iteration
@@ -318,9 +319,9 @@ constexpr uint32_t BU_LDS = 4 * BU_WQ + 4 * BU_CHUNKS * 64 + 4 * BU_CHUNKS * 2;
 // the flushes of the pass's workgroups are spread over kSpread addresses.
 template <bool SINGLE>
 __device__ __forceinline__ void bu_level(const DirArgs &D, const uint32_t *__restrict__ PR,
-                                         const uint32_t *__restrict__ PC, const uint32_t *cls, uint32_t V,
-                                         int L, uint32_t bid, uint32_t bpd, uint32_t *lds, uint32_t qcap,
-                                         uint32_t *anom)
+                                         const uint32_t *__restrict__ PC, const uint32_t *__restrict__ PF,
+                                         const uint32_t *cls, uint32_t V, int L, uint32_t bid, uint32_t bpd,
+                                         uint32_t *lds, uint32_t qcap, uint32_t *anom)
 {
     const uint32_t slot = bid % kSpread;
     uint32_t *cnt_out = &D.cslot[CT(((L + 1) % 3) * kSpread + slot)];
@@ -350,23 +351,19 @@ __device__ __forceinline__ void bu_level(const DirArgs &D, const uint32_t *__res
             word[h] = u[h] <= V ? D.vis[vword(u[h])] : ~0u;
             cand[h] = u[h] != 0 && !(word[h] & (1u << (u[h] & 31)));
         }
+        // the first parent comes from PF with the row offsets (coalesced),
+        // not from a random line of the parent list
+        uint32_t pf[BU_CHUNKS];
 #pragma unroll
         for (uint32_t h = 0; h < BU_CHUNKS; h++) {
             c[h] = (SINGLE || !cand[h]) ? 0u : cls[u[h]];
             b[h] = cand[h] ? PR[u[h]] : 0u;
             e[h] = cand[h] ? PR[u[h] + 1] : 0u;
+            pf[h] = cand[h] ? PF[u[h]] : 0u;
         }
-        uint32_t p[BU_CHUNKS][BU_SHORT];
-#pragma unroll
-        for (uint32_t h = 0; h < BU_CHUNKS; h++)
-#pragma unroll
-            for (uint32_t t = 0; t < BU_SHORT; t++) p[h][t] = b[h] + t < e[h] ? PC[b[h] + t] : 0u;
 #pragma unroll
         for (uint32_t h = 0; h < BU_CHUNKS; h++) {
-            found[h] = false;
-#pragma unroll
-            for (uint32_t t = 0; t < BU_SHORT; t++)
-                found[h] |= b[h] + t < e[h] && visited_parent<SINGLE>(D.vis, cls, p[h][t], c[h]);
+            found[h] = b[h] < e[h] && visited_parent<SINGLE>(D.vis, cls, pf[h], c[h]);
             need_long[h] = !found[h] && e[h] - b[h] > BU_SHORT;
             hits[h] = __ballot(found[h]);
         }
@@ -452,7 +449,8 @@ __global__ __launch_bounds__(256) void expand_do_kernel(DoArgs a, uint32_t bpd)
     if (st.mode == DIR_TD)
         lb_level<SINGLE>(D, a.cls, L, bid, bpd, s_lds, n, slotted ? s_pre : nullptr, a.qcap);
     else if (st.mode == DIR_BU)
-        bu_level<SINGLE>(D, a.PR[dir], a.PC[dir], a.cls, a.V, L, bid, bpd, s_lds, a.qcap, a.ctr + CT(C_ANOM));
+        bu_level<SINGLE>(D, a.PR[dir], a.PC[dir], a.PF[dir], a.cls, a.V, L, bid, bpd, s_lds, a.qcap,
+                         a.ctr + CT(C_ANOM));
 }
 
 __global__ void do_begin_kernel(uint32_t *ctr, int ndirs)

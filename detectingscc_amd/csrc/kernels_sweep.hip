@@ -98,6 +98,22 @@ void k_fill_u32(hipStream_t s, uint32_t *p, uint32_t val, uint64_t n)
     fill_u32_kernel<<<(uint32_t)(b < 1 ? 1 : (b > 4096 ? 4096 : b)), SWEEP_BLOCK, 0, s>>>(p, val, n);
 }
 
+__global__ void first_arcs_kernel(const uint32_t *__restrict__ Fr, const uint32_t *__restrict__ Fc,
+                                  const uint32_t *__restrict__ Br, const uint32_t *__restrict__ Bc, uint32_t *F1,
+                                  uint32_t *B1, uint32_t V)
+{
+    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v <= V; v += gridDim.x * blockDim.x) {
+        const uint32_t fb = Fr[v], fe = Fr[v + 1], bb = Br[v], be = Br[v + 1];
+        F1[v] = fb < fe ? Fc[fb] : 0u;
+        B1[v] = bb < be ? Bc[bb] : 0u;
+    }
+}
+
+void k_first_arcs(hipStream_t s, const DevGraph &g)
+{
+    first_arcs_kernel<<<sweep_blocks(g.V), SWEEP_BLOCK, 0, s>>>(g.Fr, g.Fc, g.Br, g.Bc, g.F1, g.B1, g.V);
+}
+
 // All vertices start live in class 0 (range 0, scc_WCC.cu:630-631) unless
 // `dead` marks them finished; lab[v] = v.  With vertices == 0 only the
 // counters and the dummy vertex 0 are written: the first trim-1 pass of a
@@ -155,16 +171,16 @@ __device__ __forceinline__ bool same_live(const uint32_t *cls, const uint32_t *l
 }
 
 // v (live, class c) keeps a live same-class in-neighbour and out-neighbour.
-// The first arc each way is probed with independent loads (row offsets,
-// then column entries, then class words: three dependent steps instead of
-// seven), and only a vertex with no hit there walks the rest of a list.
-// Probing more arcs up front adds random lines and is slower.
+// The first arc each way is probed first: pin = B1[v] and pout = F1[v],
+// loaded with the row offsets (coalesced, one dependent step to the
+// neighbours' class words or live bits instead of a random list line each),
+// and only a vertex with no hit there walks the rest of a list.  Probing
+// more arcs up front adds random lines and is slower.
 template <bool BITS>
 __device__ __forceinline__ bool trim1_keep_at(const uint32_t *__restrict__ Fc, const uint32_t *__restrict__ Bc,
                                               const uint32_t *cls, const uint32_t *live, uint32_t c, uint32_t fb,
-                                              uint32_t fe, uint32_t bb, uint32_t be)
+                                              uint32_t fe, uint32_t bb, uint32_t be, uint32_t pin, uint32_t pout)
 {
-    const uint32_t pin = bb < be ? Bc[bb] : 0u, pout = fb < fe ? Fc[fb] : 0u;
     bool kin = bb < be && same_live<BITS>(cls, live, pin, c);
     bool kout = fb < fe && same_live<BITS>(cls, live, pout, c);
     for (uint32_t i = bb + 1; !kin && i < be; i++) kin = same_live<BITS>(cls, live, Bc[i], c);
@@ -223,7 +239,9 @@ template <bool BITS>
 __global__ __launch_bounds__(256) void trim1_kernel(const uint32_t *__restrict__ Fr,
                                                     const uint32_t *__restrict__ Fc,
                                                     const uint32_t *__restrict__ Br,
-                                                    const uint32_t *__restrict__ Bc, uint32_t *cls,
+                                                    const uint32_t *__restrict__ Bc,
+                                                    const uint32_t *__restrict__ F1,
+                                                    const uint32_t *__restrict__ B1, uint32_t *cls,
                                                     uint32_t *lab, uint32_t *live, uint32_t V, uint32_t *ctr,
                                                     const uint32_t *in, uint32_t *out, uint32_t *stamp,
                                                     uint32_t epoch, uint32_t pass, uint32_t sweep_share,
@@ -261,7 +279,7 @@ __global__ __launch_bounds__(256) void trim1_kernel(const uint32_t *__restrict__
             bool gone = false;
             if (c != DEAD) {
                 const uint32_t fb = Fr[v], fe = Fr[v + 1], bb = Br[v], be = Br[v + 1];
-                if (!trim1_keep_at<BITS>(Fc, Bc, cls, live, c, fb, fe, bb, be)) {
+                if (!trim1_keep_at<BITS>(Fc, Bc, cls, live, c, fb, fe, bb, be, B1[v], F1[v])) {
                     cls[v] = DEAD;
                     gone = true;
                 }
@@ -300,7 +318,8 @@ __global__ __launch_bounds__(256) void trim1_kernel(const uint32_t *__restrict__
                     w = j < nf ? Fc[fb + j] : Bc[bb + (j - nf)];
                     const uint32_t c = BITS ? ((live[w >> 5] >> (w & 31)) & 1u ? 0u : DEAD) : cls[w];
                     if (c != DEAD && stamp[w] != epoch && atomicExch(&stamp[w], epoch) != epoch &&
-                        !trim1_keep_at<BITS>(Fc, Bc, cls, live, c, Fr[w], Fr[w + 1], Br[w], Br[w + 1]) &&
+                        !trim1_keep_at<BITS>(Fc, Bc, cls, live, c, Fr[w], Fr[w + 1], Br[w], Br[w + 1], B1[w],
+                                             F1[w]) &&
                         atomicCAS(&cls[w], c, DEAD) == c) {
                         if (BITS) atomicAnd(&live[w >> 5], ~(1u << (w & 31)));
                         gone = true;
@@ -394,11 +413,13 @@ void k_trim1(hipStream_t s, const DevGraph &g, DevState &st, const uint32_t *in,
     } else if (live) {
         if (pass == 0)
             live_bits_kernel<<<sweep_blocks(g.V), SWEEP_BLOCK, 0, s>>>(st.cls, live, g.V);
-        trim1_kernel<true><<<sweep_blocks(g.V), SWEEP_BLOCK, 0, s>>>(g.Fr, g.Fc, g.Br, g.Bc, st.cls, st.lab,
+        trim1_kernel<true><<<sweep_blocks(g.V), SWEEP_BLOCK, 0, s>>>(g.Fr, g.Fc, g.Br, g.Bc, g.F1, g.B1, st.cls,
+                                                                     st.lab,
                                                                      live, g.V, st.ctr, in, out, st.stamp,
                                                                      epoch, pass, sweep_share, cap);
     } else {
-        trim1_kernel<false><<<sweep_blocks(g.V), SWEEP_BLOCK, 0, s>>>(g.Fr, g.Fc, g.Br, g.Bc, st.cls, st.lab,
+        trim1_kernel<false><<<sweep_blocks(g.V), SWEEP_BLOCK, 0, s>>>(g.Fr, g.Fc, g.Br, g.Bc, g.F1, g.B1,
+                                                                      st.cls, st.lab,
                                                                       nullptr, g.V, st.ctr, in, out,
                                                                       st.stamp, epoch, pass, sweep_share, cap);
     }

@@ -145,6 +145,11 @@ __device__ __forceinline__ void seg_clear(uint32_t *own, uint32_t *slot0)
 struct DevGraph {
     uint32_t V = 0, E = 0;
     uint32_t *Fr = nullptr, *Fc = nullptr, *Br = nullptr, *Bc = nullptr;
+    // F1[v] = Fc[Fr[v]] and B1[v] = Bc[Br[v]] (0 for an empty row): the first
+    // out- and in-neighbour of every vertex in one coalesced word, for the
+    // probes that read only a list's first entry (trim-1, bottom-up), which
+    // would otherwise fetch a 64-byte line at a random address per vertex
+    uint32_t *F1 = nullptr, *B1 = nullptr;
 };
 
 struct DevState {
@@ -187,6 +192,7 @@ struct DirArgs {
 struct DoArgs {
     DirArgs d[2];
     const uint32_t *PR[2], *PC[2];   // parent lists of each direction (the opposite CSR)
+    const uint32_t *PF[2];           // first parent of each vertex (the opposite CSR's F1/B1)
     const uint32_t *cls;
     uint32_t *ctr;                   // counters: search state (C_DOST), C_NLIVE, C_NPIV
     uint32_t V;
@@ -301,6 +307,8 @@ __host__ __forceinline__ uint32_t dir_blocks(uint32_t bpd, int ndirs)
 
 // ---- sweep kernels (kernels_sweep.hip) ----
 void k_fill_u32(hipStream_t s, uint32_t *p, uint32_t val, uint64_t n);
+// g.F1 and g.B1 from the CSR (after an upload)
+void k_first_arcs(hipStream_t s, const DevGraph &g);
 void k_init_state(hipStream_t s, const DevGraph &g, DevState &st, const uint8_t *dead_dev,
                   bool vertices = true);
 // trim-1 pass `pass` of a fixpoint: pass 0 (and a pass whose input list,
