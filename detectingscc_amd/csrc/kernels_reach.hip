@@ -351,21 +351,23 @@ __device__ __forceinline__ void bu_level(const DirArgs &D, const uint32_t *__res
             word[h] = u[h] <= V ? D.vis[vword(u[h])] : ~0u;
             cand[h] = u[h] != 0 && !(word[h] & (1u << (u[h] & 31)));
         }
-        // the first parent comes from PF with the row offsets (coalesced),
-        // not from a random line of the parent list
+        // the first parent comes from PF (coalesced; 0: no parent), not from
+        // a random line of the parent list, and only a candidate whose first
+        // parent misses loads its row offsets to see whether more follow
         uint32_t pf[BU_CHUNKS];
 #pragma unroll
         for (uint32_t h = 0; h < BU_CHUNKS; h++) {
             c[h] = (SINGLE || !cand[h]) ? 0u : cls[u[h]];
-            b[h] = cand[h] ? PR[u[h]] : 0u;
-            e[h] = cand[h] ? PR[u[h] + 1] : 0u;
             pf[h] = cand[h] ? PF[u[h]] : 0u;
         }
 #pragma unroll
         for (uint32_t h = 0; h < BU_CHUNKS; h++) {
-            found[h] = b[h] < e[h] && visited_parent<SINGLE>(D.vis, cls, pf[h], c[h]);
-            need_long[h] = !found[h] && e[h] - b[h] > BU_SHORT;
+            found[h] = pf[h] && visited_parent<SINGLE>(D.vis, cls, pf[h], c[h]);
             hits[h] = __ballot(found[h]);
+            const bool more = pf[h] && !found[h];
+            b[h] = more ? PR[u[h]] : 0u;
+            e[h] = more ? PR[u[h] + 1] : 0u;
+            need_long[h] = more && e[h] - b[h] > BU_SHORT;
         }
         uint32_t nlong = 0;
 #pragma unroll

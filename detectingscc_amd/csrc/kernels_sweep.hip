@@ -174,17 +174,22 @@ __device__ __forceinline__ bool same_live(const uint32_t *cls, const uint32_t *l
 // The first arc each way is probed first: pin = B1[v] and pout = F1[v],
 // loaded with the row offsets (coalesced, one dependent step to the
 // neighbours' class words or live bits instead of a random list line each),
-// and only a vertex with no hit there walks the rest of a list.  Probing
-// more arcs up front adds random lines and is slower.
+// and only a vertex with no hit there loads its row offsets and walks the
+// rest of a list (a first neighbour of 0 means an empty list: vertex ids
+// start at 1).  Probing more arcs up front adds random lines and is slower.
 template <bool BITS>
-__device__ __forceinline__ bool trim1_keep_at(const uint32_t *__restrict__ Fc, const uint32_t *__restrict__ Bc,
-                                              const uint32_t *cls, const uint32_t *live, uint32_t c, uint32_t fb,
-                                              uint32_t fe, uint32_t bb, uint32_t be, uint32_t pin, uint32_t pout)
+__device__ __forceinline__ bool trim1_keep_at(const uint32_t *__restrict__ Fr, const uint32_t *__restrict__ Fc,
+                                              const uint32_t *__restrict__ Br, const uint32_t *__restrict__ Bc,
+                                              const uint32_t *cls, const uint32_t *live, uint32_t c, uint32_t v,
+                                              uint32_t pin, uint32_t pout)
 {
-    bool kin = bb < be && same_live<BITS>(cls, live, pin, c);
-    bool kout = fb < fe && same_live<BITS>(cls, live, pout, c);
-    for (uint32_t i = bb + 1; !kin && i < be; i++) kin = same_live<BITS>(cls, live, Bc[i], c);
-    for (uint32_t i = fb + 1; kin && !kout && i < fe; i++) kout = same_live<BITS>(cls, live, Fc[i], c);
+    bool kin = pin && same_live<BITS>(cls, live, pin, c);
+    bool kout = pout && same_live<BITS>(cls, live, pout, c);
+    if (!kin && pin)
+        for (uint32_t i = Br[v] + 1, be = Br[v + 1]; !kin && i < be; i++) kin = same_live<BITS>(cls, live, Bc[i], c);
+    if (kin && !kout && pout)
+        for (uint32_t i = Fr[v] + 1, fe = Fr[v + 1]; !kout && i < fe; i++)
+            kout = same_live<BITS>(cls, live, Fc[i], c);
     return kin && kout;
 }
 
@@ -278,8 +283,7 @@ __global__ __launch_bounds__(256) void trim1_kernel(const uint32_t *__restrict__
             }
             bool gone = false;
             if (c != DEAD) {
-                const uint32_t fb = Fr[v], fe = Fr[v + 1], bb = Br[v], be = Br[v + 1];
-                if (!trim1_keep_at<BITS>(Fc, Bc, cls, live, c, fb, fe, bb, be, B1[v], F1[v])) {
+                if (!trim1_keep_at<BITS>(Fr, Fc, Br, Bc, cls, live, c, v, B1[v], F1[v])) {
                     cls[v] = DEAD;
                     gone = true;
                 }
@@ -318,8 +322,7 @@ __global__ __launch_bounds__(256) void trim1_kernel(const uint32_t *__restrict__
                     w = j < nf ? Fc[fb + j] : Bc[bb + (j - nf)];
                     const uint32_t c = BITS ? ((live[w >> 5] >> (w & 31)) & 1u ? 0u : DEAD) : cls[w];
                     if (c != DEAD && stamp[w] != epoch && atomicExch(&stamp[w], epoch) != epoch &&
-                        !trim1_keep_at<BITS>(Fc, Bc, cls, live, c, Fr[w], Fr[w + 1], Br[w], Br[w + 1], B1[w],
-                                             F1[w]) &&
+                        !trim1_keep_at<BITS>(Fr, Fc, Br, Bc, cls, live, c, w, B1[w], F1[w]) &&
                         atomicCAS(&cls[w], c, DEAD) == c) {
                         if (BITS) atomicAnd(&live[w >> 5], ~(1u << (w & 31)));
                         gone = true;
