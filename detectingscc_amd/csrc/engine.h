@@ -60,7 +60,8 @@ class Engine {
     // decomposition); none_dead: besides, the state is fresh (no vertex dead,
     // all in class 0, labels v | LAB_FINAL; its per-vertex words may be unwritten)
     // wait = false: one queued batch of passes, no look at the counters
-    scc_status trim1(bool fixpoint, bool single = false, bool none_dead = false, bool wait = true);
+    scc_status trim1(bool fixpoint, bool single = false, bool none_dead = false, bool wait = true,
+                     bool labels = true);   // labels = false: a fresh state's labels stay unwritten
     void init_vis(bool elect_first = false, bool use_live_bits = false);
     // the single-class trim-1's live bitmap (in d_tmp8)
     uint32_t *live_bits() const { return reinterpret_cast<uint32_t *>(d_tmp8); }
@@ -70,8 +71,10 @@ class Engine {
     // on no live vertex being left, before the look at the counters
     // use_live_bits: the first round reads live_bits() (exact, every live vertex
     // in class 0: a single-class trim-1 just ran) instead of the class words
+    // lazy: the first round after a label-less trim (k_update lazy, labels
+    // from the bitmaps)
     scc_status pivot_round(bool first, int expand, int W, scc_stats *stats, uint32_t *live, int canon = -1,
-                           bool use_live_bits = false);
+                           bool use_live_bits = false, bool lazy = false);
     scc_status min_label();
     scc_status color_loop(int expand, int W, scc_stats *stats);
     scc_status detect_phases(const scc_opts &o, int expand, int W, scc_stats *stats, bool local_only);

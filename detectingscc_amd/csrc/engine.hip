@@ -606,14 +606,15 @@ scc_status Engine::reach_do(bool fw, bool bw, bool single, uint32_t *levels_out)
 static constexpr uint32_t kTrimSweepShare = 32;
 static constexpr uint32_t kTrimQueued = 4;
 
-scc_status Engine::trim1(bool fixpoint, bool single, bool none_dead, bool wait)
+scc_status Engine::trim1(bool fixpoint, bool single, bool none_dead, bool wait, bool labels)
 {
     mark(SCC_PHASE_TRIM1);
     zero_ctr(C_TLIST, 3);
     // the single-class live bitmap lives in the byte scratch array, free
     // during a decomposition; the first pivot round reads it
     uint32_t *live = single ? live_bits() : nullptr;
-    k_trim1(stream, g, st, st.qf[1], st.qf[0], 0u, 0, kTrimSweepShare, live, fixpoint && single && none_dead);
+    const bool degree_first = fixpoint && single && none_dead;
+    k_trim1(stream, g, st, st.qf[1], st.qf[0], 0u, 0, kTrimSweepShare, live, degree_first, labels || !degree_first);
     uint32_t pass = 1;
     if (!wait) {
         // one batch of passes and no look at the counters: the SCCs do not
@@ -660,8 +661,9 @@ scc_status Engine::no_convergence(const char *what)
 
 // One pivot round: elect, seed, FW||BW, update.  Returns live count.
 scc_status Engine::pivot_round(bool first, int expand, int W, scc_stats *stats, uint32_t *live, int canon,
-                               bool use_live_bits)
+                               bool use_live_bits, bool lazy)
 {
+    lazy = lazy && first;
     mark(SCC_PHASE_PIVOT);
     init_vis(first, first && use_live_bits);
     if (!first) {
@@ -683,7 +685,7 @@ scc_status Engine::pivot_round(bool first, int expand, int W, scc_stats *stats, 
     // which zeroed C_LIVE, and has one pivot at most
     if (!first) zero_ctr(C_LIVE, 1);
     // the first round's update also takes its SCC's minimum (one pivot)
-    k_update(stream, g, st, first, first && use_live_bits ? live_bits() : nullptr);
+    k_update(stream, g, st, first, first && use_live_bits ? live_bits() : nullptr, lazy);
     k_clear_piv(stream, st, first ? 1u : g.V + 1);
     if (canon >= 0) {
         // the labels of a decomposition this round finished, in the same
@@ -691,7 +693,8 @@ scc_status Engine::pivot_round(bool first, int expand, int W, scc_stats *stats, 
         // in the first round the only non-final label is the pivot's, whose
         // minimum the update left in aux
         mark(SCC_PHASE_LABEL);
-        if (first && canon == 0) k_canon_apply(stream, g, st, true);
+        if (lazy && canon == 0) k_label_from_bits(stream, g, st);
+        else if (first && canon == 0) k_canon_apply(stream, g, st, true);
         else k_canon(stream, g, st, canon != 0, true);
         mark(-1);
     }
@@ -786,11 +789,18 @@ scc_status Engine::run(const scc_opts &o, scc_stats *out)
     if (A == 'd') {
         if ((s = detect_phases(o, loop_expand, W, &stats, false)) != SCC_OK) return s;
     } else if (hong) {
-        if (o.trim1 && (s = trim1(true, true, true, false)) != SCC_OK) return s;
+        // lazy labels: the degree pass and the first update write no labels
+        // (nor the members' class words); a decomposition the first round
+        // finishes takes its labels from the search's bitmaps, otherwise a
+        // fix-up writes what the eager round would have (sizes needs them)
+        const bool lazy = o.trim1 && !o.sizes;
+        if (o.trim1 && (s = trim1(true, true, true, false, !lazy)) != SCC_OK) return s;
         // the single-class trim just left an exact live bitmap
-        if ((s = pivot_round(true, first_expand, W, &stats, &live, o.sizes != 0 ? 1 : 0, o.trim1 != 0)) != SCC_OK)
+        if ((s = pivot_round(true, first_expand, W, &stats, &live, o.sizes != 0 ? 1 : 0, o.trim1 != 0, lazy)) !=
+            SCC_OK)
             return s;
         labelled = live == 0;
+        if (live && lazy) k_label_fixup(stream, g, st);
         // with no live vertex left (a power-law graph's first search often
         // takes everything trim-1 left) every later phase removes nothing,
         // elects nothing and changes no label

@@ -343,15 +343,19 @@ __global__ __launch_bounds__(256) void trim1_kernel(const uint32_t *__restrict__
 // state and building the live bitmap: a vertex without an in- or out-arc is
 // removed (class DEAD, listed in `out`), every other vertex gets class 0 and
 // its live bit, and every vertex gets label v | LAB_FINAL, so the state
-// initialisation before it may skip the per-vertex words.  Pure streaming
-// of the row offsets, so each wave keeps TRIM_DEG_CHUNKS 64-vertex chunks'
-// loads in flight.
+// initialisation before it may skip the per-vertex words.  Without
+// write_labels the labels are left unwritten: a decomposition that the
+// first round finishes computes every label from the search's bitmaps
+// (label_from_bits_kernel), and otherwise label_fixup_kernel writes them.
+// Pure streaming of the row offsets, so each wave keeps TRIM_DEG_CHUNKS
+// 64-vertex chunks' loads in flight.
 constexpr uint32_t TRIM_DEG_CHUNKS = 4;
 
 __global__ __launch_bounds__(256) void trim1_degree_kernel(const uint32_t *__restrict__ Fr,
                                                            const uint32_t *__restrict__ Br, uint32_t *cls,
                                                            uint32_t *lab, uint32_t *live, uint32_t V,
-                                                           uint32_t *ctr, uint32_t *out, uint32_t cap)
+                                                           uint32_t *ctr, uint32_t *out, uint32_t cap,
+                                                           int write_labels)
 {
     __shared__ uint32_t s_q[4][TRIM_WQ];
     const uint32_t wv = threadIdx.x >> 6;
@@ -382,7 +386,7 @@ __global__ __launch_bounds__(256) void trim1_degree_kernel(const uint32_t *__res
             const bool gone = alive && (f0[h] == f1[h] || b0[h] == b1[h]);
             if (alive) {
                 cls[v[h]] = gone ? DEAD : 0u;
-                lab[v[h]] = v[h] | LAB_FINAL;
+                if (write_labels) lab[v[h]] = v[h] | LAB_FINAL;
             }
             const uint64_t m = __ballot(alive && !gone);
             if ((lane() & 31) == 0 && v[h] <= V) live[v[h] >> 5] = lane() < 32 ? (uint32_t)m : (uint32_t)(m >> 32);
@@ -407,12 +411,12 @@ __global__ void live_bits_kernel(const uint32_t *__restrict__ cls, uint32_t *liv
 }
 
 void k_trim1(hipStream_t s, const DevGraph &g, DevState &st, const uint32_t *in, uint32_t *out,
-             uint32_t epoch, uint32_t pass, uint32_t sweep_share, uint32_t *live, bool degree_first)
+             uint32_t epoch, uint32_t pass, uint32_t sweep_share, uint32_t *live, bool degree_first, bool labels)
 {
     const uint32_t cap = trim_slot_cap(g.V);
     if (live && pass == 0 && degree_first) {
         trim1_degree_kernel<<<sweep_blocks(g.V), SWEEP_BLOCK, 0, s>>>(g.Fr, g.Br, st.cls, st.lab, live, g.V,
-                                                                      st.ctr, out, cap);
+                                                                      st.ctr, out, cap, labels ? 1 : 0);
     } else if (live) {
         if (pass == 0)
             live_bits_kernel<<<sweep_blocks(g.V), SWEEP_BLOCK, 0, s>>>(st.cls, live, g.V);
@@ -773,7 +777,8 @@ constexpr uint32_t UPD_CHUNKS = 2;
 __global__ __launch_bounds__(256) void update_kernel(uint32_t *cls, uint32_t *lab, const uint64_t *__restrict__ piv,
                                                      const uint32_t *__restrict__ fvis,
                                                      const uint32_t *__restrict__ bvis, uint32_t *ctr, uint32_t V,
-                                                     uint32_t *scc_min, const uint32_t *__restrict__ live_bits)
+                                                     uint32_t *scc_min, const uint32_t *__restrict__ live_bits,
+                                                     int lazy)
 {
     uint32_t live = 0, found = 0, mmin = DEAD;
     const uint32_t stride = gridDim.x * blockDim.x;
@@ -803,8 +808,10 @@ __global__ __launch_bounds__(256) void update_kernel(uint32_t *cls, uint32_t *la
             }
             const uint32_t f = (fw[h] >> (v[h] & 31)) & 1u, b = (bw[h] >> (v[h] & 31)) & 1u;
             if (f && b) {
-                lab[v[h]] = p;
-                cls[v[h]] = DEAD;
+                if (!lazy) {   // lazy: label_from_bits_kernel or label_fixup_kernel writes these
+                    lab[v[h]] = p;
+                    cls[v[h]] = DEAD;
+                }
                 found += v[h] == p;
                 mmin = v[h] < mmin ? v[h] : mmin;
             } else {
@@ -833,11 +840,44 @@ __global__ __launch_bounds__(256) void update_kernel(uint32_t *cls, uint32_t *la
     }
 }
 
-void k_update(hipStream_t s, const DevGraph &g, DevState &st, bool scc_min, const uint32_t *live_bits)
+void k_update(hipStream_t s, const DevGraph &g, DevState &st, bool scc_min, const uint32_t *live_bits, bool lazy)
 {
     update_kernel<<<sweep_blocks(g.V), SWEEP_BLOCK, 0, s>>>(st.cls, st.lab, st.piv, st.fvis,
                                                             st.bvis, st.ctr, g.V, scc_min ? st.aux : nullptr,
-                                                            live_bits);
+                                                            live_bits, lazy ? 1 : 0);
+}
+
+// Members of the first pivot's SCC after a lazy first round: visited by
+// both searches and not finished when the bitmaps were initialised (each
+// bitmap's second word per group holds the finished bits).
+__device__ __forceinline__ bool first_member(const uint32_t *fvis, const uint32_t *bvis, uint32_t v)
+{
+    const uint64_t f = reinterpret_cast<const uint64_t *>(fvis)[v >> 5];
+    const uint64_t b = reinterpret_cast<const uint64_t *>(bvis)[v >> 5];
+    const uint32_t fv = (uint32_t)f & ~(uint32_t)(f >> 32), bv = (uint32_t)b & ~(uint32_t)(b >> 32);
+    return ((fv & bv) >> (v & 31)) & 1u;
+}
+
+// A lazy first round that left live vertices: the labels and the members'
+// class words the eager round would have written (members: the pivot,
+// non-final; every other vertex: itself, final), before the later phases.
+__global__ void label_fixup_kernel(uint32_t *cls, uint32_t *lab, const uint32_t *__restrict__ fvis,
+                                   const uint32_t *__restrict__ bvis, const uint32_t *__restrict__ ctr, uint32_t V)
+{
+    const uint32_t p0 = ctr[CT(C_PIV0)];
+    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x + 1; v <= V; v += gridDim.x * blockDim.x) {
+        if (first_member(fvis, bvis, v)) {
+            lab[v] = p0;
+            cls[v] = DEAD;
+        } else {
+            lab[v] = v | LAB_FINAL;
+        }
+    }
+}
+
+void k_label_fixup(hipStream_t s, const DevGraph &g, DevState &st)
+{
+    label_fixup_kernel<<<sweep_blocks(g.V), SWEEP_BLOCK, 0, s>>>(st.cls, st.lab, st.fvis, st.bvis, st.ctr, g.V);
 }
 
 // The next round's classes are 3p+k for this round's pivots p: clear them.
@@ -1233,6 +1273,40 @@ __global__ void canon_largest_kernel(const uint32_t *__restrict__ cnt, uint32_t 
         m = x > m ? x : m;
     }
     if (lane() == 0 && m) atomicMax(&ctr[CT(C_LARGEST)], m);
+}
+
+// The labels of a decomposition that a lazy first round finished (gated:
+// a no-op unless C_LIVE is 0): a member of the pivot's SCC gets the SCC's
+// smallest vertex (aux[pivot], left by the update), every other vertex
+// (trimmed) itself.  Reads two bitmap words per 32 vertices instead of the
+// labels; counts the roots like canon_apply_kernel.
+__global__ __launch_bounds__(256) void label_from_bits_kernel(uint32_t *lab, const uint32_t *__restrict__ fvis,
+                                                              const uint32_t *__restrict__ bvis,
+                                                              const uint32_t *__restrict__ aux, uint32_t *ctr,
+                                                              uint32_t V, const uint32_t *gate)
+{
+    if (gate_closed(gate)) return;
+    const uint32_t p0 = ctr[CT(C_PIV0)];
+    const uint32_t rmin = p0 >= 1 && p0 <= V ? aux[p0] : DEAD;
+    uint32_t roots = 0;
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t base = blockIdx.x * blockDim.x + (threadIdx.x & ~63u); base < V; base += CANON_CHUNKS * stride) {
+#pragma unroll
+        for (uint32_t h = 0; h < CANON_CHUNKS; h++) {
+            const uint32_t v = base + h * stride + lane() + 1;
+            if (v > V) continue;
+            const uint32_t r = first_member(fvis, bvis, v) ? rmin : v;
+            lab[v] = r;
+            roots += r == v;
+        }
+    }
+    spread_add32(ctr, C_NSCC, roots);
+}
+
+void k_label_from_bits(hipStream_t s, const DevGraph &g, DevState &st)
+{
+    label_from_bits_kernel<<<sweep_blocks(g.V), SWEEP_BLOCK, 0, s>>>(st.lab, st.fvis, st.bvis, st.aux, st.ctr,
+                                                                     g.V, st.ctr);
 }
 
 void k_canon_apply(hipStream_t s, const DevGraph &g, DevState &st, bool gated)

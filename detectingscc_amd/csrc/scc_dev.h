@@ -328,7 +328,8 @@ void k_init_state(hipStream_t s, const DevGraph &g, DevState &st, const uint8_t 
 // neighbour is live then, and writes every vertex's class word and label
 // itself, so the fresh state's per-vertex words need not be written before.
 void k_trim1(hipStream_t s, const DevGraph &g, DevState &st, const uint32_t *in, uint32_t *out,
-             uint32_t epoch, uint32_t pass, uint32_t sweep_share, uint32_t *live, bool degree_first);
+             uint32_t epoch, uint32_t pass, uint32_t sweep_share, uint32_t *live, bool degree_first,
+             bool labels = true);   // labels = false: the degree pass leaves the labels unwritten
 // Largest number of vertices the workgroups b with b % kSpread == j own in
 // a pass over 64-vertex chunks where wave w of `grid` 4-wave workgroups
 // takes chunks w, w + 4 * grid, ...: the size of slot j's list region.
@@ -348,8 +349,17 @@ void k_elect(hipStream_t s, const DevGraph &g, DevState &st);
 void k_select(hipStream_t s, const DevGraph &g, DevState &st, int seed_bw);
 // scc_min: a round with one pivot (C_PIV0) also leaves its SCC's smallest
 // vertex in aux[pivot]
+// live_bits as for k_init_vis; lazy (a first round after a label-less
+// degree pass): the members of the pivot's SCC keep their words, which
+// k_label_from_bits (nothing left live) or k_label_fixup writes
 void k_update(hipStream_t s, const DevGraph &g, DevState &st, bool scc_min = false,
-              const uint32_t *live_bits = nullptr);   // live_bits as for k_init_vis
+              const uint32_t *live_bits = nullptr, bool lazy = false);
+// Canonical labels of a decomposition that a lazy first round finished, from
+// the search's bitmaps (gated: a no-op unless C_LIVE is 0).
+void k_label_from_bits(hipStream_t s, const DevGraph &g, DevState &st);
+// After a lazy first round that left live vertices: the labels and class
+// words an eager first round would have left.
+void k_label_fixup(hipStream_t s, const DevGraph &g, DevState &st);
 void k_clear_piv(hipStream_t s, DevState &st, uint32_t max_pivots);
 void k_reach_stats(hipStream_t s, const DevGraph &g, DevState &st);
 void k_minlabel_init(hipStream_t s, const DevGraph &g, DevState &st);
