@@ -686,7 +686,7 @@ scc_status Engine::pivot_round(bool first, int expand, int W, scc_stats *stats, 
     if (!first) zero_ctr(C_LIVE, 1);
     // the first round's update also takes its SCC's minimum (one pivot)
     k_update(stream, g, st, first, first && use_live_bits ? live_bits() : nullptr, lazy);
-    k_clear_piv(stream, st, first ? 1u : g.V + 1);
+    if (!first) k_clear_piv(stream, st, g.V + 1);   // the first round's update clears its pivot's entries
     if (canon >= 0) {
         // the labels of a decomposition this round finished, in the same
         // look at the counters (a no-op on the device if a vertex is live);

@@ -774,12 +774,19 @@ void k_seed_single(hipStream_t s, DevState &st, uint32_t src, int dir)
 // coherent read, like the label pass's.
 constexpr uint32_t UPD_CHUNKS = 2;
 
-__global__ __launch_bounds__(256) void update_kernel(uint32_t *cls, uint32_t *lab, const uint64_t *__restrict__ piv,
+__global__ __launch_bounds__(256) void update_kernel(uint32_t *cls, uint32_t *lab, uint64_t *piv,
                                                      const uint32_t *__restrict__ fvis,
                                                      const uint32_t *__restrict__ bvis, uint32_t *ctr, uint32_t V,
                                                      uint32_t *scc_min, const uint32_t *__restrict__ live_bits,
                                                      int lazy)
 {
+    // the first round (scc_min) also clears its pivot's next-round entries
+    // 3p..3p+2 of the pivot table, which this round does not read (its one
+    // class is 0 and p >= 1), so no separate clearing launch follows
+    if (scc_min && blockIdx.x == 0 && threadIdx.x < 3) {
+        const uint32_t p0 = ctr[CT(C_PIV0)];
+        if (p0 >= 1 && p0 <= V) piv[3ull * p0 + threadIdx.x] = 0;
+    }
     uint32_t live = 0, found = 0, mmin = DEAD;
     const uint32_t stride = gridDim.x * blockDim.x;
     for (uint32_t base = blockIdx.x * blockDim.x + (threadIdx.x & ~63u); base <= V; base += UPD_CHUNKS * stride) {
