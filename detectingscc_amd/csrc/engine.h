@@ -95,8 +95,8 @@ class Engine {
     uint8_t *d_tmp8 = nullptr;        // V+1 bytes for phase inputs/outputs; during a decomposition the live bitmap
     int check_every = 0;
     bool search_stats = false;        // count |R| and |E_R| of the first search
-    bool do_seeded = false;
-    bool phase_marks = true;          // scc_opts.phase_times of the current run (phase APIs: on)           // init_vis(true) wrote the level-0 state of a two-direction search
+    bool do_seeded = false;           // init_vis(true) wrote the level-0 state of a two-direction search
+    bool phase_marks = true;          // scc_opts.phase_times of the current run (phase APIs: on)
     uint32_t trim_epoch = 0;          // last epoch stamped by a trim-1 list pass
     bool trace_levels = false;        // print each reachability decision to stderr
     std::vector<hipEvent_t> events;
