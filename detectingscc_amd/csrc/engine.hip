@@ -614,7 +614,8 @@ scc_status Engine::trim1(bool fixpoint, bool single, bool none_dead, bool wait, 
     // during a decomposition; the first pivot round reads it
     uint32_t *live = single ? live_bits() : nullptr;
     const bool degree_first = fixpoint && single && none_dead;
-    k_trim1(stream, g, st, st.qf[1], st.qf[0], 0u, 0, kTrimSweepShare, live, degree_first, labels || !degree_first);
+    const bool eager = labels || !degree_first;   // lazy: the live bitmap alone (no class or label writes)
+    k_trim1(stream, g, st, st.qf[1], st.qf[0], 0u, 0, kTrimSweepShare, live, degree_first, eager);
     uint32_t pass = 1;
     if (!wait) {
         // one batch of passes and no look at the counters: the SCCs do not
@@ -627,7 +628,7 @@ scc_status Engine::trim1(bool fixpoint, bool single, bool none_dead, bool wait, 
                 trim_epoch = 1;
             }
             k_trim1(stream, g, st, st.qf[(pass + 1) & 1], st.qf[pass & 1], trim_epoch, pass, kTrimSweepShare, live,
-                    true);
+                    true, eager);
         }
         return SCC_OK;
     }
@@ -638,7 +639,7 @@ scc_status Engine::trim1(bool fixpoint, bool single, bool none_dead, bool wait, 
                 trim_epoch = 1;
             }
             k_trim1(stream, g, st, st.qf[(pass + 1) & 1], st.qf[pass & 1], trim_epoch, pass, kTrimSweepShare, live,
-                    true);
+                    true, eager);
         }
         scc_status s = read_ctr();
         if (s != SCC_OK) return s;
@@ -800,7 +801,7 @@ scc_status Engine::run(const scc_opts &o, scc_stats *out)
             SCC_OK)
             return s;
         labelled = live == 0;
-        if (live && lazy) k_label_fixup(stream, g, st);
+        if (live && lazy) k_label_fixup(stream, g, st, live_bits());
         // with no live vertex left (a power-law graph's first search often
         // takes everything trim-1 left) every later phase removes nothing,
         // elects nothing and changes no label

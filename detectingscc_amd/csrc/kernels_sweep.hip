@@ -250,7 +250,7 @@ __global__ __launch_bounds__(256) void trim1_kernel(const uint32_t *__restrict__
                                                     uint32_t *lab, uint32_t *live, uint32_t V, uint32_t *ctr,
                                                     const uint32_t *in, uint32_t *out, uint32_t *stamp,
                                                     uint32_t epoch, uint32_t pass, uint32_t sweep_share,
-                                                    uint32_t cap)
+                                                    uint32_t cap, int lazy)
 {
     __shared__ uint32_t s_q[4][TRIM_WQ];
     __shared__ uint32_t s_rounds;
@@ -284,7 +284,7 @@ __global__ __launch_bounds__(256) void trim1_kernel(const uint32_t *__restrict__
             bool gone = false;
             if (c != DEAD) {
                 if (!trim1_keep_at<BITS>(Fr, Fc, Br, Bc, cls, live, c, v, B1[v], F1[v])) {
-                    cls[v] = DEAD;
+                    if (!lazy) cls[v] = DEAD;
                     gone = true;
                 }
             }
@@ -323,8 +323,9 @@ __global__ __launch_bounds__(256) void trim1_kernel(const uint32_t *__restrict__
                     const uint32_t c = BITS ? ((live[w >> 5] >> (w & 31)) & 1u ? 0u : DEAD) : cls[w];
                     if (c != DEAD && stamp[w] != epoch && atomicExch(&stamp[w], epoch) != epoch &&
                         !trim1_keep_at<BITS>(Fr, Fc, Br, Bc, cls, live, c, w, B1[w], F1[w]) &&
-                        atomicCAS(&cls[w], c, DEAD) == c) {
-                        if (BITS) atomicAnd(&live[w >> 5], ~(1u << (w & 31)));
+                        (BITS ? (atomicAnd(&live[w >> 5], ~(1u << (w & 31))) >> (w & 31)) & 1u
+                              : atomicCAS(&cls[w], c, DEAD) == c)) {
+                        if (BITS && !lazy) cls[w] = DEAD;
                         gone = true;
                     }
                 }
@@ -344,9 +345,11 @@ __global__ __launch_bounds__(256) void trim1_kernel(const uint32_t *__restrict__
 // removed (class DEAD, listed in `out`), every other vertex gets class 0 and
 // its live bit, and every vertex gets label v | LAB_FINAL, so the state
 // initialisation before it may skip the per-vertex words.  Without
-// write_labels the labels are left unwritten: a decomposition that the
-// first round finishes computes every label from the search's bitmaps
-// (label_from_bits_kernel), and otherwise label_fixup_kernel writes them.
+// write_labels (lazy) the labels and class words are left unwritten and the
+// whole single-class trim works from the live bitmap: a decomposition that
+// the first round finishes computes every label from the search's bitmaps
+// (label_from_bits_kernel), and otherwise label_fixup_kernel writes the
+// labels and class words the eager passes would have.
 // Pure streaming of the row offsets, so each wave keeps TRIM_DEG_CHUNKS
 // 64-vertex chunks' loads in flight.
 constexpr uint32_t TRIM_DEG_CHUNKS = 4;
@@ -384,9 +387,9 @@ __global__ __launch_bounds__(256) void trim1_degree_kernel(const uint32_t *__res
         for (uint32_t h = 0; h < TRIM_DEG_CHUNKS; h++) {
             const bool alive = v[h] >= 1 && v[h] <= V;   // vertex 0, the dummy, is dead
             const bool gone = alive && (f0[h] == f1[h] || b0[h] == b1[h]);
-            if (alive) {
+            if (alive && write_labels) {
                 cls[v[h]] = gone ? DEAD : 0u;
-                if (write_labels) lab[v[h]] = v[h] | LAB_FINAL;
+                lab[v[h]] = v[h] | LAB_FINAL;
             }
             const uint64_t m = __ballot(alive && !gone);
             if ((lane() & 31) == 0 && v[h] <= V) live[v[h] >> 5] = lane() < 32 ? (uint32_t)m : (uint32_t)(m >> 32);
@@ -423,12 +426,12 @@ void k_trim1(hipStream_t s, const DevGraph &g, DevState &st, const uint32_t *in,
         trim1_kernel<true><<<sweep_blocks(g.V), SWEEP_BLOCK, 0, s>>>(g.Fr, g.Fc, g.Br, g.Bc, g.F1, g.B1, st.cls,
                                                                      st.lab,
                                                                      live, g.V, st.ctr, in, out, st.stamp,
-                                                                     epoch, pass, sweep_share, cap);
+                                                                     epoch, pass, sweep_share, cap, labels ? 0 : 1);
     } else {
         trim1_kernel<false><<<sweep_blocks(g.V), SWEEP_BLOCK, 0, s>>>(g.Fr, g.Fc, g.Br, g.Bc, g.F1, g.B1,
                                                                       st.cls, st.lab,
                                                                       nullptr, g.V, st.ctr, in, out,
-                                                                      st.stamp, epoch, pass, sweep_share, cap);
+                                                                      st.stamp, epoch, pass, sweep_share, cap, 0);
     }
 }
 
@@ -868,8 +871,11 @@ __device__ __forceinline__ bool first_member(const uint32_t *fvis, const uint32_
 // A lazy first round that left live vertices: the labels and the members'
 // class words the eager round would have written (members: the pivot,
 // non-final; every other vertex: itself, final), before the later phases.
+// Trimmed vertices (no live bit) get class DEAD too; live non-members keep
+// the class the update gave them.
 __global__ void label_fixup_kernel(uint32_t *cls, uint32_t *lab, const uint32_t *__restrict__ fvis,
-                                   const uint32_t *__restrict__ bvis, const uint32_t *__restrict__ ctr, uint32_t V)
+                                   const uint32_t *__restrict__ bvis, const uint32_t *__restrict__ live_bits,
+                                   const uint32_t *__restrict__ ctr, uint32_t V)
 {
     const uint32_t p0 = ctr[CT(C_PIV0)];
     for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x + 1; v <= V; v += gridDim.x * blockDim.x) {
@@ -878,13 +884,15 @@ __global__ void label_fixup_kernel(uint32_t *cls, uint32_t *lab, const uint32_t 
             cls[v] = DEAD;
         } else {
             lab[v] = v | LAB_FINAL;
+            if (!((live_bits[v >> 5] >> (v & 31)) & 1u)) cls[v] = DEAD;
         }
     }
 }
 
-void k_label_fixup(hipStream_t s, const DevGraph &g, DevState &st)
+void k_label_fixup(hipStream_t s, const DevGraph &g, DevState &st, const uint32_t *live_bits)
 {
-    label_fixup_kernel<<<sweep_blocks(g.V), SWEEP_BLOCK, 0, s>>>(st.cls, st.lab, st.fvis, st.bvis, st.ctr, g.V);
+    label_fixup_kernel<<<sweep_blocks(g.V), SWEEP_BLOCK, 0, s>>>(st.cls, st.lab, st.fvis, st.bvis, live_bits,
+                                                                 st.ctr, g.V);
 }
 
 // The next round's classes are 3p+k for this round's pivots p: clear them.

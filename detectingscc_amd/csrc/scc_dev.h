@@ -329,7 +329,8 @@ void k_init_state(hipStream_t s, const DevGraph &g, DevState &st, const uint8_t 
 // itself, so the fresh state's per-vertex words need not be written before.
 void k_trim1(hipStream_t s, const DevGraph &g, DevState &st, const uint32_t *in, uint32_t *out,
              uint32_t epoch, uint32_t pass, uint32_t sweep_share, uint32_t *live, bool degree_first,
-             bool labels = true);   // labels = false: the degree pass leaves the labels unwritten
+             bool labels = true);   // labels = false (single class only): no label or class word is
+                                    // written, the live bitmap alone tracks removals
 // Largest number of vertices the workgroups b with b % kSpread == j own in
 // a pass over 64-vertex chunks where wave w of `grid` 4-wave workgroups
 // takes chunks w, w + 4 * grid, ...: the size of slot j's list region.
@@ -358,8 +359,8 @@ void k_update(hipStream_t s, const DevGraph &g, DevState &st, bool scc_min = fal
 // the search's bitmaps (gated: a no-op unless C_LIVE is 0).
 void k_label_from_bits(hipStream_t s, const DevGraph &g, DevState &st);
 // After a lazy first round that left live vertices: the labels and class
-// words an eager first round would have left.
-void k_label_fixup(hipStream_t s, const DevGraph &g, DevState &st);
+// words an eager first round would have left (live_bits: the trim's bitmap).
+void k_label_fixup(hipStream_t s, const DevGraph &g, DevState &st, const uint32_t *live_bits);
 void k_clear_piv(hipStream_t s, DevState &st, uint32_t max_pivots);
 void k_reach_stats(hipStream_t s, const DevGraph &g, DevState &st);
 void k_minlabel_init(hipStream_t s, const DevGraph &g, DevState &st);
