@@ -686,7 +686,8 @@ scc_status Engine::pivot_round(bool first, int expand, int W, scc_stats *stats, 
     // which zeroed C_LIVE, and has one pivot at most
     if (!first) zero_ctr(C_LIVE, 1);
     // the first round's update also takes its SCC's minimum (one pivot)
-    k_update(stream, g, st, first, first && use_live_bits ? live_bits() : nullptr, lazy);
+    if (lazy && use_live_bits) k_update_words(stream, g, st, live_bits());
+    else k_update(stream, g, st, first, first && use_live_bits ? live_bits() : nullptr, lazy);
     if (!first) k_clear_piv(stream, st, g.V + 1);   // the first round's update clears its pivot's entries
     if (canon >= 0) {
         // the labels of a decomposition this round finished, in the same

@@ -857,6 +857,70 @@ void k_update(hipStream_t s, const DevGraph &g, DevState &st, bool scc_min, cons
                                                             live_bits, lazy ? 1 : 0);
 }
 
+// The lazy first round's update a 32-vertex group per lane: with one class
+// (0) and one pivot p0 = C_PIV0, the pivot's SCC is the live vertices
+// visited both ways (member = live & fw & bw, bitwise), every other live
+// vertex gets class 3*p0 + fw + 2*bw (the only per-vertex writes), and the
+// counts and the SCC's smallest vertex come from popcounts and the lowest
+// set bit.  Members and trimmed vertices keep their words, as in the lazy
+// update_kernel.  Also clears p0's next-round pivot entries.
+__global__ __launch_bounds__(256) void update_words_kernel(uint32_t *cls, uint64_t *piv,
+                                                           const uint32_t *__restrict__ fvis,
+                                                           const uint32_t *__restrict__ bvis,
+                                                           const uint32_t *__restrict__ live_bits, uint32_t *ctr,
+                                                           uint32_t V, uint32_t *scc_min)
+{
+    const uint32_t p0 = ctr[CT(C_PIV0)];
+    if (blockIdx.x == 0 && threadIdx.x < 3 && p0 >= 1 && p0 <= V) piv[3ull * p0 + threadIdx.x] = 0;
+    uint32_t live = 0, found = 0, mmin = DEAD;
+    const uint32_t ngroups = (V >> 5) + 1;
+    for (uint32_t gi = blockIdx.x * blockDim.x + threadIdx.x; gi < ngroups; gi += gridDim.x * blockDim.x) {
+        const uint32_t lw = live_bits[gi];
+        if (!lw) continue;
+        const uint64_t f = reinterpret_cast<const uint64_t *>(fvis)[gi];
+        const uint64_t b = reinterpret_cast<const uint64_t *>(bvis)[gi];
+        const uint32_t fv = (uint32_t)f & lw, bv = (uint32_t)b & lw;   // live vertices are not finished
+        const uint32_t member = fv & bv;
+        uint32_t rest = lw & ~member;
+        live += (uint32_t)__popc(rest);
+        if (member) {
+            const uint32_t m = (gi << 5) + (uint32_t)(__ffs(member) - 1);
+            mmin = m < mmin ? m : mmin;
+            if ((p0 >> 5) == gi) found += (member >> (p0 & 31)) & 1u;
+        }
+        while (rest) {
+            const uint32_t bit = (uint32_t)(__ffs(rest) - 1);
+            rest &= rest - 1;
+            cls[(gi << 5) + bit] = 3u * p0 + ((fv >> bit) & 1u) + 2u * ((bv >> bit) & 1u);
+        }
+    }
+    spread_add32(ctr, C_LIVE, live);
+    spread_add32(ctr, C_PIVOT, found);
+    __shared__ uint32_t s_min[4];
+    for (int o = 32; o > 0; o >>= 1) {
+        const uint32_t x = __shfl_xor(mmin, o);
+        mmin = x < mmin ? x : mmin;
+    }
+    if (lane() == 0) s_min[threadIdx.x >> 6] = mmin;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t m = DEAD;
+        for (uint32_t w = 0; w < (blockDim.x >> 6); w++) m = s_min[w] < m ? s_min[w] : m;
+        if (m != DEAD && p0 >= 1 && p0 <= V &&
+            m < __hip_atomic_load(&scc_min[p0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+            atomicMin(&scc_min[p0], m);
+    }
+}
+
+void k_update_words(hipStream_t s, const DevGraph &g, DevState &st, const uint32_t *live_bits)
+{
+    const uint64_t ngroups = ((uint64_t)g.V >> 5) + 1;
+    uint64_t b = (ngroups + SWEEP_BLOCK - 1) / SWEEP_BLOCK;
+    b = b < 1 ? 1 : (b > 1024 ? 1024 : b);
+    update_words_kernel<<<(uint32_t)b, SWEEP_BLOCK, 0, s>>>(st.cls, st.piv, st.fvis, st.bvis, live_bits, st.ctr,
+                                                            g.V, st.aux);
+}
+
 // Members of the first pivot's SCC after a lazy first round: visited by
 // both searches and not finished when the bitmaps were initialised (each
 // bitmap's second word per group holds the finished bits).

@@ -355,6 +355,10 @@ void k_select(hipStream_t s, const DevGraph &g, DevState &st, int seed_bw);
 // k_label_from_bits (nothing left live) or k_label_fixup writes
 void k_update(hipStream_t s, const DevGraph &g, DevState &st, bool scc_min = false,
               const uint32_t *live_bits = nullptr, bool lazy = false);
+// The lazy first round's update (one class, one pivot C_PIV0, live_bits
+// exact) a 32-vertex group per lane; same results as k_update(scc_min,
+// live_bits, lazy) including the pivot-entry clearing.
+void k_update_words(hipStream_t s, const DevGraph &g, DevState &st, const uint32_t *live_bits);
 // Canonical labels of a decomposition that a lazy first round finished, from
 // the search's bitmaps (gated: a no-op unless C_LIVE is 0).
 void k_label_from_bits(hipStream_t s, const DevGraph &g, DevState &st);
