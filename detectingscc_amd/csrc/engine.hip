@@ -46,7 +46,7 @@ void Engine::release()
     auto f = [](void *p) {
         if (p) (void)hipFree(p);
     };
-    f(g.Fr); f(g.Fc); f(g.Br); f(g.Bc); f(g.F1); f(g.B1);
+    f(g.Fr); f(g.Fc); f(g.Br); f(g.Bc); f(g.F1); f(g.B1); f(g.DP);
     f(st.cls); f(st.lab); f(st.aux); f(st.fvis); f(st.bvis); f(st.piv);
     for (int i = 0; i < 2; i++) {
         f(st.qf[i]); f(st.qb[i]);
@@ -124,7 +124,7 @@ scc_status Engine::reserve(uint32_t V, uint32_t E)
     auto f = [](void *p) {
         if (p) (void)hipFree(p);
     };
-    f(g.Fr); f(g.Fc); f(g.Br); f(g.Bc); f(g.F1); f(g.B1);
+    f(g.Fr); f(g.Fc); f(g.Br); f(g.Bc); f(g.F1); f(g.B1); f(g.DP);
     f(st.cls); f(st.lab); f(st.aux); f(st.fvis); f(st.bvis); f(st.piv);
     for (int i = 0; i < 2; i++) {
         f(st.qf[i]); f(st.qb[i]);
@@ -146,7 +146,7 @@ scc_status Engine::reserve(uint32_t V, uint32_t E)
     };
     A(dalloc(&g.Fr, nv + 1)); A(dalloc(&g.Br, nv + 1));
     A(dalloc(&g.Fc, nE)); A(dalloc(&g.Bc, nE));
-    A(dalloc(&g.F1, nv)); A(dalloc(&g.B1, nv));
+    A(dalloc(&g.F1, nv)); A(dalloc(&g.B1, nv)); A(dalloc(&g.DP, nv));
     A(dalloc(&st.cls, nv)); A(dalloc(&st.lab, nv)); A(dalloc(&st.aux, nv));
     A(dalloc(&st.fvis, 2 * nwords)); A(dalloc(&st.bvis, 2 * nwords));
     A(dalloc(&st.piv, 3 * nv + 3));

@@ -100,18 +100,20 @@ void k_fill_u32(hipStream_t s, uint32_t *p, uint32_t val, uint64_t n)
 
 __global__ void first_arcs_kernel(const uint32_t *__restrict__ Fr, const uint32_t *__restrict__ Fc,
                                   const uint32_t *__restrict__ Br, const uint32_t *__restrict__ Bc, uint32_t *F1,
-                                  uint32_t *B1, uint32_t V)
+                                  uint32_t *B1, uint32_t *DP, uint32_t V)
 {
     for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v <= V; v += gridDim.x * blockDim.x) {
         const uint32_t fb = Fr[v], fe = Fr[v + 1], bb = Br[v], be = Br[v + 1];
         F1[v] = fb < fe ? Fc[fb] : 0u;
         B1[v] = bb < be ? Bc[bb] : 0u;
+        const uint64_t p = (uint64_t)(fe - fb) * (uint64_t)(be - bb);
+        DP[v] = p > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)p;
     }
 }
 
 void k_first_arcs(hipStream_t s, const DevGraph &g)
 {
-    first_arcs_kernel<<<sweep_blocks(g.V), SWEEP_BLOCK, 0, s>>>(g.Fr, g.Fc, g.Br, g.Bc, g.F1, g.B1, g.V);
+    first_arcs_kernel<<<sweep_blocks(g.V), SWEEP_BLOCK, 0, s>>>(g.Fr, g.Fc, g.Br, g.Bc, g.F1, g.B1, g.DP, g.V);
 }
 
 // All vertices start live in class 0 (range 0, scc_WCC.cu:630-631) unless
@@ -527,8 +529,7 @@ constexpr uint32_t VIS_CHUNKS = 4;
 template <bool ELECT>
 __global__ __launch_bounds__(256) void init_vis_kernel(const uint32_t *__restrict__ cls,
                                                        const uint32_t *__restrict__ live_bits,
-                                                       const uint32_t *__restrict__ Fr,
-                                                       const uint32_t *__restrict__ Br, uint32_t *fvis,
+                                                       const uint32_t *__restrict__ DP, uint32_t *fvis,
                                                        uint32_t *bvis, uint64_t *bred, uint32_t V,
                                                        uint32_t nchunks)
 {
@@ -539,7 +540,7 @@ __global__ __launch_bounds__(256) void init_vis_kernel(const uint32_t *__restric
     uint64_t best = 0;
     // VIS_CHUNKS chunks per step keep that many loads in flight per wave
     for (uint32_t ch0 = wave; ch0 < nchunks; ch0 += VIS_CHUNKS * nwaves) {
-        uint32_t c[VIS_CHUNKS], f0[VIS_CHUNKS], f1[VIS_CHUNKS], b0[VIS_CHUNKS], b1[VIS_CHUNKS];
+        uint32_t c[VIS_CHUNKS], dp[VIS_CHUNKS];
 #pragma unroll
         for (uint32_t h = 0; h < VIS_CHUNKS; h++) {
             const uint32_t ch = ch0 + h * nwaves, v = ch * 64 + lane();
@@ -548,12 +549,7 @@ __global__ __launch_bounds__(256) void init_vis_kernel(const uint32_t *__restric
                 c[h] = in && ((live_bits[v >> 5] >> (v & 31)) & 1u) ? 0u : DEAD;
             else
                 c[h] = in ? cls[v] : DEAD;
-            if (ELECT) {
-                f0[h] = in ? Fr[v] : 0u;
-                f1[h] = in ? Fr[v + 1] : 0u;
-                b0[h] = in ? Br[v] : 0u;
-                b1[h] = in ? Br[v + 1] : 0u;
-            }
+            if (ELECT) dp[h] = in ? DP[v] : 0u;
         }
 #pragma unroll
         for (uint32_t h = 0; h < VIS_CHUNKS; h++) {
@@ -561,7 +557,7 @@ __global__ __launch_bounds__(256) void init_vis_kernel(const uint32_t *__restric
             const bool dead = c[h] == DEAD;
             live += !dead;
             if (ELECT && !dead) {
-                const uint64_t k = pivot_key_deg(f1[h] - f0[h], b1[h] - b0[h], v);
+                const uint64_t k = ((uint64_t)dp[h] << 32) | (uint64_t)(0xFFFFFFFFu - v);   // pivot_key
                 best = k > best ? k : best;
             }
             const uint64_t m = __ballot(dead);
@@ -651,12 +647,12 @@ void k_init_vis(hipStream_t s, const DevGraph &g, DevState &st, bool elect_first
     uint32_t blocks = (nchunks + 3) / 4;
     blocks = blocks < 1 ? 1 : (blocks > VIS_BLOCKS ? VIS_BLOCKS : blocks);
     if (elect_first) {
-        init_vis_kernel<true><<<blocks, SWEEP_BLOCK, 0, s>>>(st.cls, live_bits, g.Fr, g.Br, st.fvis, st.bvis,
+        init_vis_kernel<true><<<blocks, SWEEP_BLOCK, 0, s>>>(st.cls, live_bits, g.DP, st.fvis, st.bvis,
                                                              st.bred, g.V, nchunks);
         vis_finish_kernel<true><<<1, 1024, 0, s>>>(st.bred, blocks, st.ctr, st.piv, st.fvis, st.bvis, st.qf[0],
                                                    st.qb[0], st.pivlist, st.aux);
     } else {
-        init_vis_kernel<false><<<blocks, SWEEP_BLOCK, 0, s>>>(st.cls, live_bits, g.Fr, g.Br, st.fvis, st.bvis,
+        init_vis_kernel<false><<<blocks, SWEEP_BLOCK, 0, s>>>(st.cls, live_bits, g.DP, st.fvis, st.bvis,
                                                               st.bred, g.V, nchunks);
         vis_finish_kernel<false><<<1, 1024, 0, s>>>(st.bred, blocks, st.ctr, st.piv, st.fvis, st.bvis, st.qf[0],
                                                     st.qb[0], st.pivlist, st.aux);

@@ -150,6 +150,9 @@ struct DevGraph {
     // probes that read only a list's first entry (trim-1, bottom-up), which
     // would otherwise fetch a 64-byte line at a random address per vertex
     uint32_t *F1 = nullptr, *B1 = nullptr;
+    // DP[v] = min(outdeg(v) * indeg(v), 2^32 - 1): the high word of v's pivot
+    // key, one word per vertex instead of four row offsets
+    uint32_t *DP = nullptr;
 };
 
 struct DevState {
@@ -307,7 +310,7 @@ __host__ __forceinline__ uint32_t dir_blocks(uint32_t bpd, int ndirs)
 
 // ---- sweep kernels (kernels_sweep.hip) ----
 void k_fill_u32(hipStream_t s, uint32_t *p, uint32_t val, uint64_t n);
-// g.F1 and g.B1 from the CSR (after an upload)
+// g.F1, g.B1 and g.DP from the CSR (after an upload)
 void k_first_arcs(hipStream_t s, const DevGraph &g);
 void k_init_state(hipStream_t s, const DevGraph &g, DevState &st, const uint8_t *dead_dev,
                   bool vertices = true);
