@@ -352,12 +352,12 @@ __global__ __launch_bounds__(256) void trim1_kernel(const uint32_t *__restrict__
 // the first round finishes computes every label from the search's bitmaps
 // (label_from_bits_kernel), and otherwise label_fixup_kernel writes the
 // labels and class words the eager passes would have.
-// Pure streaming of the row offsets, so each wave keeps TRIM_DEG_CHUNKS
-// 64-vertex chunks' loads in flight.
+// Pure streaming of the degree products (DP[v] == 0: no out-arc or no
+// in-arc), so each wave keeps TRIM_DEG_CHUNKS 64-vertex chunks' loads in
+// flight.
 constexpr uint32_t TRIM_DEG_CHUNKS = 4;
 
-__global__ __launch_bounds__(256) void trim1_degree_kernel(const uint32_t *__restrict__ Fr,
-                                                           const uint32_t *__restrict__ Br, uint32_t *cls,
+__global__ __launch_bounds__(256) void trim1_degree_kernel(const uint32_t *__restrict__ DP, uint32_t *cls,
                                                            uint32_t *lab, uint32_t *live, uint32_t V,
                                                            uint32_t *ctr, uint32_t *out, uint32_t cap,
                                                            int write_labels)
@@ -374,21 +374,17 @@ __global__ __launch_bounds__(256) void trim1_degree_kernel(const uint32_t *__res
     const uint32_t stride = gridDim.x * 256;
     for (uint32_t base = (blockIdx.x * 4 + wv) * 64; base <= V; base += TRIM_DEG_CHUNKS * stride) {
         if (nq + TRIM_DEG_CHUNKS * 64 > TRIM_WQ) wave_flush(wq, nq, cnt_out, qout, cap, anom);
-        uint32_t v[TRIM_DEG_CHUNKS], f0[TRIM_DEG_CHUNKS], f1[TRIM_DEG_CHUNKS], b0[TRIM_DEG_CHUNKS],
-            b1[TRIM_DEG_CHUNKS];
+        uint32_t v[TRIM_DEG_CHUNKS], dp[TRIM_DEG_CHUNKS];
 #pragma unroll
         for (uint32_t h = 0; h < TRIM_DEG_CHUNKS; h++) {
             v[h] = base + h * stride + lane();
             const bool in = v[h] <= V;
-            f0[h] = in ? Fr[v[h]] : 0u;
-            f1[h] = in ? Fr[v[h] + 1] : 0u;
-            b0[h] = in ? Br[v[h]] : 0u;
-            b1[h] = in ? Br[v[h] + 1] : 0u;
+            dp[h] = in ? DP[v[h]] : 0u;   // 0: no out-arc or no in-arc
         }
 #pragma unroll
         for (uint32_t h = 0; h < TRIM_DEG_CHUNKS; h++) {
             const bool alive = v[h] >= 1 && v[h] <= V;   // vertex 0, the dummy, is dead
-            const bool gone = alive && (f0[h] == f1[h] || b0[h] == b1[h]);
+            const bool gone = alive && dp[h] == 0;
             if (alive && write_labels) {
                 cls[v[h]] = gone ? DEAD : 0u;
                 lab[v[h]] = v[h] | LAB_FINAL;
@@ -420,7 +416,7 @@ void k_trim1(hipStream_t s, const DevGraph &g, DevState &st, const uint32_t *in,
 {
     const uint32_t cap = trim_slot_cap(g.V);
     if (live && pass == 0 && degree_first) {
-        trim1_degree_kernel<<<sweep_blocks(g.V), SWEEP_BLOCK, 0, s>>>(g.Fr, g.Br, st.cls, st.lab, live, g.V,
+        trim1_degree_kernel<<<sweep_blocks(g.V), SWEEP_BLOCK, 0, s>>>(g.DP, st.cls, st.lab, live, g.V,
                                                                       st.ctr, out, cap, labels ? 1 : 0);
     } else if (live) {
         if (pass == 0)
