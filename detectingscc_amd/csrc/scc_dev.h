@@ -328,8 +328,9 @@ void k_init_state(hipStream_t s, const DevGraph &g, DevState &st, const uint8_t 
 // pass 0 builds it from the class words.  With degree_first (a fixpoint
 // from a fresh state: no vertex dead, all in class 0, labels v | LAB_FINAL)
 // pass 0 removes only vertices without an in- or out-arc, since every
-// neighbour is live then, and writes every vertex's class word and label
-// itself, so the fresh state's per-vertex words need not be written before.
+// neighbour is live then (it reads each vertex's degree product), and
+// writes every vertex's class word and label itself (unless `labels` is
+// false), so the fresh state's per-vertex words need not be written before.
 void k_trim1(hipStream_t s, const DevGraph &g, DevState &st, const uint32_t *in, uint32_t *out,
              uint32_t epoch, uint32_t pass, uint32_t sweep_share, uint32_t *live, bool degree_first,
              bool labels = true);   // labels = false (single class only): no label or class word is
