@@ -88,6 +88,7 @@ class Engine {
     scc_status fetch32(uint32_t *out, const uint32_t *src);
 
     uint32_t capV = 0, capE = 0;
+    size_t qf_words = 0, qb_words = 0;   // allocated words of each FW / BW queue
     uint32_t *h_ctr = nullptr;        // pinned mirror of st.ctr
     uint32_t *h_pub = nullptr;        // mapped coherent host memory the GPU publishes counters into (2 buffers)
     uint32_t *d_pub = nullptr;        // its device address

@@ -118,7 +118,11 @@ template <class T> static hipError_t dalloc(T **p, size_t n)
 scc_status Engine::reserve(uint32_t V, uint32_t E)
 {
     HIP_TRY(hipSetDevice(device));
-    if (V <= capV && E <= capE) return SCC_OK;
+    // the queues' slot regions depend on V through the grid sizes: a smaller
+    // graph in a larger context must fit the queues as allocated (they grow
+    // with V, and this keeps it so)
+    const bool queues_fit = std::max(queue_words(V), trim_list_words(V)) <= qf_words && queue_words(V) <= qb_words;
+    if (V <= capV && E <= capE && queues_fit) return SCC_OK;
     uint32_t nV = V > capV ? V : capV, nE = E > capE ? E : capE;
     // keep the context, drop the arrays
     auto f = [](void *p) {
@@ -150,8 +154,12 @@ scc_status Engine::reserve(uint32_t V, uint32_t E)
     A(dalloc(&st.cls, nv)); A(dalloc(&st.lab, nv)); A(dalloc(&st.aux, nv));
     A(dalloc(&st.fvis, 2 * nwords)); A(dalloc(&st.bvis, 2 * nwords));
     A(dalloc(&st.piv, 3 * nv + 3));
-    for (int i = 0; i < 2; i++) {   // the FW queues also hold the trim-1 lists
-        A(dalloc(&st.qf[i], std::max(queue_words(nV), trim_list_words(nV)))); A(dalloc(&st.qb[i], queue_words(nV)));
+    // the FW queues also hold the trim-1 lists; sized for the largest of
+    // nV and V (slot regions follow the grid sizes of the graph uploaded)
+    qf_words = std::max({queue_words(nV), trim_list_words(nV), queue_words(V), trim_list_words(V)});
+    qb_words = std::max(queue_words(nV), queue_words(V));
+    for (int i = 0; i < 2; i++) {
+        A(dalloc(&st.qf[i], qf_words)); A(dalloc(&st.qb[i], qb_words));
     }
     // a hub list holds vertices of degree >= LB_BIG only
     const size_t nhub = std::min<size_t>(nv, (size_t)nE / LB_BIG + 1);

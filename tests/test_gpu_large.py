@@ -43,3 +43,22 @@ def test_rmat21_slota_and_vertex_mapping(rmat21):
             st = ctx.run(algo=algo, expand=expand, sizes=False)
             assert_same_labels(ctx.labels(), want, f"rmat21 -a {algo} expand {expand}")
             assert st.num_sccs == n
+
+
+def test_one_context_many_sizes():
+    """Graphs of decreasing and increasing size in one context: the queues'
+    slot regions follow each graph's grid sizes and must fit the buffers
+    the context keeps (sizes around the 1,024-workgroup boundary)."""
+    graphs = [scc.gen_er(300000, 600000, cycles=10, cmin=2, cmax=500, seed=1),
+              scc.gen_er(262143, 520000, cycles=10, cmin=2, cmax=500, seed=2),
+              scc.gen_rmat(14, seed=5),
+              scc.gen_er(65537, 200000, cycles=10, cmin=2, cmax=500, seed=3),
+              scc.gen_er(262145, 520000, cycles=10, cmin=2, cmax=500, seed=4)]
+    with scc.Context(0) as ctx:
+        for i, g in enumerate(graphs):
+            want, n = orc.tarjan(g)
+            ctx.upload(g)
+            for sizes in (False, True):
+                st = ctx.run(algo="g", expand=scc.EXPAND_DO, sizes=sizes)
+                assert_same_labels(ctx.labels(), want, f"graph {i} V={g.V} sizes={sizes}")
+                assert st.num_sccs == n
