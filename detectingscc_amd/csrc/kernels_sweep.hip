@@ -520,7 +520,7 @@ __device__ __forceinline__ void zero_hub_counts(uint32_t *ctr)
 // its vertex, so no second sweep is needed to find it.  Each workgroup
 // leaves its count and key in `bred`; vis_finish_kernel reduces them with
 // one workgroup (thousands of same-address atomics would serialise).
-constexpr uint32_t VIS_CHUNKS = 4;
+constexpr uint32_t VIS_CHUNKS = 8;
 
 template <bool ELECT>
 __global__ __launch_bounds__(256) void init_vis_kernel(const uint32_t *__restrict__ cls,
@@ -1197,7 +1197,7 @@ constexpr int MATCH_ROUNDS = 8;
 // of its wave, with all their label loads in flight together (then, in the
 // apply pass, all their representative loads): one load per step leaves
 // these passes waiting on memory latency.
-constexpr uint32_t CANON_CHUNKS = 4;
+constexpr uint32_t CANON_CHUNKS = 8;
 
 // Like the size counts below, each wave keeps a running minimum for one
 // label several of its lanes shared and writes it once at the end.
