@@ -617,8 +617,13 @@ def scc(g: Graph, algo: str = "g", trim1: bool = True, trim2: bool = True, warp:
 
 # -- drop-in driver names (include/scc.h:49-53); they print the reference's lines --
 def _dropin(name, *args):
-    fn = getattr(lib(), name)
-    return fn(*args)
+    """Calls the C-linkage driver scc_<name>; returns its status (0 = ok).
+    A failure's message is printed, as the reference's main prints what the
+    drivers throw (src/scc.cu:104-108)."""
+    rc = getattr(lib(), "scc_" + name)(*args)
+    if rc != 0:
+        print(lib().scc_last_error().decode(), flush=True)
+    return rc
 
 
 def vHong(CSize, RSize, Fc, Fr, Bc, Br, t1, t2) -> int:

@@ -30,7 +30,7 @@ CXX = os.environ.get("CXX", "g++")
 
 HIP_SOURCES = ["kernels_sweep.hip", "kernels_reach.hip", "kernels_merge.hip", "kernels_batch.hip",
                "engine.hip", "batch_engine.hip", "server.hip", "capi.hip", "measure.hip"]
-HOST_SOURCES = ["graph_io.cpp", "generators.cpp"]
+HOST_SOURCES = ["graph_io.cpp", "generators.cpp", "dropin.cpp"]
 
 HIP_FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-fPIC", "-std=c++17", "-Wall", "-fvisibility=hidden",

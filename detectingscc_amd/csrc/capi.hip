@@ -1,6 +1,7 @@
 // capi.hip -- extern "C" entry points of libscc_amd.so: context management,
-// the device engine, phase entries, and the drop-in wrappers with the
-// reference driver names (include/scc.h:49-53, include/load.h:6).
+// the device engine, phase entries, and the drop-in drivers with the
+// reference's parameters (include/scc.h:49-53) under scc_ names; dropin.cpp
+// gives them the reference's own C++ names.
 #include <chrono>
 #include <cstdio>
 #include <cstring>
@@ -176,7 +177,7 @@ static scc_ctx *g_dropin_ctx = nullptr;
 // result, then the stdout lines.  "Time" covers the same span as the
 // reference's cudaEvent pair (scc_WCC.cu:577-581 -> 951-953): it starts
 // before device allocation and ends after the count.
-static int dropin(char algo, uint32_t CSize, uint32_t RSize, uint32_t *Fc, uint32_t *Fr,
+static scc_status dropin(char algo, uint32_t CSize, uint32_t RSize, uint32_t *Fc, uint32_t *Fr,
                   uint32_t *Bc, uint32_t *Br, uint32_t *Pr, bool t1, bool t2, int warp)
 {
     std::lock_guard<std::mutex> lk(g_dropin_mu);
@@ -199,10 +200,7 @@ static int dropin(char algo, uint32_t CSize, uint32_t RSize, uint32_t *Fc, uint3
         o.warp = warp;
         s = g_dropin_ctx->eng.run(o, &st);
     }
-    if (s != SCC_OK) {
-        printf("%s\n", scc_last_error());
-        return (int)s;
-    }
+    if (s != SCC_OK) return s;   // the message stays in scc_last_error()
     auto t1c = std::chrono::steady_clock::now();
     double ms = std::chrono::duration<double, std::milli>(t1c - t0).count();
     if (algo == 'g') printf("numberOf1Sccs: %u\n", st.num_trim1);
@@ -215,69 +213,37 @@ static int dropin(char algo, uint32_t CSize, uint32_t RSize, uint32_t *Fc, uint3
     printf("\nTime : %f", ms);
     if (algo == 'd') printf("\n");
     fflush(stdout);
-    return 0;
+    return SCC_OK;
 }
 
-int vHong(uint32_t CSize, uint32_t RSize, uint32_t *Fc, uint32_t *Fr, uint32_t *Bc, uint32_t *Br,
+scc_status scc_vHong(uint32_t CSize, uint32_t RSize, uint32_t *Fc, uint32_t *Fr, uint32_t *Bc, uint32_t *Br,
           bool t1, bool t2)
 {
     return dropin('g', CSize, RSize, Fc, Fr, Bc, Br, nullptr, t1, t2, 1);
 }
 
-int wHong(uint32_t CSize, uint32_t RSize, uint32_t *Fc, uint32_t *Fr, uint32_t *Bc, uint32_t *Br,
+scc_status scc_wHong(uint32_t CSize, uint32_t RSize, uint32_t *Fc, uint32_t *Fr, uint32_t *Bc, uint32_t *Br,
           bool t1, bool t2, int warpSize)
 {
     return dropin('x', CSize, RSize, Fc, Fr, Bc, Br, nullptr, t1, t2, warpSize);
 }
 
-int vSlota(uint32_t CSize, uint32_t RSize, uint32_t *Fc, uint32_t *Fr, uint32_t *Bc, uint32_t *Br,
+scc_status scc_vSlota(uint32_t CSize, uint32_t RSize, uint32_t *Fc, uint32_t *Fr, uint32_t *Bc, uint32_t *Br,
            bool t1, bool t2)
 {
     return dropin('h', CSize, RSize, Fc, Fr, Bc, Br, nullptr, t1, t2, 1);
 }
 
-int wSlota(uint32_t CSize, uint32_t RSize, uint32_t *Fc, uint32_t *Fr, uint32_t *Bc, uint32_t *Br,
+scc_status scc_wSlota(uint32_t CSize, uint32_t RSize, uint32_t *Fc, uint32_t *Fr, uint32_t *Bc, uint32_t *Br,
            bool t1, bool t2, int warpSize)
 {
     return dropin('y', CSize, RSize, Fc, Fr, Bc, Br, nullptr, t1, t2, warpSize);
 }
 
-int detectSCC(uint32_t CSize, uint32_t RSize, uint32_t *Fc, uint32_t *Fr, uint32_t *Bc,
+scc_status scc_detectSCC(uint32_t CSize, uint32_t RSize, uint32_t *Fc, uint32_t *Fr, uint32_t *Bc,
               uint32_t *Br, uint32_t *Pr, bool t1, bool t2)
 {
     return dropin('d', CSize, RSize, Fc, Fr, Bc, Br, Pr, t1, t2, 1);
-}
-
-void loadFullGraph(const char *filename, uint32_t *oCSize, uint32_t *oRSize, uint32_t **oFc,
-                   uint32_t **oFr, uint32_t **oBc, uint32_t **oBr, uint32_t **oPr)
-{
-    scc_host_graph g;
-    std::memset(&g, 0, sizeof g);
-    scc_status s = scc_load_gtgraph(filename, &g);
-    if (oPr) *oPr = nullptr;
-    if (s != SCC_OK) {
-        printf("Error Reading graph file\n");
-        if (oCSize) *oCSize = 0;
-        if (oRSize) *oRSize = 0;
-        if (oFc) *oFc = nullptr;
-        if (oFr) *oFr = nullptr;
-        if (oBc) *oBc = nullptr;
-        if (oBr) *oBr = nullptr;
-        return;
-    }
-    // the reference's caller releases these with delete[] (scc.cu:110-113)
-    auto dup = [](const uint32_t *p, size_t n) {
-        uint32_t *q = new uint32_t[n ? n : 1];
-        if (n) std::memcpy(q, p, n * sizeof(uint32_t));
-        return q;
-    };
-    if (oCSize) *oCSize = g.E;
-    if (oRSize) *oRSize = g.V;
-    if (oFc) *oFc = dup(g.Fc, g.E);
-    if (oFr) *oFr = dup(g.Fr, (size_t)g.V + 2);
-    if (oBc) *oBc = dup(g.Bc, g.E);
-    if (oBr) *oBr = dup(g.Br, (size_t)g.V + 2);
-    scc_host_graph_free(&g);
 }
 
 } // extern "C"

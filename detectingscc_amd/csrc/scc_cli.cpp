@@ -93,12 +93,14 @@ int main(int argc, char **argv)
     int rc;
     if (expand == SCC_EXPAND_AUTO && !label_file && !dag_file && device == 0) {
         switch (algo) {
-        case 'g': rc = vHong(g.E, g.V, g.Fc, g.Fr, g.Bc, g.Br, trim1, trim2); break;
-        case 'h': rc = vSlota(g.E, g.V, g.Fc, g.Fr, g.Bc, g.Br, trim1, trim2); break;
-        case 'x': rc = wHong(g.E, g.V, g.Fc, g.Fr, g.Bc, g.Br, trim1, trim2, warp); break;
-        case 'y': rc = wSlota(g.E, g.V, g.Fc, g.Fr, g.Bc, g.Br, trim1, trim2, warp); break;
-        default: rc = detectSCC(g.E, g.V, g.Fc, g.Fr, g.Bc, g.Br, part.data(), trim1, trim2); break;
+        case 'g': s = scc_vHong(g.E, g.V, g.Fc, g.Fr, g.Bc, g.Br, trim1, trim2); break;
+        case 'h': s = scc_vSlota(g.E, g.V, g.Fc, g.Fr, g.Bc, g.Br, trim1, trim2); break;
+        case 'x': s = scc_wHong(g.E, g.V, g.Fc, g.Fr, g.Bc, g.Br, trim1, trim2, warp); break;
+        case 'y': s = scc_wSlota(g.E, g.V, g.Fc, g.Fr, g.Bc, g.Br, trim1, trim2, warp); break;
+        default: s = scc_detectSCC(g.E, g.V, g.Fc, g.Fr, g.Bc, g.Br, part.data(), trim1, trim2); break;
         }
+        if (s != SCC_OK) printf("%s\n", scc_last_error());   // scc.cu:104-108
+        rc = s == SCC_OK ? 0 : 1;
     } else {
         scc_ctx *ctx = nullptr;
         scc_opts o;

@@ -331,30 +331,28 @@ scc_status scc_server_stop(scc_ctx *ctx);
  * counters used to profile the kernels (no reference counterpart). */
 scc_status scc_copy_bench(scc_ctx *ctx, uint64_t bytes, int iters, double *ms_per_copy);
 
-/* --------------------------------------------------- drop-in wrappers --- */
+/* --------------------------------------------------- drop-in drivers --- */
 /*
- * The five drivers of include/scc.h:49-53 with the same parameters:
- * CSize = E, RSize = V, reference-layout host arrays.  They run on device 0
- * and print the reference's stdout lines (scc_WCC.cu:948, 966-967;
- * scc_Coloring.cu:669-670; scc_WCC.cu:1576-1577).  They return 0 on
- * success (a caller declaring them void ignores it) and print the error
- * text instead of throwing (scc_WCC.cu:618-623, scc.cu:104-108).
+ * The five drivers of include/scc.h:49-53 with the same parameters and C
+ * linkage (for FFI callers: ctypes, cgo, JNI): CSize = E, RSize = V,
+ * reference-layout host arrays.  They run on device 0 and print the
+ * reference's stdout lines (scc_WCC.cu:948, 966-967; scc_Coloring.cu:669-670;
+ * scc_WCC.cu:1576-1577).  They return SCC_OK or an error status and never
+ * exit or throw (the reference exits on CUDA errors, scc.h:12-20, and throws
+ * on OOM, scc_WCC.cu:618-623).  C++ callers of the reference's own
+ * declarations use include/scc_dropin.h, whose void functions with the
+ * reference names wrap these.
  */
-int vHong(uint32_t CSize, uint32_t RSize, uint32_t *Fc, uint32_t *Fr, uint32_t *Bc, uint32_t *Br,
-          bool t1, bool t2);
-int wHong(uint32_t CSize, uint32_t RSize, uint32_t *Fc, uint32_t *Fr, uint32_t *Bc, uint32_t *Br,
-          bool t1, bool t2, int warpSize);
-int vSlota(uint32_t CSize, uint32_t RSize, uint32_t *Fc, uint32_t *Fr, uint32_t *Bc, uint32_t *Br,
-           bool t1, bool t2);
-int wSlota(uint32_t CSize, uint32_t RSize, uint32_t *Fc, uint32_t *Fr, uint32_t *Bc, uint32_t *Br,
-           bool t1, bool t2, int warpSize);
-int detectSCC(uint32_t CSize, uint32_t RSize, uint32_t *Fc, uint32_t *Fr, uint32_t *Bc,
-              uint32_t *Br, uint32_t *Pr, bool t1, bool t2);
-/* loadFullGraph (include/load.h:6) with the reference parameters; *oPr is
- * set to NULL (no hard-coded partition path); on error it prints
- * "Error Reading graph file" as load.cpp:26 and zeroes the outputs. */
-void loadFullGraph(const char *filename, uint32_t *oCSize, uint32_t *oRSize, uint32_t **oFc,
-                   uint32_t **oFr, uint32_t **oBc, uint32_t **oBr, uint32_t **oPr);
+scc_status scc_vHong(uint32_t CSize, uint32_t RSize, uint32_t *Fc, uint32_t *Fr, uint32_t *Bc,
+                     uint32_t *Br, bool t1, bool t2);
+scc_status scc_wHong(uint32_t CSize, uint32_t RSize, uint32_t *Fc, uint32_t *Fr, uint32_t *Bc,
+                     uint32_t *Br, bool t1, bool t2, int warpSize);
+scc_status scc_vSlota(uint32_t CSize, uint32_t RSize, uint32_t *Fc, uint32_t *Fr, uint32_t *Bc,
+                      uint32_t *Br, bool t1, bool t2);
+scc_status scc_wSlota(uint32_t CSize, uint32_t RSize, uint32_t *Fc, uint32_t *Fr, uint32_t *Bc,
+                      uint32_t *Br, bool t1, bool t2, int warpSize);
+scc_status scc_detectSCC(uint32_t CSize, uint32_t RSize, uint32_t *Fc, uint32_t *Fr, uint32_t *Bc,
+                         uint32_t *Br, uint32_t *Pr, bool t1, bool t2);
 
 #pragma GCC visibility pop
 #ifdef __cplusplus
