@@ -61,7 +61,7 @@ class _Stats(ctypes.Structure):
                 ("first_fw_vertices", ctypes.c_uint64), ("first_fw_arcs", ctypes.c_uint64),
                 ("first_bw_vertices", ctypes.c_uint64), ("first_bw_arcs", ctypes.c_uint64),
                 ("ms_scc", ctypes.c_double), ("ms_first_reach", ctypes.c_double),
-                ("ms_phase", ctypes.c_double * len(PHASES))]
+                ("ms_phase", ctypes.c_double * len(PHASES)), ("first_work", ctypes.c_uint64 * 8)]
 
 
 class _BatchHost(ctypes.Structure):
@@ -271,13 +271,24 @@ class Stats:
     ms_scc: float
     ms_first_reach: float
     ms_phase: dict
+    first_work: tuple = ()
+
+    # bytes of each first-search work unit (scc_stats.first_work)
+    WORK_BYTES = (12.0, 8.0, 4.0, 0.125, 4.0, 8.0, 8.0, 12.0)
+
+    @property
+    def first_work_bytes(self) -> float:
+        """Algorithmic bytes of the work the first search did (counted on the
+        device in a run with sizes and direction-optimised expansion)."""
+        return float(sum(n * b for n, b in zip(self.first_work, self.WORK_BYTES)))
 
 
 def _stats(s: _Stats) -> Stats:
     return Stats(s.num_sccs, s.num_trim1, s.num_trim2, s.num_pivot, s.largest, s.rounds, s.levels,
                  s.first_pivot, s.first_fw_vertices, s.first_fw_arcs, s.first_bw_vertices,
                  s.first_bw_arcs, s.ms_scc, s.ms_first_reach,
-                 {PHASES[i]: float(s.ms_phase[i]) for i in range(len(PHASES))})
+                 {PHASES[i]: float(s.ms_phase[i]) for i in range(len(PHASES))},
+                 tuple(int(x) for x in s.first_work))
 
 
 def make_opts(algo: str = "g", trim1: bool = True, trim2: bool = True, warp: int = 1,
@@ -460,6 +471,21 @@ class Context:
         lab = np.zeros(self.graph.V + 1, np.uint32)
         _check(lib().scc_labels(self._ctx, _p32(lab)))
         return lab
+
+    def histogram(self) -> tuple[list[tuple[int, int]], float]:
+        """SCC size histogram of the last run computed on the device:
+        [(size, count), ...] ascending by size, and the device ms."""
+        V = self.graph.V
+        cap = 2
+        while cap * (cap - 1) // 2 <= V:
+            cap += 1
+        pairs = np.zeros(2 * cap, np.uint64)
+        n = ctypes.c_uint32(0)
+        ms = ctypes.c_double(0)
+        _check(lib().scc_histogram(self._ctx, pairs.ctypes.data_as(_u64p), ctypes.c_uint32(cap),
+                                   ctypes.byref(n), ctypes.byref(ms)))
+        k = min(n.value, cap)
+        return [(int(pairs[2 * i]), int(pairs[2 * i + 1])) for i in range(k)], ms.value
 
     def condensation(self) -> tuple[Graph, np.ndarray]:
         """The DAG of SCCs of the last run (node k = k-th smallest canonical

@@ -152,6 +152,14 @@ scc_status scc_phase_min_label(scc_ctx *ctx, const uint8_t *dead, uint32_t *out)
     return ctx->eng.phase_min_label(dead, out);
 }
 
+scc_status scc_histogram(scc_ctx *ctx, uint64_t *pairs, uint32_t cap, uint32_t *npairs, double *ms)
+{
+    clear_error();
+    CTX_CHECK(ctx);
+    if (!npairs || (cap && !pairs)) return fail(SCC_ERR_INVALID, "null output");
+    return ctx->eng.histogram(pairs, cap, npairs, ms);
+}
+
 scc_status scc_phase_local(scc_ctx *ctx, uint32_t *out)
 {
     clear_error();

@@ -24,6 +24,9 @@ class Engine {
     scc_status run(const scc_opts &o, scc_stats *out);
     scc_status phase_local(uint32_t *out);
     scc_status labels(uint32_t *lab);
+    // size histogram of the last run's labels, computed on the device:
+    // (size, count) pairs ascending by size
+    scc_status histogram(uint64_t *pairs, uint32_t cap, uint32_t *npairs, double *ms);
     scc_status condense(scc_host_graph *out, uint32_t *node_of);
 
     scc_status phase_trim1(const uint8_t *dead, uint8_t *removed);

@@ -222,7 +222,6 @@ scc_status Engine::upload(uint32_t V, uint32_t E, const uint32_t *Fr, const uint
         HIP_TRY(hipMemcpyAsync(g.Fc, Fc, sizeof(uint32_t) * E, hipMemcpyHostToDevice, stream));
         HIP_TRY(hipMemcpyAsync(g.Bc, Bc, sizeof(uint32_t) * E, hipMemcpyHostToDevice, stream));
     }
-    k_first_arcs(stream, g);
     HIP_TRY(hipStreamSynchronize(stream));
     has_graph = true;
     has_part = false;
@@ -267,7 +266,6 @@ scc_status Engine::upload_device(uint32_t V, uint32_t E, const uint32_t *dFr, co
         HIP_TRY(hipMemcpyAsync(g.Fc, dFc, sizeof(uint32_t) * E, hipMemcpyDeviceToDevice, stream));
         HIP_TRY(hipMemcpyAsync(g.Bc, dBc, sizeof(uint32_t) * E, hipMemcpyDeviceToDevice, stream));
     }
-    k_first_arcs(stream, g);
     has_graph = true;
     has_part = false;
     ran = false;
@@ -524,6 +522,7 @@ scc_status Engine::reach_do(bool fw, bool bw, bool single, uint32_t *levels_out)
     a.V = g.V;
     a.ndirs = nd;
     a.qcap = slot_cap(g.V, full);   // bottom-up levels run on the full grid only
+    a.count = search_stats && single ? 1 : 0;   // the first search of a run with sizes
     for (int i = 0; i < nd; i++) {
         a.PR[i] = dirs[i] ? g.Fr : g.Br;
         a.PC[i] = dirs[i] ? g.Fc : g.Bc;
@@ -541,6 +540,7 @@ scc_status Engine::reach_do(bool fw, bool bw, bool single, uint32_t *levels_out)
     queue(single ? kFirstWindow : kDoWindow, full);
     uint32_t seen = level, seq = publish_ctr();
     int window = 0;   // nothing is queued ahead of the first look
+    bool narrow_ok = true;
     uint32_t blocks = full;
     int64_t nf_prev[2] = {0, 0};
     for (;;) {
@@ -587,11 +587,40 @@ scc_status Engine::reach_do(bool fw, bool bw, bool single, uint32_t *levels_out)
         if (!any) {
             // a window queued meanwhile runs as no-ops; its counters are
             // published into the other buffer, which nothing waits for
-            if (levels_out) *levels_out += end;
+            if (levels_out) *levels_out += end + h_ctr[CT(C_NLEVELS)] + h_ctr[CT(C_NLEVELS) + 1];
             return SCC_OK;
         }
         // every level claims at least one vertex, so V+1 levels bound a search
         if (seen > limit()) return no_convergence("reachability");
+        // Narrow frontiers: every active direction top-down from a top-down
+        // level, no hubs pending, a frontier that fits the workgroup's LDS.
+        // The narrow kernel continues from `level` (everything before it is
+        // queued) and rewrites that level's input; nothing is queued behind
+        // it until the host has read where it stopped.  After a narrow run at
+        // least one window of device-wide levels runs before the next one.
+        bool narrow = narrow_ok;
+        for (int i = 0; i < nd && narrow; i++) {
+            const int d = dirs[i];
+            const uint32_t last = seen - 1;
+            const DirState prev = load_dir_state(h_ctr, last == 0 ? 0 : last & 1, i);
+            if (prev.mode == DIR_OFF) continue;
+            const uint32_t nf = h_ctr[CT((d ? C_BWCNT : C_FWCNT) + seen % 3)];
+            const uint32_t hubs = (uint32_t)(*reinterpret_cast<const unsigned long long *>(
+                                                 h_ctr + CT((d ? C_BIGB : C_BIGF) + last % 3)) >> 32);
+            narrow = prev.mode == DIR_TD && !hubs && nf <= narrow_cap();
+        }
+        narrow_ok = true;
+        if (narrow) {
+            for (int i = 0; i < nd; i++) a.d[i] = dir_args(dirs[i], level);
+            a.level = (int)level;
+            k_narrow_levels(stream, a, single);
+            seen = level;
+            seq = publish_ctr();
+            window = 0;
+            narrow_ok = false;
+            nf_prev[0] = nf_prev[1] = 0;
+            continue;
+        }
         window = spec ? kSpecLevels : kDoWindow;
         blocks = spec ? (uint32_t)std::min<int64_t>(full, std::max<int64_t>(32, (nf_max * 4 + 255) / 256)) : full;
         if (!ahead) {   // after the first look: queue now, ahead from then on
@@ -703,9 +732,14 @@ scc_status Engine::pivot_round(bool first, int expand, int W, scc_stats *stats, 
         // in the first round the only non-final label is the pivot's, whose
         // minimum the update left in aux
         mark(SCC_PHASE_LABEL);
-        if (lazy && canon == 0) k_label_from_bits(stream, g, st);
-        else if (first && canon == 0) k_canon_apply(stream, g, st, true);
-        else k_canon(stream, g, st, canon != 0, true);
+        if (lazy && canon == 0) {
+            k_label_from_bits(stream, g, st);
+        } else if (first) {
+            k_canon_apply(stream, g, st, true);
+            if (canon) k_sizes(stream, g, st, true);
+        } else {
+            k_canon(stream, g, st, canon != 0, true);
+        }
         mark(-1);
     }
     if ((s = read_ctr()) != SCC_OK) return s;
@@ -792,6 +826,11 @@ scc_status Engine::run(const scc_opts &o, scc_stats *out)
     // the first trim-1 pass of Hong and detectSCC starts from the fresh
     // state and writes the per-vertex words itself
     const bool fresh_trim = o.trim1 && (hong || A == 'd');
+    // The first-neighbour and degree-product arrays belong to the
+    // decomposition, not to the upload: every run builds them from the CSR.
+    // The fresh trim's degree pass builds them for the vertices it keeps;
+    // any other path builds them for every vertex first.
+    if (!fresh_trim) k_first_arcs(stream, g);
     k_init_state(stream, g, st, nullptr, !fresh_trim);   // also zeroes the counters
     scc_status s;
     uint32_t live = g.V;
@@ -837,9 +876,12 @@ scc_status Engine::run(const scc_opts &o, scc_stats *out)
         if ((s = color_loop(loop_expand, W, &stats)) != SCC_OK) return s;
     }
     if (!labelled) {
-        if (A != 'd') {   // detectSCC labels come out of the merge already canonical
+        if (A != 'd') {
             mark(SCC_PHASE_LABEL);
             k_canon(stream, g, st, o.sizes != 0);
+        } else if (o.sizes) {   // detectSCC labels come out of the merge already canonical
+            mark(SCC_PHASE_LABEL);
+            k_sizes(stream, g, st);
         }
         mark(-1);
         if ((s = read_ctr()) != SCC_OK) return s;
@@ -862,6 +904,7 @@ scc_status Engine::run(const scc_opts &o, scc_stats *out)
     stats.first_fw_arcs = u64(C_FWE);
     stats.first_bw_vertices = u64(C_BWV);
     stats.first_bw_arcs = u64(C_BWE);
+    for (int k = 0; k < 8; k++) stats.first_work[k] = u64(C_WORK + k);
     ran = true;
     if (out) *out = stats;
     return SCC_OK;
@@ -1041,6 +1084,54 @@ scc_status Engine::labels(uint32_t *lab)
     return SCC_OK;
 }
 
+// The size histogram of the last run, on the device: sizes from the
+// canonical labels (k_sizes), then the per-size counts and their non-zero
+// entries (at most floor(sqrt(2V)) + 1 distinct sizes: the sizes of
+// distinct SCCs sum to at most V).  *ms is the device time of the three
+// passes plus the compaction.
+scc_status Engine::histogram(uint64_t *pairs, uint32_t cap, uint32_t *npairs, double *ms)
+{
+    if (!ran) return fail(SCC_ERR_NOTREADY, "no completed run");
+    HIP_TRY(hipSetDevice(device));
+    uint32_t kmax = 1;
+    while ((uint64_t)kmax * (kmax + 1) / 2 <= g.V) kmax++;
+    uint64_t *dpairs;
+    scc_status s;
+    if ((s = scratch(buf[11], 2 * (size_t)kmax + 2, &dpairs)) != SCC_OK) return s;
+    uint32_t *dn = reinterpret_cast<uint32_t *>(dpairs + 2 * (size_t)kmax);
+    if (events.size() < 2) {
+        for (size_t i = events.size(); i < 2; i++) {
+            hipEvent_t e;
+            HIP_TRY(hipEventCreate(&e));
+            events.push_back(e);
+        }
+    }
+    nmarks = 0;   // the run's phase marks are already collected
+    HIP_TRY(hipEventRecord(events[0], stream));
+    k_fill_u32(stream, dn, 0u, 1);
+    k_sizes(stream, g, st);
+    k_size_histogram(stream, g, st, st.aux, dpairs, kmax, dn);
+    HIP_TRY(hipEventRecord(events[1], stream));
+    uint32_t n = 0;
+    HIP_TRY(hipMemcpyAsync(&n, dn, sizeof n, hipMemcpyDeviceToHost, stream));
+    HIP_TRY(hipStreamSynchronize(stream));
+    if (n > kmax) return fail(SCC_ERR_DEVICE, "%u distinct SCC sizes exceed the bound %u", n, kmax);
+    std::vector<uint64_t> h(2 * (size_t)n);
+    if (n) HIP_TRY(hipMemcpy(h.data(), dpairs, sizeof(uint64_t) * h.size(), hipMemcpyDeviceToHost));
+    float t = 0;
+    (void)hipEventElapsedTime(&t, events[0], events[1]);
+    if (ms) *ms = t;
+    std::vector<std::pair<uint64_t, uint64_t>> v(n);
+    for (uint32_t i = 0; i < n; i++) v[i] = {h[2 * i], h[2 * i + 1]};
+    std::sort(v.begin(), v.end());
+    for (uint32_t i = 0; i < n && i < cap; i++) {
+        pairs[2 * i] = v[i].first;
+        pairs[2 * i + 1] = v[i].second;
+    }
+    *npairs = n;
+    return SCC_OK;
+}
+
 // --------------------------------------------------- phase entries ---
 
 scc_status Engine::phase_begin(const uint8_t *dead)
@@ -1050,6 +1141,7 @@ scc_status Engine::phase_begin(const uint8_t *dead)
     ran = false;
     phase_marks = true;   // the phase entry points time their phase
     if (dead) HIP_TRY(hipMemcpyAsync(d_tmp8, dead, (size_t)g.V + 1, hipMemcpyHostToDevice, stream));
+    k_first_arcs(stream, g);
     k_init_state(stream, g, st, dead ? d_tmp8 : nullptr);   // also zeroes the counters
     return SCC_OK;
 }

@@ -128,10 +128,10 @@ constexpr uint32_t LB_LDS = QBUF + 2 + (LB_TILE + 1) + 3 * LB_TILE + 4;
 // after a bottom-up level (pre not null), in segments read through seg_pos
 // with the prefixes `pre` and slot regions of qcap.  Called by the whole
 // workgroup.
-template <bool SINGLE>
+template <bool SINGLE, bool COUNT = false>
 __device__ __forceinline__ void lb_level(const DirArgs &D, const uint32_t *cls, int L, uint32_t bid,
                                          uint32_t bpd, uint32_t *lds, uint32_t n, const uint32_t *pre,
-                                         uint32_t qcap)
+                                         uint32_t qcap, WorkCount &wc)
 {
     const unsigned long long packed = *D.pbigc;
     const uint32_t items = (uint32_t)(packed >> 32), chunks = (uint32_t)packed;
@@ -171,6 +171,10 @@ __device__ __forceinline__ void lb_level(const DirArgs &D, const uint32_t *cls, 
             if (j < e) {
                 w = C[j];
                 claimed = try_claim<SINGLE>(D.vis, cls, w, c);
+                if (COUNT) {
+                    wc.n[W_TDE]++;
+                    wc.n[W_CLAIM] += claimed;
+                }
             }
             lq_push(q, claimed, w);
             lq_flush(q, cnt_out, D.qout, false);
@@ -192,6 +196,7 @@ __device__ __forceinline__ void lb_level(const DirArgs &D, const uint32_t *cls, 
             beg = R[v];
             deg = R[v + 1] - beg;
             if (!SINGLE) c = cls[v];
+            if (COUNT && slice == 0) wc.n[W_TDV]++;
             if (deg >= LB_BIG && slice == 0) {
                 const unsigned long long nch = (deg + LB_CHUNK - 1) / LB_CHUNK;
                 const unsigned long long old = atomicAdd(D.bigc, (1ull << 32) | nch);
@@ -231,6 +236,10 @@ __device__ __forceinline__ void lb_level(const DirArgs &D, const uint32_t *cls, 
                 const uint32_t bit = 1u << (w & 31);
                 claimed = direct ? !(atomicOr(&D.vis[vword(w)], bit) & bit)
                                  : try_claim<SINGLE>(D.vis, cls, w, SINGLE ? 0u : s_cls[lo]);
+                if (COUNT) {
+                    wc.n[W_TDE]++;
+                    wc.n[W_CLAIM] += claimed;
+                }
             }
             lq_push(q, claimed, w);
             lq_flush(q, cnt_out, D.qout, false);
@@ -249,7 +258,8 @@ __global__ __launch_bounds__(256) void expand_lb_kernel(LevelArgs a, uint32_t bp
     split_dirs(a.ndirs, bpd, dir, bid);
     const DirArgs &D = a.d[dir];
     // no bottom-up level runs in this mapping: the queues start at index 0
-    lb_level<SINGLE>(D, a.cls, a.level, bid, bpd, s_lds, D.cnt[CT(a.level % 3)], nullptr, 0u);
+    WorkCount wc;
+    lb_level<SINGLE>(D, a.cls, a.level, bid, bpd, s_lds, D.cnt[CT(a.level % 3)], nullptr, 0u, wc);
 }
 
 static void launch_lb(hipStream_t s, const LevelArgs &a, bool single, uint32_t bpd)
@@ -317,11 +327,11 @@ constexpr uint32_t BU_LDS = 4 * BU_WQ + 4 * BU_CHUNKS * 64 + 4 * BU_CHUNKS * 2;
 // output queue (slot_cap(V, bpd) = qcap entries: the slot's workgroups own
 // that many vertices at most) and are counted in that slot of the count, so
 // the flushes of the pass's workgroups are spread over kSpread addresses.
-template <bool SINGLE>
+template <bool SINGLE, bool COUNT = false>
 __device__ __forceinline__ void bu_level(const DirArgs &D, const uint32_t *__restrict__ PR,
                                          const uint32_t *__restrict__ PC, const uint32_t *__restrict__ PF,
                                          const uint32_t *cls, uint32_t V, int L, uint32_t bid, uint32_t bpd,
-                                         uint32_t *lds, uint32_t qcap, uint32_t *anom)
+                                         uint32_t *lds, uint32_t qcap, uint32_t *anom, WorkCount &wc)
 {
     const uint32_t slot = bid % kSpread;
     uint32_t *cnt_out = &D.cslot[CT(((L + 1) % 3) * kSpread + slot)];
@@ -368,6 +378,13 @@ __device__ __forceinline__ void bu_level(const DirArgs &D, const uint32_t *__res
             b[h] = more ? PR[u[h]] : 0u;
             e[h] = more ? PR[u[h] + 1] : 0u;
             need_long[h] = more && e[h] - b[h] > BU_SHORT;
+            if (COUNT) {
+                wc.n[W_BUS] += u[h] <= V;
+                wc.n[W_BUC] += cand[h];
+                wc.n[W_BUF] += pf[h] != 0;
+                wc.n[W_BUR] += more;
+                wc.n[W_CLAIM] += found[h];
+            }
         }
         uint32_t nlong = 0;
 #pragma unroll
@@ -392,10 +409,12 @@ __device__ __forceinline__ void bu_level(const DirArgs &D, const uint32_t *__res
                 for (uint32_t j0 = PR[x] + BU_SHORT; j0 < ex && !hit; j0 += BU_GROUP) {
                     const uint32_t j = j0 + sub;
                     const bool h = j < ex && visited_parent<SINGLE>(D.vis, cls, PC[j], cx);
+                    if (COUNT) wc.n[W_BUM] += j < ex;
                     hit = ((__ballot(h) >> gshift) & ((1ull << BU_GROUP) - 1)) != 0;
                 }
             }
             const bool claimed = hit && sub == 0;
+            if (COUNT) wc.n[W_CLAIM] += claimed;
             if (claimed) {
                 const uint32_t hx = (x - base) / stride, off = x - base - hx * stride;
                 atomicOr(&wh[hx * 2 + (off >> 5)], 1u << (off & 31));   // LDS
@@ -421,7 +440,20 @@ __device__ __forceinline__ void bu_level(const DirArgs &D, const uint32_t *__res
 // stores the result in slot L&1 for level L+1.  So the host can queue any
 // number of levels without reading counters in between; a direction that
 // has finished turns its workgroups into no-ops.
-template <bool SINGLE>
+// Adds each lane's work units to the counters, one 64-bit atomic per wave
+// and unit (the counted run only).
+__device__ __forceinline__ void add_work(uint32_t *ctr, const WorkCount &wc)
+{
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+        uint32_t x = wc.n[k];
+        for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o);
+        if (wlane() == 0 && x)
+            atomicAdd(reinterpret_cast<unsigned long long *>(ctr + CT(C_WORK + k)), (unsigned long long)x);
+    }
+}
+
+template <bool SINGLE, bool COUNT>
 __global__ __launch_bounds__(256) void expand_do_kernel(DoArgs a, uint32_t bpd)
 {
     __shared__ uint32_t s_lds[BU_LDS > LB_LDS ? BU_LDS : LB_LDS];
@@ -448,11 +480,231 @@ __global__ __launch_bounds__(256) void expand_do_kernel(DoArgs a, uint32_t bpd)
         st = dir_decide(st, n, (uint32_t)(*D.pbigc >> 32), a.ctr[CT(C_NLIVE)], (uint32_t)L, a.bu_ok != 0);
         if (bid == 0 && threadIdx.x == 0) store_dir_state(a.ctr, L & 1, dir, st);
     }
+    WorkCount wc;
     if (st.mode == DIR_TD)
-        lb_level<SINGLE>(D, a.cls, L, bid, bpd, s_lds, n, slotted ? s_pre : nullptr, a.qcap);
+        lb_level<SINGLE, COUNT>(D, a.cls, L, bid, bpd, s_lds, n, slotted ? s_pre : nullptr, a.qcap, wc);
     else if (st.mode == DIR_BU)
-        bu_level<SINGLE>(D, a.PR[dir], a.PC[dir], a.PF[dir], a.cls, a.V, L, bid, bpd, s_lds, a.qcap,
-                         a.ctr + CT(C_ANOM));
+        bu_level<SINGLE, COUNT>(D, a.PR[dir], a.PC[dir], a.PF[dir], a.cls, a.V, L, bid, bpd, s_lds, a.qcap,
+                                a.ctr + CT(C_ANOM), wc);
+    if (COUNT) add_work(a.ctr, wc);
+}
+
+// ---------------------------------------------------------------------
+// Narrow frontiers: many top-down levels inside one workgroup per direction.
+// A high-diameter graph (the 4096 x 4096 grid: 8,193 levels, frontiers of
+// a few thousand vertices) pays a launch and a chain of global round trips
+// (queue count, queue, row offsets, columns, claim, queue reservation) per
+// device-wide level.  Here one 1,024-lane workgroup per direction keeps the
+// frontier in LDS and loops over levels with workgroup barriers only: per
+// level the row offsets of the frontier, an LDS scan of the degrees, the
+// columns and the claims (workgroup-scope atomics: during this kernel no
+// other workgroup touches this direction's bitmap), and an LDS append.
+//
+// The kernel takes over at level L0 in the state the level-L0 launch of
+// expand_do_kernel would start from and leaves that same position rewritten
+// for the device-wide levels that follow: the frontier it stops at is level
+// L0's input queue and count, and the search state of slot (L0-1)&1 (slot
+// 0 for L0 = 0) is the one the last expanded level would have left.  It
+// stops when the frontier is empty, when the next frontier exceeds NL_CAP
+// (the claims past NL_CAP go straight to the global queue), or when a
+// level after the first would scan more than NL_ARCS arcs.  It does nothing
+// unless level L0 would run top-down from a top-down level's queue with no
+// hubs pending and at most NL_CAP vertices, so the host may queue it behind
+// levels whose outcome it has not seen.  At least one level is expanded
+// whenever it runs, and every level either claims a vertex or ends the
+// loop, so it ends.
+constexpr uint32_t NL_THREADS = 1024;
+constexpr uint32_t NL_CAP = 6144;                 // frontier entries held in LDS
+constexpr uint32_t NL_PER = NL_CAP / NL_THREADS;  // frontier entries per lane
+constexpr uint32_t NL_ARCS = 32768;               // larger levels go back to the device-wide kernel
+constexpr uint32_t NL_K = 8;                      // arcs per lane per round
+
+// Exclusive scan of deg[0..n) in place over the workgroup (n <= NL_CAP):
+// each lane sums its NL_PER consecutive entries, a wave scan and a scan of
+// the 16 wave totals place them.  Returns the total.
+__device__ __forceinline__ uint32_t nl_scan(uint32_t *deg, uint32_t n, uint32_t *s_wsum)
+{
+    const uint32_t t = threadIdx.x, l = wlane(), w = t >> 6;
+    uint32_t loc[NL_PER], sum = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < NL_PER; k++) {
+        const uint32_t i = t * NL_PER + k;
+        loc[k] = i < n ? deg[i] : 0u;
+        sum += loc[k];
+    }
+    uint32_t incl = sum;
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t y = __shfl_up(incl, o);
+        if (l >= (uint32_t)o) incl += y;
+    }
+    if (l == 63) s_wsum[w] = incl;
+    __syncthreads();
+    uint32_t base = 0, total = 0;
+    for (uint32_t i = 0; i < NL_THREADS / 64; i++) {
+        const uint32_t x = s_wsum[i];
+        base += i < w ? x : 0u;
+        total += x;
+    }
+    uint32_t run = base + incl - sum;
+#pragma unroll
+    for (uint32_t k = 0; k < NL_PER; k++) {
+        const uint32_t i = t * NL_PER + k;
+        if (i < n) deg[i] = run;
+        run += loc[k];
+    }
+    __syncthreads();   // s_wsum reuse, deg complete
+    return total;
+}
+
+template <bool SINGLE, bool COUNT>
+__global__ __launch_bounds__(NL_THREADS) void expand_narrow_kernel(DoArgs a)
+{
+    __shared__ uint32_t s_q[2][NL_CAP];
+    __shared__ uint32_t s_beg[NL_CAP];
+    __shared__ uint32_t s_off[NL_CAP + 1];
+    __shared__ uint32_t s_cls[SINGLE ? 1 : NL_CAP];
+    __shared__ uint32_t s_wsum[NL_THREADS / 64];
+    __shared__ uint32_t s_n;
+    const int dir = (int)blockIdx.x;
+    if (dir >= a.ndirs) return;
+    const DirArgs &D = a.d[dir];
+    const int L0 = a.level;
+    const uint32_t t = threadIdx.x;
+    // the state the level-L0 launch would compute
+    const DirState prev = load_dir_state(a.ctr, L0 == 0 ? 0 : (L0 - 1) & 1, dir);
+    if (L0 > 0 && prev.mode != DIR_TD) return;   // finished, or a bottom-up level's slotted queue
+    uint32_t n = D.cnt[CT(L0 % 3)];
+    const uint32_t hubs = (uint32_t)(*D.pbigc >> 32);
+    const DirState st0 = L0 > 0 ? dir_decide(prev, n, hubs, a.ctr[CT(C_NLIVE)], (uint32_t)L0, true) : prev;
+    if (st0.mode != DIR_TD || hubs || n > NL_CAP) return;
+    uint32_t *q = const_cast<uint32_t *>(D.qin);   // level L0's queue: read once, then rewritten
+    for (uint32_t i = t; i < n; i += NL_THREADS) s_q[0][i] = q[i];
+    __syncthreads();
+    const uint32_t *__restrict__ R = D.R;
+    const uint32_t *__restrict__ C = D.C;
+    uint64_t visited = st0.visited;      // includes the current frontier
+    uint32_t last_nf = prev.nf, levels = 0, cur = 0;
+    uint32_t wc_v = 0, wc_e = 0, wc_c = 0;
+    bool spilled = false;                // the frontier is in the global queue
+    while (n) {
+        // row offsets (and classes) of the frontier, every lane's loads in flight together
+        uint32_t b[NL_PER], e[NL_PER], c[NL_PER];
+#pragma unroll
+        for (uint32_t k = 0; k < NL_PER; k++) {
+            const uint32_t i = t * NL_PER + k;
+            const uint32_t v = i < n ? s_q[cur][i] : 0u;
+            b[k] = i < n ? R[v] : 0u;
+            e[k] = i < n ? R[v + 1] : 0u;
+            if (!SINGLE) c[k] = i < n ? a.cls[v] : 0u;
+        }
+#pragma unroll
+        for (uint32_t k = 0; k < NL_PER; k++) {
+            const uint32_t i = t * NL_PER + k;
+            if (i < n) {
+                s_beg[i] = b[k];
+                s_off[i] = e[k] - b[k];
+                if (!SINGLE) s_cls[i] = c[k];
+            }
+        }
+        if (t == 0) s_n = 0;
+        __syncthreads();
+        const uint32_t T = nl_scan(s_off, n, s_wsum);
+        if (levels > 0 && T > NL_ARCS) break;   // wide again: the device-wide levels take over
+        if (t == 0) s_off[n] = T;
+        __syncthreads();
+        uint32_t *nq = s_q[cur ^ 1];
+        for (uint32_t rb = 0; rb < T; rb += NL_THREADS * NL_K) {
+            const uint32_t j0 = rb + t * NL_K;
+            uint32_t w[NL_K], wcl[NL_K];
+            uint32_t o = 0;
+            if (j0 < T) {   // owner of arc j0: the last entry whose offset is <= j0
+                uint32_t lo = 0, hi = n - 1;
+                while (lo < hi) {
+                    const uint32_t mid = (lo + hi + 1) >> 1;
+                    if (s_off[mid] <= j0) lo = mid;
+                    else hi = mid - 1;
+                }
+                o = lo;
+            }
+#pragma unroll
+            for (uint32_t k = 0; k < NL_K; k++) {
+                const uint32_t j = j0 + k;
+                w[k] = 0;
+                wcl[k] = 0;
+                if (j < T) {
+                    while (s_off[o + 1] <= j) o++;
+                    w[k] = C[s_beg[o] + (j - s_off[o])];
+                    if (!SINGLE) wcl[k] = s_cls[o];
+                }
+            }
+#pragma unroll
+            for (uint32_t k = 0; k < NL_K; k++) {
+                const uint32_t j = j0 + k;
+                bool claimed = false;
+                if (j < T && (SINGLE || a.cls[w[k]] == wcl[k])) {
+                    const uint32_t bit = 1u << (w[k] & 31);
+                    claimed = !(__hip_atomic_fetch_or(&D.vis[vword(w[k])], bit, __ATOMIC_RELAXED,
+                                                      __HIP_MEMORY_SCOPE_WORKGROUP) & bit);
+                }
+                const uint64_t m = __ballot(claimed);
+                if (m) {
+                    const int leader = __ffsll((unsigned long long)m) - 1;
+                    uint32_t base = 0;
+                    if ((int)wlane() == leader) base = atomicAdd(&s_n, (uint32_t)__popcll(m));
+                    base = __shfl(base, leader);
+                    const uint32_t at = base + (uint32_t)__popcll(m & ((1ull << wlane()) - 1));
+                    if (claimed) {
+                        if (at < NL_CAP) nq[at] = w[k];
+                        else q[at] = w[k];   // past the LDS frontier: straight to the global queue
+                    }
+                }
+                if (COUNT) {
+                    wc_e += j < T;
+                    wc_c += claimed;
+                }
+            }
+        }
+        if (COUNT && t == 0) wc_v += n;
+        __syncthreads();
+        const uint32_t n2 = s_n;
+        levels++;
+        last_nf = n;
+        visited += n2;
+        n = n2;
+        cur ^= 1;
+        if (n > NL_CAP) {   // the LDS part joins the claims already in the global queue
+            for (uint32_t i = t; i < NL_CAP; i += NL_THREADS) q[i] = s_q[cur][i];
+            spilled = true;
+            break;
+        }
+        __syncthreads();   // s_n reuse
+    }
+    if (!spilled)
+        for (uint32_t i = t; i < n; i += NL_THREADS) q[i] = s_q[cur][i];
+    if (t == 0) {
+        D.cnt[CT(L0 % 3)] = n;
+        DirState s;
+        s.mode = DIR_TD;
+        s.end = 0;
+        s.pad = 0;
+        if (L0 == 0) {   // level 0 reads its state as it is
+            s.nf = n;
+            s.visited = visited;
+            store_dir_state(a.ctr, 0, dir, s);
+        } else {          // level L0 adds its input to `visited` itself
+            s.nf = last_nf;
+            s.visited = visited - n;
+            store_dir_state(a.ctr, (L0 - 1) & 1, dir, s);
+        }
+        atomicAdd(&a.ctr[CT(C_NLEVELS) + dir], levels);
+    }
+    if (COUNT) {
+        WorkCount wc;
+        wc.n[W_TDV] = wc_v;
+        wc.n[W_TDE] = wc_e;
+        wc.n[W_CLAIM] = wc_c;
+        add_work(a.ctr, wc);
+    }
 }
 
 __global__ void do_begin_kernel(uint32_t *ctr, int ndirs)
@@ -467,6 +719,18 @@ __global__ void do_begin_kernel(uint32_t *ctr, int ndirs)
     st.visited = ctr[CT(C_NPIV)];
     store_dir_state(ctr, 0, d, st);
 }
+
+void k_narrow_levels(hipStream_t s, const DoArgs &a, bool single_class)
+{
+    if (single_class && a.count)
+        expand_narrow_kernel<true, true><<<a.ndirs, NL_THREADS, 0, s>>>(a);
+    else if (single_class)
+        expand_narrow_kernel<true, false><<<a.ndirs, NL_THREADS, 0, s>>>(a);
+    else
+        expand_narrow_kernel<false, false><<<a.ndirs, NL_THREADS, 0, s>>>(a);
+}
+
+uint32_t narrow_cap() { return NL_CAP; }
 
 uint32_t do_blocks(uint32_t V, int ndirs)
 {
@@ -487,10 +751,12 @@ void k_do_level(hipStream_t s, const DoArgs &a, bool single_class, uint32_t bpd)
 {
     bpd = dir_blocks(bpd, a.ndirs);
     const uint32_t grid = bpd * (uint32_t)a.ndirs;
-    if (single_class)
-        expand_do_kernel<true><<<grid, 256, 0, s>>>(a, bpd);
+    if (single_class && a.count)
+        expand_do_kernel<true, true><<<grid, 256, 0, s>>>(a, bpd);
+    else if (single_class)
+        expand_do_kernel<true, false><<<grid, 256, 0, s>>>(a, bpd);
     else
-        expand_do_kernel<false><<<grid, 256, 0, s>>>(a, bpd);
+        expand_do_kernel<false, false><<<grid, 256, 0, s>>>(a, bpd);
 }
 
 template <int W>

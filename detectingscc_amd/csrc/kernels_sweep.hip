@@ -352,12 +352,20 @@ __global__ __launch_bounds__(256) void trim1_kernel(const uint32_t *__restrict__
 // the first round finishes computes every label from the search's bitmaps
 // (label_from_bits_kernel), and otherwise label_fixup_kernel writes the
 // labels and class words the eager passes would have.
-// Pure streaming of the degree products (DP[v] == 0: no out-arc or no
-// in-arc), so each wave keeps TRIM_DEG_CHUNKS 64-vertex chunks' loads in
-// flight.
+// The pass also builds the per-vertex arrays the later phases read, for
+// the vertices it keeps (nothing reads them for a removed vertex): the
+// degree product DP (the first pivot's election key), and the first out-
+// and in-neighbour F1/B1 (the trim-1 and bottom-up first-arc probes).  So
+// it streams the row offsets (coalesced; 0 degree product: no out-arc or no
+// in-arc) and gathers the first column entry of each kept vertex's two
+// lists, with TRIM_DEG_CHUNKS 64-vertex chunks' loads in flight per wave.
 constexpr uint32_t TRIM_DEG_CHUNKS = 4;
 
-__global__ __launch_bounds__(256) void trim1_degree_kernel(const uint32_t *__restrict__ DP, uint32_t *cls,
+__global__ __launch_bounds__(256) void trim1_degree_kernel(const uint32_t *__restrict__ Fr,
+                                                           const uint32_t *__restrict__ Fc,
+                                                           const uint32_t *__restrict__ Br,
+                                                           const uint32_t *__restrict__ Bc, uint32_t *F1,
+                                                           uint32_t *B1, uint32_t *DP, uint32_t *cls,
                                                            uint32_t *lab, uint32_t *live, uint32_t V,
                                                            uint32_t *ctr, uint32_t *out, uint32_t cap,
                                                            int write_labels)
@@ -374,12 +382,27 @@ __global__ __launch_bounds__(256) void trim1_degree_kernel(const uint32_t *__res
     const uint32_t stride = gridDim.x * 256;
     for (uint32_t base = (blockIdx.x * 4 + wv) * 64; base <= V; base += TRIM_DEG_CHUNKS * stride) {
         if (nq + TRIM_DEG_CHUNKS * 64 > TRIM_WQ) wave_flush(wq, nq, cnt_out, qout, cap, anom);
-        uint32_t v[TRIM_DEG_CHUNKS], dp[TRIM_DEG_CHUNKS];
+        uint32_t v[TRIM_DEG_CHUNKS], fb[TRIM_DEG_CHUNKS], fe[TRIM_DEG_CHUNKS], bb[TRIM_DEG_CHUNKS],
+            be[TRIM_DEG_CHUNKS], dp[TRIM_DEG_CHUNKS];
 #pragma unroll
         for (uint32_t h = 0; h < TRIM_DEG_CHUNKS; h++) {
             v[h] = base + h * stride + lane();
             const bool in = v[h] <= V;
-            dp[h] = in ? DP[v[h]] : 0u;   // 0: no out-arc or no in-arc
+            fb[h] = in ? Fr[v[h]] : 0u;
+            fe[h] = in ? Fr[v[h] + 1] : 0u;
+            bb[h] = in ? Br[v[h]] : 0u;
+            be[h] = in ? Br[v[h] + 1] : 0u;
+        }
+#pragma unroll
+        for (uint32_t h = 0; h < TRIM_DEG_CHUNKS; h++) {
+            // 0: no out-arc or no in-arc (vertex 0, the dummy, has neither)
+            const uint64_t p = (uint64_t)(fe[h] - fb[h]) * (uint64_t)(be[h] - bb[h]);
+            dp[h] = p > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)p;
+            if (v[h] >= 1 && dp[h]) {
+                F1[v[h]] = Fc[fb[h]];
+                B1[v[h]] = Bc[bb[h]];
+                DP[v[h]] = dp[h];
+            }
         }
 #pragma unroll
         for (uint32_t h = 0; h < TRIM_DEG_CHUNKS; h++) {
@@ -416,8 +439,9 @@ void k_trim1(hipStream_t s, const DevGraph &g, DevState &st, const uint32_t *in,
 {
     const uint32_t cap = trim_slot_cap(g.V);
     if (live && pass == 0 && degree_first) {
-        trim1_degree_kernel<<<sweep_blocks(g.V), SWEEP_BLOCK, 0, s>>>(g.DP, st.cls, st.lab, live, g.V,
-                                                                      st.ctr, out, cap, labels ? 1 : 0);
+        trim1_degree_kernel<<<sweep_blocks(g.V), SWEEP_BLOCK, 0, s>>>(g.Fr, g.Fc, g.Br, g.Bc, g.F1, g.B1, g.DP,
+                                                                      st.cls, st.lab, live, g.V, st.ctr, out,
+                                                                      cap, labels ? 1 : 0);
     } else if (live) {
         if (pass == 0)
             live_bits_kernel<<<sweep_blocks(g.V), SWEEP_BLOCK, 0, s>>>(st.cls, live, g.V);
@@ -1177,13 +1201,10 @@ __device__ __forceinline__ bool gate_closed(const uint32_t *gate)
     return x != 0;
 }
 
-__global__ void canon_reset_kernel(uint32_t *aux, uint32_t *cnt, uint32_t V, const uint32_t *gate)
+__global__ void canon_reset_kernel(uint32_t *aux, uint32_t V, const uint32_t *gate)
 {
     if (gate_closed(gate)) return;
-    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v <= V; v += gridDim.x * blockDim.x) {
-        aux[v] = DEAD;
-        if (cnt) cnt[v] = 0;
-    }
+    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v <= V; v += gridDim.x * blockDim.x) aux[v] = DEAD;
 }
 
 // Lanes of a wave that share a label are combined before touching memory:
@@ -1277,15 +1298,11 @@ __global__ __launch_bounds__(256) void canon_min_kernel(uint32_t *lab, uint32_t 
     }
 }
 
-// With sizes, each wave also keeps a running count for one label that
-// several of its lanes shared (a giant SCC recurs in every wave), added to
-// memory once at the end instead of once per wave iteration.
 __global__ __launch_bounds__(256) void canon_apply_kernel(uint32_t *lab, const uint32_t *__restrict__ aux,
-                                                          uint32_t *cnt, uint32_t *ctr, uint32_t V, int sizes,
-                                                          const uint32_t *gate)
+                                                          uint32_t *ctr, uint32_t V, const uint32_t *gate)
 {
     if (gate_closed(gate)) return;
-    uint32_t roots = 0, heavy = DEAD, hcount = 0;
+    uint32_t roots = 0;
     const uint32_t stride = gridDim.x * blockDim.x;
     for (uint32_t base = blockIdx.x * blockDim.x + (threadIdx.x & ~63u); base < V; base += CANON_CHUNKS * stride) {
         uint32_t vs[CANON_CHUNKS], rs[CANON_CHUNKS];
@@ -1302,48 +1319,13 @@ __global__ __launch_bounds__(256) void canon_apply_kernel(uint32_t *lab, const u
 #pragma unroll
         for (uint32_t h = 0; h < CANON_CHUNKS; h++) {
             const uint32_t v = vs[h], r = rs[h];
-            const bool valid = v <= V;
-            if (valid) {
+            if (v <= V) {
                 lab[v] = r;
                 roots += r == v;
             }
-            if (!sizes) continue;
-            bool pending = valid;
-            for (int it = 0; it < MATCH_ROUNDS; it++) {
-                const uint64_t pm = __ballot(pending);
-                if (!pm) break;
-                const int L = __ffsll((unsigned long long)pm) - 1;
-                const uint32_t rl = __shfl(r, L);
-                const uint32_t n = (uint32_t)__popcll(__ballot(pending && r == rl));
-                if (rl == heavy) {
-                    hcount += n;
-                } else if (heavy == DEAD && n > 1) {
-                    heavy = rl;
-                    hcount = n;
-                } else if ((int)lane() == L) {
-                    atomicAdd(&cnt[rl], n);
-                }
-                pending = pending && r != rl;
-            }
-            if (pending) atomicAdd(&cnt[r], 1u);
         }
     }
-    if (hcount && lane() == 0) atomicAdd(&cnt[heavy], hcount);
     spread_add32(ctr, C_NSCC, roots);
-}
-
-__global__ void canon_largest_kernel(const uint32_t *__restrict__ cnt, uint32_t *ctr, uint32_t V,
-                                     const uint32_t *gate)
-{
-    if (gate_closed(gate)) return;
-    uint32_t m = 0;
-    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x + 1; v <= V; v += gridDim.x * blockDim.x)
-        m = cnt[v] > m ? cnt[v] : m;
-    for (int o = 32; o > 0; o >>= 1) {
-        uint32_t x = __shfl_xor(m, o);
-        m = x > m ? x : m;
-    }
-    if (lane() == 0 && m) atomicMax(&ctr[CT(C_LARGEST)], m);
 }
 
 // The labels of a decomposition that a lazy first round finished (gated:
@@ -1382,7 +1364,7 @@ void k_label_from_bits(hipStream_t s, const DevGraph &g, DevState &st)
 
 void k_canon_apply(hipStream_t s, const DevGraph &g, DevState &st, bool gated)
 {
-    canon_apply_kernel<<<sweep_blocks(g.V), SWEEP_BLOCK, 0, s>>>(st.lab, st.aux, st.cls, st.ctr, g.V, 0,
+    canon_apply_kernel<<<sweep_blocks(g.V), SWEEP_BLOCK, 0, s>>>(st.lab, st.aux, st.ctr, g.V,
                                                                  gated ? st.ctr : nullptr);
 }
 
@@ -1390,11 +1372,153 @@ void k_canon(hipStream_t s, const DevGraph &g, DevState &st, bool sizes, bool ga
 {
     const uint32_t *gate = gated ? st.ctr : nullptr;
     const uint32_t b = sweep_blocks(g.V);
-    // cls is all DEAD at this point; it holds the SCC sizes from here on
-    canon_reset_kernel<<<b, SWEEP_BLOCK, 0, s>>>(st.aux, sizes ? st.cls : nullptr, g.V, gate);
+    canon_reset_kernel<<<b, SWEEP_BLOCK, 0, s>>>(st.aux, g.V, gate);
     canon_min_kernel<<<b, SWEEP_BLOCK, 0, s>>>(st.lab, st.aux, st.ctr, g.V, gate);
-    canon_apply_kernel<<<b, SWEEP_BLOCK, 0, s>>>(st.lab, st.aux, st.cls, st.ctr, g.V, sizes ? 1 : 0, gate);
-    if (sizes) canon_largest_kernel<<<b, SWEEP_BLOCK, 0, s>>>(st.cls, st.ctr, g.V, gate);
+    canon_apply_kernel<<<b, SWEEP_BLOCK, 0, s>>>(st.lab, st.aux, st.ctr, g.V, gate);
+    if (sizes) k_sizes(s, g, st, gated);
+}
+
+// ------------------------------------------------- SCC sizes ---
+// From canonical labels (lab[v] = the smallest vertex of v's SCC): cnt[r]
+// counts the members of r's SCC other than r itself, so a singleton (most
+// SCCs of a power-law graph: trim-1's) costs no atomic at all, and the
+// members of a large SCC are combined per wave (MATCH_ROUNDS leaders, then
+// one running count per wave for a recurring label) before touching memory.
+// The size of the SCC of root r (lab[r] == r) is cnt[r] + 1.  cnt is the
+// class-word array, free once the labels are final.
+__global__ void size_reset_kernel(uint32_t *cnt, uint32_t V, const uint32_t *gate)
+{
+    if (gate_closed(gate)) return;
+    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v <= V; v += gridDim.x * blockDim.x) cnt[v] = 0;
+}
+
+__global__ __launch_bounds__(256) void size_count_kernel(const uint32_t *__restrict__ lab, uint32_t *cnt,
+                                                         uint32_t V, const uint32_t *gate)
+{
+    if (gate_closed(gate)) return;
+    uint32_t heavy = DEAD, hcount = 0;
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t base = blockIdx.x * blockDim.x + (threadIdx.x & ~63u); base < V; base += CANON_CHUNKS * stride) {
+        uint32_t vs[CANON_CHUNKS], rs[CANON_CHUNKS];
+#pragma unroll
+        for (uint32_t h = 0; h < CANON_CHUNKS; h++) {
+            vs[h] = base + h * stride + lane() + 1;
+            rs[h] = vs[h] <= V ? lab[vs[h]] : 0u;
+        }
+#pragma unroll
+        for (uint32_t h = 0; h < CANON_CHUNKS; h++) {
+            const uint32_t r = rs[h];
+            bool pending = vs[h] <= V && r != vs[h] && r >= 1 && r <= V;
+            for (int it = 0; it < MATCH_ROUNDS; it++) {
+                const uint64_t pm = __ballot(pending);
+                if (!pm) break;
+                const int L = __ffsll((unsigned long long)pm) - 1;
+                const uint32_t rl = __shfl(r, L);
+                const uint32_t n = (uint32_t)__popcll(__ballot(pending && r == rl));
+                if (rl == heavy) {
+                    hcount += n;
+                } else if (heavy == DEAD && n > 1) {
+                    heavy = rl;
+                    hcount = n;
+                } else if ((int)lane() == L) {
+                    atomicAdd(&cnt[rl], n);
+                }
+                pending = pending && r != rl;
+            }
+            if (pending) atomicAdd(&cnt[r], 1u);
+        }
+    }
+    if (hcount && lane() == 0) atomicAdd(&cnt[heavy], hcount);
+}
+
+// Largest SCC: the largest cnt[r] + 1 over roots r.
+__global__ void size_largest_kernel(const uint32_t *__restrict__ lab, const uint32_t *__restrict__ cnt,
+                                    uint32_t *ctr, uint32_t V, const uint32_t *gate)
+{
+    if (gate_closed(gate)) return;
+    uint32_t m = 0;
+    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x + 1; v <= V; v += gridDim.x * blockDim.x)
+        if (lab[v] == v) m = cnt[v] + 1 > m ? cnt[v] + 1 : m;
+    for (int o = 32; o > 0; o >>= 1) {
+        uint32_t x = __shfl_xor(m, o);
+        m = x > m ? x : m;
+    }
+    if (lane() == 0 && m) atomicMax(&ctr[CT(C_LARGEST)], m);
+}
+
+void k_sizes(hipStream_t s, const DevGraph &g, DevState &st, bool gated)
+{
+    const uint32_t *gate = gated ? st.ctr : nullptr;
+    const uint32_t b = sweep_blocks(g.V);
+    size_reset_kernel<<<b, SWEEP_BLOCK, 0, s>>>(st.cls, g.V, gate);
+    size_count_kernel<<<b, SWEEP_BLOCK, 0, s>>>(st.lab, st.cls, g.V, gate);
+    size_largest_kernel<<<b, SWEEP_BLOCK, 0, s>>>(st.lab, st.cls, st.ctr, g.V, gate);
+}
+
+// Size histogram from the sizes pass: hist[k] = number of SCCs of k
+// vertices (hist has V + 1 words, cleared here), combined per wave like
+// the counts (a power-law graph has millions of roots of size 1), then the
+// non-zero entries compacted into (size, count) pairs: `pairs` holds up to
+// `cap` of them (64-bit words, size then count), *npairs (a counter line)
+// the number found.  At most about sqrt(2V) sizes occur, so the list is
+// short; the host sorts it.
+__global__ __launch_bounds__(256) void hist_count_kernel(const uint32_t *__restrict__ lab,
+                                                         const uint32_t *__restrict__ cnt, uint32_t *hist,
+                                                         uint32_t V)
+{
+    uint32_t heavy = 0, hcount = 0;
+    for (uint32_t v0 = blockIdx.x * blockDim.x + (threadIdx.x & ~63u); v0 < V; v0 += gridDim.x * blockDim.x) {
+        const uint32_t v = v0 + lane() + 1;
+        const uint32_t k = (v <= V && lab[v] == v) ? cnt[v] + 1 : 0u;
+        bool pending = k != 0;
+        for (int it = 0; it < MATCH_ROUNDS; it++) {
+            const uint64_t pm = __ballot(pending);
+            if (!pm) break;
+            const int L = __ffsll((unsigned long long)pm) - 1;
+            const uint32_t kl = __shfl(k, L);
+            const uint32_t n = (uint32_t)__popcll(__ballot(pending && k == kl));
+            if (kl == heavy) {
+                hcount += n;
+            } else if (heavy == 0 && n > 1) {
+                heavy = kl;
+                hcount = n;
+            } else if ((int)lane() == L) {
+                atomicAdd(&hist[kl], n);
+            }
+            pending = pending && k != kl;
+        }
+        if (pending) atomicAdd(&hist[k], 1u);
+    }
+    if (hcount && lane() == 0) atomicAdd(&hist[heavy], hcount);
+}
+
+__global__ void hist_compact_kernel(const uint32_t *__restrict__ hist, uint64_t *pairs, uint32_t cap,
+                                    uint32_t *npairs, uint32_t V)
+{
+    for (uint32_t k0 = blockIdx.x * blockDim.x + (threadIdx.x & ~63u); k0 < V; k0 += gridDim.x * blockDim.x) {
+        const uint32_t k = k0 + lane() + 1;
+        const uint32_t c = k <= V ? hist[k] : 0u;
+        const uint64_t m = __ballot(c != 0);
+        if (!m) continue;
+        const int leader = __ffsll((unsigned long long)m) - 1;
+        uint32_t base = 0;
+        if ((int)lane() == leader) base = atomicAdd(npairs, (uint32_t)__popcll(m));
+        base = __shfl(base, leader);
+        const uint32_t at = base + (uint32_t)__popcll(m & ((1ull << lane()) - 1));
+        if (c && at < cap) {
+            pairs[2 * (size_t)at] = k;
+            pairs[2 * (size_t)at + 1] = c;
+        }
+    }
+}
+
+void k_size_histogram(hipStream_t s, const DevGraph &g, DevState &st, uint32_t *hist, uint64_t *pairs,
+                      uint32_t cap, uint32_t *npairs)
+{
+    const uint32_t b = sweep_blocks(g.V);
+    size_reset_kernel<<<b, SWEEP_BLOCK, 0, s>>>(hist, g.V, nullptr);
+    hist_count_kernel<<<b, SWEEP_BLOCK, 0, s>>>(st.lab, st.cls, hist, g.V);
+    hist_compact_kernel<<<b, SWEEP_BLOCK, 0, s>>>(hist, pairs, cap, npairs, g.V);
 }
 
 // ------------------------------------------------ phase exports ---

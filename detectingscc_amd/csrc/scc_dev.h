@@ -63,11 +63,13 @@ enum : int {
     C_BWE = 20,
     C_BIGF = 22,     // 3 rotating 64-bit packed (items, chunks) counts of the FW hub lists (22..24)
     C_BIGB = 25,     // same for BW (25..27)
+    C_NLEVELS = 28,  // levels run by the narrow-frontier kernel, word 0 FW, word 1 BW
     C_PIV0 = 30,     // first pivot of the latest election
     C_ANOM = 31,     // invariant violations caught by kernel guards (must stay 0)
     C_NLIVE = 32,    // live vertices when the visited bitmaps were initialised
     C_TLIST = 33,    // 3 rotating counts of trim-1 removal lists (33..35)
     C_DOST = 36,     // direction-optimised search state: 2 slots x 2 directions (36..39)
+    C_WORK = 40,     // work units of the first search, counted with sizes (40..47, 64-bit each)
     C_COUNT = 48
 };
 
@@ -203,7 +205,24 @@ struct DoArgs {
     int level;
     int bu_ok;                       // 0 on a grid too small for a bottom-up pass: stay top-down
     uint32_t qcap;                   // slot region of a bottom-up level's queue: slot_cap(V, full grid)
+    int count;                       // 1: add the level's work units (WorkCount) to C_WORK..C_WORK+7
 };
+
+// Work units of a direction-optimised search, counted per lane in the
+// untimed run with sizes and added to the counters C_WORK + k (64-bit) at
+// the end of each level.  Their bytes (the search's algorithmic traffic):
+//   td_vertices  12 B  queue entry + row offsets of an expanded vertex
+//   td_arcs       8 B  column entry + visited word of a scanned arc
+//   claims        4 B  queue entry of a newly visited vertex
+//   bu_swept    1/8 B  visited bit of a vertex a bottom-up pass looks at
+//   bu_cand       4 B  first-parent entry of an unvisited candidate
+//   bu_first      8 B  visited/finished word of that first parent
+//   bu_rows       8 B  row offsets of a candidate whose first parent missed
+//   bu_more      12 B  parent id + visited/finished word of a further parent
+struct WorkCount {
+    uint32_t n[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+};
+enum : int { W_TDV = 0, W_TDE, W_CLAIM, W_BUS, W_BUC, W_BUF, W_BUR, W_BUM };
 
 // Direction-optimising switch points (Beamer et al.): go bottom-up when the
 // frontier holds more than 1/kToBottomUp of the vertices still unvisited,
@@ -381,6 +400,14 @@ void k_color_finish(hipStream_t s, const DevGraph &g, DevState &st);
 // Canonical labels; gated: each pass is a no-op unless C_LIVE is 0 (queued
 // before the host has read the live count of the round before).
 void k_canon(hipStream_t s, const DevGraph &g, DevState &st, bool sizes, bool gated = false);
+// SCC sizes from canonical labels into the class-word array (root r's SCC
+// has cnt[r] + 1 vertices) and the largest size into C_LARGEST
+void k_sizes(hipStream_t s, const DevGraph &g, DevState &st, bool gated = false);
+// Size histogram after k_sizes: (size, count) pairs, unsorted, into pairs
+// (up to cap), their number into *npairs (zeroed by the caller); hist is
+// scratch of V + 1 words
+void k_size_histogram(hipStream_t s, const DevGraph &g, DevState &st, uint32_t *hist, uint64_t *pairs,
+                      uint32_t cap, uint32_t *npairs);
 // The apply pass alone: every non-final label's minimum is already in aux
 // (k_update with scc_min in a round that left no live vertex).
 void k_canon_apply(hipStream_t s, const DevGraph &g, DevState &st, bool gated);
@@ -398,6 +425,11 @@ void k_expand_level(hipStream_t s, const LevelArgs &a, int expand, int warp, boo
 // directions top-down, visited = the seeds), then one k_do_level per level.
 void k_do_begin(hipStream_t s, uint32_t *ctr, int ndirs);
 void k_do_level(hipStream_t s, const DoArgs &a, bool single_class, uint32_t blocks_per_dir);
+// Narrow-frontier levels (one workgroup per direction) from level a.level,
+// leaving that level's input rewritten; a no-op unless that level would run
+// top-down on at most narrow_cap() vertices with no hubs pending
+void k_narrow_levels(hipStream_t s, const DoArgs &a, bool single_class);
+uint32_t narrow_cap();
 uint32_t do_blocks(uint32_t V, int ndirs);
 // Words of a search queue: bottom-up levels write kSpread slot regions.
 size_t queue_words(uint32_t V);

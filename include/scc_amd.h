@@ -173,6 +173,14 @@ typedef struct {
     double ms_scc;           /* CSR resident -> labels resident (device clock) */
     double ms_first_reach;   /* first-pivot FW||BW (device clock) */
     double ms_phase[SCC_PHASE_COUNT];   /* with opts.phase_times, else 0 */
+    /* Work units of the first search when it is direction-optimised and
+     * opts.sizes is set (else 0), both directions: [0] vertices expanded
+     * top-down, [1] arcs scanned top-down, [2] vertices claimed, [3]
+     * vertices a bottom-up pass looked at, [4] unvisited candidates, [5]
+     * first parents probed, [6] row-offset pairs read after a first-parent
+     * miss, [7] further parents probed.  Their bytes are 12, 8, 4, 1/8, 4,
+     * 8, 8 and 12 each (the search's algorithmic traffic). */
+    uint64_t first_work[8];
 } scc_stats;
 
 /* Create a context on HIP device `device` (the reference hard-codes
@@ -202,6 +210,14 @@ scc_status scc_condensation(scc_ctx *ctx, scc_host_graph *dag, uint32_t *node_of
 /* Upload + run + labels in one call. */
 scc_status scc_solve(scc_ctx *ctx, const scc_host_graph *g, const uint32_t *part,
                      const scc_opts *opts, uint32_t *lab, scc_stats *stats);
+
+/* SCC size histogram of the last scc_run, computed on the device from its
+ * canonical labels: (size, count) pairs ascending by size, up to `cap` of
+ * them written to pairs[2k], pairs[2k+1]; *npairs = the number of distinct
+ * sizes (at most floor(sqrt(2V)) + 1).  *ms (may be NULL) = device time.
+ * The reference prints only a count (scc_WCC.cu:966); this output has no
+ * counterpart there. */
+scc_status scc_histogram(scc_ctx *ctx, uint64_t *pairs, uint32_t cap, uint32_t *npairs, double *ms);
 
 /* SCC size histogram from labels: pairs (size, count) ascending by size,
  * at most `cap` pairs; *npairs receives the number of distinct sizes. */

@@ -216,3 +216,23 @@ def test_dropin_wrappers_print_reference_lines(ctx, capfd, fixture_path):
     rc = scc.wSlota(g.E, g.V, g.Fc, g.Fr, g.Bc, g.Br, True, False, 64)
     out = capfd.readouterr().out
     assert rc == 0 and out.split("\n")[1] == "Number Of Sccs : 13"
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_device_histogram(ctx, seed):
+    """The size histogram computed on the device equals the one counted on
+    the host from Tarjan's labels, for every algorithm, with and without the
+    sizes pass of the run itself."""
+    rng = np.random.default_rng(900 + seed)
+    V = int(rng.integers(1, 20000))
+    g = random_graph(V, int(rng.integers(0, 3 * V + 1)), rng)
+    want, n = orc.tarjan(g)
+    sizes = np.bincount(want[1:].astype(np.int64))
+    hist = np.bincount(sizes[sizes > 0])
+    pairs = [(k, int(c)) for k, c in enumerate(hist) if c]
+    ctx.upload(g)
+    for algo, sized in [("g", True), ("g", False), ("h", True), ("x", False), ("y", True)]:
+        ctx.run(algo=algo, warp=8, sizes=sized)
+        got, ms = ctx.histogram()
+        assert got == pairs, f"seed {seed} -a {algo} sizes={sized}"
+        assert ms >= 0
