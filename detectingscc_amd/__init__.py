@@ -219,6 +219,13 @@ def gen_communities(V: int, K: int, deg: float = 12.0, inter: float = 0.02, seed
     return g
 
 
+def seeded_permutation(n: int, seed: int) -> np.ndarray:
+    """The generators' seeded permutation of [0, n) (i -> perm[i])."""
+    out = np.zeros(max(n, 1), np.uint32)
+    _check(lib().scc_seeded_permutation(ctypes.c_uint32(n), ctypes.c_uint64(seed), _p32(out)))
+    return out[:n]
+
+
 def write_gtgraph(path: str, g: Graph) -> None:
     hg = _HostGraph(g.V, g.E, _p32(g.Fr), _p32(g.Fc), _p32(g.Br), _p32(g.Bc))
     _check(lib().scc_write_gtgraph(os.fsencode(path), ctypes.byref(hg)))
@@ -454,6 +461,45 @@ class Context:
         self.graph = g
         if g.part is not None:
             self.upload_partition(g.part)
+
+    # -- generators on the device (bit-identical to gen_* on the host) --
+    def _adopt(self, V: int, E: int, part=None) -> None:
+        z = np.zeros(0, np.uint32)
+        self.graph = Graph(V, E, z, z, z, z, part)
+
+    def gen_er_dev(self, V: int, draws: int, cycles: int = 0, cmin: int = 2, cmax: int = 256, seed: int = 1):
+        _check(lib().scc_gen_er_dev(self._ctx, ctypes.c_uint32(V), ctypes.c_uint64(draws), ctypes.c_uint32(cycles),
+                                    ctypes.c_uint32(cmin), ctypes.c_uint32(cmax), ctypes.c_uint64(seed)))
+        self._adopt(V, self._device_E())
+
+    def gen_rmat_dev(self, scale: int, E: int | None = None, a: float = 0.57, b: float = 0.19, c: float = 0.19,
+                     seed: int = 22):
+        E = 16 * (1 << scale) if E is None else E
+        _check(lib().scc_gen_rmat_dev(self._ctx, ctypes.c_uint32(scale), ctypes.c_uint64(E), ctypes.c_double(a),
+                                      ctypes.c_double(b), ctypes.c_double(c), ctypes.c_uint64(seed)))
+        self._adopt(1 << scale, E)
+
+    def gen_grid_dev(self, side: int, p: float = 0.7, seed: int = 4096):
+        _check(lib().scc_gen_grid_dev(self._ctx, ctypes.c_uint32(side), ctypes.c_double(p), ctypes.c_uint64(seed)))
+        self._adopt(side * side, self._device_E())
+
+    def gen_communities_dev(self, V: int, K: int, deg: float = 12.0, inter: float = 0.02, seed: int = 55):
+        part = np.zeros(V + 2, np.uint32)
+        _check(lib().scc_gen_communities_dev(self._ctx, ctypes.c_uint32(V), ctypes.c_uint32(K), ctypes.c_double(deg),
+                                             ctypes.c_double(inter), ctypes.c_uint64(seed), _p32(part)))
+        self._adopt(V, self._device_E(), part)
+
+    def _device_E(self) -> int:
+        V = ctypes.c_uint32(0)
+        E = ctypes.c_uint32(0)
+        _check(lib().scc_graph_size(self._ctx, ctypes.byref(V), ctypes.byref(E)))
+        return E.value
+
+    def download(self) -> Graph:
+        """The context's device graph (CSR + reverse CSR) copied to the host."""
+        hg = _HostGraph()
+        _check(lib().scc_graph_download(self._ctx, ctypes.byref(hg)))
+        return _take(hg)
 
     def upload_partition(self, part: np.ndarray) -> None:
         part = np.ascontiguousarray(part, dtype=np.uint32)

@@ -21,6 +21,11 @@ class Engine {
     scc_status upload_partition(const uint32_t *part);
     scc_status upload_device(uint32_t V, uint32_t E, const uint32_t *dFr, const uint32_t *dFc,
                              const uint32_t *dBr, const uint32_t *dBc, hipStream_t from);
+    // A graph built in place on the device (gen_dev.hip): adopt_begin sizes
+    // the arrays for V vertices and E arcs, the caller fills g.Fr/Fc/Br/Bc in
+    // the reference layout, adopt_end makes it the uploaded graph.
+    scc_status adopt_begin(uint32_t V, uint32_t E);
+    void adopt_end();
     scc_status run(const scc_opts &o, scc_stats *out);
     scc_status phase_local(uint32_t *out);
     scc_status labels(uint32_t *lab);

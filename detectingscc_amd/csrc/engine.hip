@@ -248,6 +248,24 @@ scc_status Engine::upload_partition(const uint32_t *part)
     return SCC_OK;
 }
 
+scc_status Engine::adopt_begin(uint32_t V, uint32_t E)
+{
+    if (V >= (1u << 30)) return fail(SCC_ERR_INVALID, "V=%u exceeds 2^30-1", V);
+    has_graph = has_part = ran = false;
+    scc_status s = reserve(V, E);
+    if (s != SCC_OK) return s;
+    g.V = V;
+    g.E = E;
+    return SCC_OK;
+}
+
+void Engine::adopt_end()
+{
+    has_graph = true;
+    has_part = false;
+    ran = false;
+}
+
 // Adopt a graph that is already in device memory (copied device to device).
 // Used for the quotient graph, whose CSR is built by kernels and valid by
 // construction.

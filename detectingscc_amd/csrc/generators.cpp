@@ -14,18 +14,6 @@
 
 namespace sccamd {
 
-// Seeded Fisher-Yates permutation of [0, n).
-std::vector<uint32_t> seeded_permutation(uint32_t n, uint64_t seed)
-{
-    std::vector<uint32_t> p(n);
-    for (uint32_t i = 0; i < n; i++) p[i] = i;
-    for (uint32_t i = n; i-- > 1;) {
-        uint32_t j = (uint32_t)below(rnd(seed, 9, i), (uint64_t)i + 1);
-        std::swap(p[i], p[j]);
-    }
-    return p;
-}
-
 void er_arcs(uint32_t V, uint64_t draws, uint64_t seed, std::vector<uint32_t> &s,
              std::vector<uint32_t> &d, int threads)
 {
@@ -69,43 +57,41 @@ void planted_cycles(uint32_t V, uint32_t cycles, uint32_t cmin, uint32_t cmax, u
     }
 }
 
+RmatParams rmat_params(uint32_t scale, uint32_t vcap, uint64_t E, double a, double b, double c)
+{
+    RmatParams P;
+    P.ta = (uint32_t)std::lround(a * 65536.0);
+    P.tab = (uint32_t)std::lround((a + b) * 65536.0);
+    P.tabc = (uint32_t)std::lround((a + b + c) * 65536.0);
+    P.scale = scale;
+    P.words = (scale + 3) / 4;
+    P.vcap = vcap;
+    P.E = E;
+    return P;
+}
+
 // R-MAT over [1, 2^scale] with 16-bit quadrant thresholds per level; arcs
 // with an endpoint above vcap are redrawn (counter advanced), which gives
-// R-MAT graphs of any vertex count.
+// R-MAT graphs of any vertex count.  With `permute`, both endpoints go
+// through the seeded permutation of [0, vcap).
 void rmat_arcs(uint32_t scale, uint32_t vcap, uint64_t E, double a, double b, double c,
-               uint64_t seed, const std::vector<uint32_t> *perm, std::vector<uint32_t> &s,
-               std::vector<uint32_t> &d, int threads)
+               uint64_t seed, bool permute, std::vector<uint32_t> &s, std::vector<uint32_t> &d, int threads)
 {
-    const uint32_t ta = (uint32_t)std::lround(a * 65536.0);
-    const uint32_t tab = (uint32_t)std::lround((a + b) * 65536.0);
-    const uint32_t tabc = (uint32_t)std::lround((a + b + c) * 65536.0);
-    const uint32_t words = (scale + 3) / 4;
+    const RmatParams P = rmat_params(scale, vcap, E, a, b, c);
+    const SeededPerm perm = make_perm(vcap, seed);
     size_t base = s.size();
     s.resize(base + E);
     d.resize(base + E);
     parallel_chunks(E, [&](uint64_t lo, uint64_t hi, int) {
         for (uint64_t i = lo; i < hi; i++) {
-            for (uint64_t attempt = 0;; attempt++) {
-                uint64_t ctr = i + attempt * E;
-                uint32_t r = 0, q = 0, lvl = 0;
-                for (uint32_t w = 0; w < words; w++) {
-                    uint64_t bits = rnd(seed, 10 + w, ctr);
-                    for (int k = 0; k < 4 && lvl < scale; k++, lvl++) {
-                        uint32_t x = (uint32_t)(bits >> (16 * k)) & 0xFFFF;
-                        uint32_t rb = x >= tab, cb = (x >= ta && x < tab) || x >= tabc;
-                        r = (r << 1) | rb;
-                        q = (q << 1) | cb;
-                    }
-                }
-                if (r >= vcap || q >= vcap) continue;
-                if (perm) {
-                    r = (*perm)[r];
-                    q = (*perm)[q];
-                }
-                s[base + i] = r + 1;
-                d[base + i] = q + 1;
-                break;
+            uint32_t r, q;
+            rmat_arc(P, seed, i, r, q);
+            if (permute) {
+                r = perm_at(perm, r);
+                q = perm_at(perm, q);
             }
+            s[base + i] = r + 1;
+            d[base + i] = q + 1;
         }
     }, threads);
 }
@@ -132,8 +118,7 @@ MixGraph mix_graph(uint64_t seed, uint32_t index, uint32_t lgmin, uint32_t lgmax
     } else if (m.kind == 1) {
         uint32_t scale = 1;
         while ((1ull << scale) < m.V) scale++;
-        std::vector<uint32_t> perm = seeded_permutation(m.V, gs);
-        rmat_arcs(scale, m.V, E, 0.57, 0.19, 0.19, gs, &perm, m.s, m.d, 1);
+        rmat_arcs(scale, m.V, E, 0.57, 0.19, 0.19, gs, true, m.s, m.d, 1);
     } else {
         const uint32_t cycles = (uint32_t)std::max<uint64_t>(1, E / (2 * 257));
         planted_cycles(m.V, cycles, 2, 512, gs, m.s, m.d);
@@ -160,8 +145,7 @@ MixGraph serving_graph(uint64_t seed, uint32_t index, double median_v, double si
     } else {
         uint32_t scale = 1;
         while ((1ull << scale) < m.V) scale++;
-        std::vector<uint32_t> perm = seeded_permutation(m.V, gs);
-        rmat_arcs(scale, m.V, E, 0.57, 0.19, 0.19, gs, &perm, m.s, m.d, 1);
+        rmat_arcs(scale, m.V, E, 0.57, 0.19, 0.19, gs, true, m.s, m.d, 1);
     }
     return m;
 }
@@ -227,10 +211,20 @@ scc_status scc_gen_rmat(uint32_t scale, uint64_t E, double a, double b, double c
     if (!out || scale < 1 || scale > 29 || a < 0 || b < 0 || c < 0 || a + b + c > 1.0)
         return fail(SCC_ERR_INVALID, "bad R-MAT arguments");
     uint32_t V = 1u << scale;
-    std::vector<uint32_t> perm = seeded_permutation(V, seed);
     std::vector<uint32_t> s, d;
-    rmat_arcs(scale, V, E, a, b, c, seed, &perm, s, d);
+    rmat_arcs(scale, V, E, a, b, c, seed, true, s, d);
     return finish(V, s, d, out);
+}
+
+scc_status scc_seeded_permutation(uint32_t n, uint64_t seed, uint32_t *out)
+{
+    clear_error();
+    if (!out) return fail(SCC_ERR_INVALID, "null output");
+    const SeededPerm p = make_perm(n, seed);
+    parallel_chunks(n, [&](uint64_t b, uint64_t e, int) {
+        for (uint64_t i = b; i < e; i++) out[i] = perm_at(p, (uint32_t)i);
+    });
+    return SCC_OK;
 }
 
 scc_status scc_gen_grid(uint32_t side, double p, uint64_t seed, scc_host_graph *out)
@@ -351,18 +345,20 @@ scc_status scc_gen_communities(uint32_t V, uint32_t K, double deg, double inter,
     if (!out || !part || V < 1 || K < 1 || K > V || V >= (1u << 30) || deg < 0 || inter < 0 ||
         inter > 1)
         return fail(SCC_ERR_INVALID, "bad community arguments");
-    std::vector<uint32_t> perm = seeded_permutation(V, seed);   // position -> vertex-1
-    std::vector<uint32_t> pos(V);
-    for (uint32_t i = 0; i < V; i++) pos[perm[i]] = i;
+    const SeededPerm perm = make_perm(V, seed);   // position -> vertex-1
     // community c holds positions [c*V/K, (c+1)*V/K)
     auto lo = [&](uint64_t c) { return (uint32_t)(c * V / K); };
     part[0] = 0;
     part[V + 1] = 0;
-    for (uint32_t v = 0; v < V; v++) part[v + 1] = (uint32_t)(((uint64_t)pos[v] * K) / V);
-    for (uint32_t v = 0; v < V; v++)   // integer rounding can misplace the boundary; fix it
-        while (part[v + 1] + 1 < K && pos[v] >= lo(part[v + 1] + 1)) part[v + 1]++;
-    for (uint32_t v = 0; v < V; v++)
-        while (part[v + 1] > 0 && pos[v] < lo(part[v + 1])) part[v + 1]--;
+    parallel_chunks(V, [&](uint64_t b, uint64_t e, int) {
+        for (uint64_t v = b; v < e; v++) {
+            const uint32_t pos = perm_inv(perm, (uint32_t)v);
+            uint32_t c = (uint32_t)(((uint64_t)pos * K) / V);
+            while (c + 1 < K && pos >= lo(c + 1)) c++;   // integer rounding can misplace the boundary
+            while (c > 0 && pos < lo(c)) c--;
+            part[v + 1] = c;
+        }
+    });
     const uint64_t E = (uint64_t)std::llround(deg * V);
     std::vector<uint32_t> s(E), d(E);
     parallel_chunks(E, [&](uint64_t b, uint64_t e, int) {
@@ -374,7 +370,7 @@ scc_status scc_gen_communities(uint32_t V, uint32_t K, double deg, double inter,
             } else {
                 uint32_t c = part[u + 1];
                 uint32_t a0 = lo(c), n = lo(c + 1) - a0;
-                v = perm[a0 + (uint32_t)below(rnd(seed, 32, i), n)];
+                v = perm_at(perm, a0 + (uint32_t)below(rnd(seed, 32, i), n));
             }
             s[i] = u + 1;
             d[i] = v + 1;

@@ -4,16 +4,18 @@
 #include <cstdint>
 #include <vector>
 
+#include "rng.h"
+
 namespace sccamd {
 
-std::vector<uint32_t> seeded_permutation(uint32_t n, uint64_t seed);
 void er_arcs(uint32_t V, uint64_t draws, uint64_t seed, std::vector<uint32_t> &s,
              std::vector<uint32_t> &d, int threads = 0);
 void planted_cycles(uint32_t V, uint32_t cycles, uint32_t cmin, uint32_t cmax, uint64_t seed,
                     std::vector<uint32_t> &s, std::vector<uint32_t> &d);
+RmatParams rmat_params(uint32_t scale, uint32_t vcap, uint64_t E, double a, double b, double c);
 void rmat_arcs(uint32_t scale, uint32_t vcap, uint64_t E, double a, double b, double c,
-               uint64_t seed, const std::vector<uint32_t> *perm, std::vector<uint32_t> &s,
-               std::vector<uint32_t> &d, int threads = 0);
+               uint64_t seed, bool permute, std::vector<uint32_t> &s, std::vector<uint32_t> &d,
+               int threads = 0);
 void grid_arcs(uint32_t side, double p, uint64_t seed, std::vector<uint32_t> &s,
                std::vector<uint32_t> &d);
 void csr_build_serial(uint32_t V, uint64_t E, const uint32_t *src, const uint32_t *dst, uint32_t *Fr,

@@ -108,6 +108,10 @@ scc_status scc_gen_er(uint32_t V, uint64_t draws, uint32_t cycles, uint32_t cmin
 scc_status scc_gen_rmat(uint32_t scale, uint64_t E, double a, double b, double c, uint64_t seed,
                         scc_host_graph *out);
 scc_status scc_gen_grid(uint32_t side, double p, uint64_t seed, scc_host_graph *out);
+/* The generators' seeded permutation of [0, n): out[i] for i < n (a
+ * four-round Feistel network with cycle walking, so R-MAT and community
+ * graphs can be permuted per arc on the host or the device alike). */
+scc_status scc_seeded_permutation(uint32_t n, uint64_t seed, uint32_t *out);
 scc_status scc_gen_communities(uint32_t V, uint32_t K, double deg, double inter, uint64_t seed,
                                scc_host_graph *out, uint32_t *part);
 
@@ -340,6 +344,27 @@ scc_status scc_server_poll(scc_ctx *ctx, uint64_t request_id, uint32_t *lab, uin
 scc_status scc_server_wait_all(scc_ctx *ctx);
 scc_status scc_server_stats_get(scc_ctx *ctx, scc_server_stats *stats);
 scc_status scc_server_stop(scc_ctx *ctx);
+
+/* ------------------------------------------------ device generators --- */
+/*
+ * The generators above, run on the context's device: the same counter-based
+ * draws (so each graph is bit-identical to the host generator's with the
+ * same arguments), the CSR and reverse CSR sorted on the device, and the
+ * result installed as the context's graph as scc_graph_upload would (no
+ * host copy).  scc_gen_communities_dev also installs the community
+ * partition and copies it to `part` (V+2 entries) when not NULL.  No
+ * reference counterpart: the reference only loads files (load.cpp:12-140).
+ */
+scc_status scc_gen_er_dev(scc_ctx *ctx, uint32_t V, uint64_t draws, uint32_t cycles, uint32_t cmin,
+                          uint32_t cmax, uint64_t seed);
+scc_status scc_gen_rmat_dev(scc_ctx *ctx, uint32_t scale, uint64_t E, double a, double b, double c,
+                            uint64_t seed);
+scc_status scc_gen_grid_dev(scc_ctx *ctx, uint32_t side, double p, uint64_t seed);
+scc_status scc_gen_communities_dev(scc_ctx *ctx, uint32_t V, uint32_t K, double deg, double inter,
+                                   uint64_t seed, uint32_t *part);
+/* Size of the context's graph, and a host copy of it (arrays from malloc). */
+scc_status scc_graph_size(scc_ctx *ctx, uint32_t *V, uint32_t *E);
+scc_status scc_graph_download(scc_ctx *ctx, scc_host_graph *out);
 
 /* ------------------------------------------------------- measurement --- */
 /* A plain device copy of `bytes` (read + write), `iters` times, on the

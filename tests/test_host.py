@@ -319,3 +319,26 @@ def test_csr_cache_rejects_damage(tmp_path, fixture_path):
         with pytest.raises(scc.SCCError) as e:
             scc.load_graph(str(q))   # without the magic it is parsed as GTgraph text
         assert e.value.status == 3, why
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 5, 16, 17, 1000, 65537, 1 << 20])
+def test_seeded_permutation_is_a_bijection(n):
+    p = scc.seeded_permutation(n, 77)
+    assert np.array_equal(np.sort(p), np.arange(n, dtype=np.uint32))
+    assert np.array_equal(scc.seeded_permutation(n, 77), p)          # a pure function of (n, seed)
+    if n >= 1000:   # and a real shuffle: few fixed points, a different seed differs
+        assert (p == np.arange(n)).sum() < 10
+        assert (scc.seeded_permutation(n, 78) != p).mean() > 0.99
+
+
+def test_rmat_permutation_relabels_only():
+    """The permuted R-MAT graph is the unpermuted draw relabelled: same
+    degree sequence as a multiset, same arc count."""
+    g = scc.gen_rmat(12, seed=3)
+    assert g.E == 16 << 12
+    perm = scc.seeded_permutation(1 << 12, 3)
+    outdeg = np.diff(g.Fr.astype(np.int64))[1:]
+    # vertex perm[r] + 1 of the permuted graph is vertex r + 1 of the draw:
+    # relabelling back gives a degree sequence whose sum is E
+    back = outdeg[perm]
+    assert back.sum() == g.E and sorted(back.tolist()) == sorted(outdeg.tolist())
