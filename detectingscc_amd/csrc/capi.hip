@@ -152,6 +152,24 @@ scc_status scc_phase_min_label(scc_ctx *ctx, const uint8_t *dead, uint32_t *out)
     return ctx->eng.phase_min_label(dead, out);
 }
 
+scc_status scc_layout_info(uint32_t V, uint64_t *out, uint32_t n)
+{
+    clear_error();
+    if (!out) return fail(SCC_ERR_INVALID, "null output");
+    const uint64_t v[] = {sweep_blocks(V),
+                          trim_slot_cap(V),
+                          trim_list_words(V),
+                          do_blocks(V, 1),
+                          do_blocks(V, 2),
+                          slot_cap(V, do_blocks(V, 1)),
+                          slot_cap(V, do_blocks(V, 2)),
+                          queue_words(V),
+                          (uint64_t)kSpread,
+                          narrow_cap()};
+    for (uint32_t i = 0; i < n && i < sizeof v / sizeof v[0]; i++) out[i] = v[i];
+    return SCC_OK;
+}
+
 scc_status scc_histogram(scc_ctx *ctx, uint64_t *pairs, uint32_t cap, uint32_t *npairs, double *ms)
 {
     clear_error();

@@ -366,6 +366,13 @@ scc_status scc_gen_communities_dev(scc_ctx *ctx, uint32_t V, uint32_t K, double 
 scc_status scc_graph_size(scc_ctx *ctx, uint32_t *V, uint32_t *E);
 scc_status scc_graph_download(scc_ctx *ctx, scc_host_graph *out);
 
+/* Device list layout for V vertices (no reference counterpart; for the
+ * layout checks in the tests): out[0..9] = sweep workgroups, trim-1 list
+ * slot capacity, trim-1 list words, search workgroups per direction with
+ * one and two directions, bottom-up queue slot capacity with one and two
+ * directions, queue words, spread slots, narrow-kernel frontier capacity. */
+scc_status scc_layout_info(uint32_t V, uint64_t *out, uint32_t n);
+
 /* ------------------------------------------------------- measurement --- */
 /* A plain device copy of `bytes` (read + write), `iters` times, on the
  * context's stream; *ms_per_copy = mean duration.  Calibrates the HBM byte
