@@ -59,17 +59,20 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def build_graph(name):
-    import detectingscc_amd as scc
+def generate_on_device(ctx, name):
+    """The workload generated in place on the device (bit-identical to the
+    host generators of the same arguments); returns the graph's (V, E)."""
     if name == "rmat22":
-        return scc.gen_rmat(22, seed=22)
-    if name == "rmat24":
-        return scc.gen_rmat(24, seed=24)
-    if name == "er16k":
-        return scc.gen_er(16384, 131072, cycles=64, cmin=2, cmax=256, seed=1)
-    if name == "grid4096":
-        return scc.gen_grid(4096, 0.7, seed=4096)
-    raise SystemExit(f"unknown workload {name}")
+        ctx.gen_rmat_dev(22, seed=22)
+    elif name == "rmat24":
+        ctx.gen_rmat_dev(24, seed=24)
+    elif name == "er16k":
+        ctx.gen_er_dev(16384, 131072, cycles=64, cmin=2, cmax=256, seed=1)
+    elif name == "grid4096":
+        ctx.gen_grid_dev(4096, 0.7, seed=4096)
+    else:
+        raise SystemExit(f"unknown workload {name}")
+    return ctx.graph.V, ctx.graph.E
 
 
 def model_bytes(nv, ne):
@@ -134,9 +137,9 @@ def bench_single(ctx, name, steps, warmup, algo="g", warp=1, expand=4, phases=Tr
     warm-up, `steps` timed decompositions, per-phase breakdown."""
     import detectingscc_amd as scc
     t0 = time.perf_counter()
-    g = build_graph(name)
-    log(f"built {name}: V={g.V} E={g.E} in {time.perf_counter() - t0:.1f}s")
-    ctx.upload(g)
+    V, E = generate_on_device(ctx, name)
+    gen_s = time.perf_counter() - t0
+    log(f"generated {name} on the device: V={V} E={E} in {gen_s:.2f}s")
     kw = dict(algo=algo, warp=warp, expand=expand, sizes=False)
     # untimed: |R|, |E_R|, the first search's work units, largest SCC, and
     # the labels every timed run must reproduce
@@ -160,8 +163,9 @@ def bench_single(ctx, name, steps, warmup, algo="g", warp=1, expand=4, phases=Tr
     model = (model_bytes(counted.first_fw_vertices, counted.first_fw_arcs) +
              model_bytes(counted.first_bw_vertices, counted.first_bw_arcs))
     out = {
-        "workload": WORKLOADS[name], "V": g.V, "E": g.E, "algo": algo, "warp": warp, "expand": expand,
-        "value": g.E * steps / wall, "ms_per_step": 1e3 * wall / steps,
+        "workload": WORKLOADS[name], "V": V, "E": E, "algo": algo, "warp": warp, "expand": expand,
+        "generated_on_device_s": gen_s,
+        "value": E * steps / wall, "ms_per_step": 1e3 * wall / steps,
         "device_ms_per_step": sum(s.ms_scc for s in stats) / len(stats),
         "num_sccs": counted.num_sccs, "largest_scc": counted.largest,
         "labels_checked": "every timed run's SCC count and the last run's labels equal the counted run's",
@@ -178,8 +182,7 @@ def bench_single(ctx, name, steps, warmup, algo="g", warp=1, expand=4, phases=Tr
         phased = [ctx.run(**dict(kw, phase_times=True)) for _ in range(max(1, min(steps, 5)))]
         ph = {k: sum(s.ms_phase[k] for s in phased) / len(phased) for k in scc.PHASES}
         out["phase_ms"] = {k: round(v, 4) for k, v in ph.items() if v > 0}
-        out["phase_edges_per_s"] = {k: g.E / (v * 1e-3) for k, v in ph.items() if v > 0}
-    del g
+        out["phase_edges_per_s"] = {k: E / (v * 1e-3) for k, v in ph.items() if v > 0}
     return out
 
 
@@ -204,12 +207,13 @@ def bench_matrix(ctx, g):
 
 def bench_batched(graphs=4096, steps=3, sample=1024, serial_baseline=True):
     import detectingscc_amd as scc
-    t0 = time.perf_counter()
-    b = scc.gen_batch_mix(graphs, 8, 16, 2.0, 16.0, seed=4096)
-    log(f"built the {graphs}-graph mix: {int(b.V.sum())} V, {int(b.E.sum())} arcs "
-        f"in {time.perf_counter() - t0:.1f}s")
     with scc.Context(0) as ctx:
-        ctx.batch_upload(b)
+        t0 = time.perf_counter()
+        ctx.gen_batch_mix_dev(graphs, 8, 16, 2.0, 16.0, seed=4096)
+        gen_s = time.perf_counter() - t0
+        b = ctx.batch_info()
+        log(f"generated the {graphs}-graph mix on the device: {int(b.V.sum())} V, {int(b.E.sum())} arcs "
+            f"in {gen_s:.2f}s")
         report = ctx.batch_autotune(sample=sample)
         ctx.batch_run(autotuned=True)                        # warm-up
         runs = [ctx.batch_run(autotuned=True) for _ in range(steps)]
@@ -227,7 +231,7 @@ def bench_batched(graphs=4096, steps=3, sample=1024, serial_baseline=True):
     return {
         "workload": f"{graphs} graphs, V log-uniform [2^8, 2^16], degree [2, 16], 40% ER / 40% R-MAT / "
                     "20% planted cycles 2..512, seed 4096",
-        "vertices": int(b.V.sum()), "arcs": int(b.E.sum()),
+        "vertices": int(b.V.sum()), "arcs": int(b.E.sum()), "generated_on_device_s": gen_s,
         "graphs_per_s": best.graphs_per_s, "ms": best.ms, "p50_ms": best.p50_ms, "p99_ms": best.p99_ms,
         "launches": best.launches, "steps": steps,
         "serial_graphs_per_s": serial.graphs_per_s if serial else None,
@@ -303,11 +307,11 @@ def bench_partitioned(steps=2):
     degree 12, 2% inter-community arcs), partitioned by community: local
     SCCs per partition, then the merge (quotient-graph SCC)."""
     import detectingscc_amd as scc
-    t0 = time.perf_counter()
-    g = scc.gen_communities(1 << 23, 200, 12.0, 0.02, seed=55)
-    log(f"built the community graph: V={g.V} E={g.E} in {time.perf_counter() - t0:.1f}s")
     with scc.Context(0) as ctx:
-        ctx.upload(g)
+        t0 = time.perf_counter()
+        ctx.gen_communities_dev(1 << 23, 200, 12.0, 0.02, seed=55)   # also installs the partition
+        g = ctx.graph
+        log(f"generated the community graph on the device: V={g.V} E={g.E} in {time.perf_counter() - t0:.2f}s")
         counted = ctx.run(algo="d", sizes=True)
         ctx.run(algo="d", sizes=False)
         runs = [ctx.run(algo="d", sizes=False) for _ in range(steps)]
@@ -332,6 +336,8 @@ def main():
     ap.add_argument("--expand", type=int, default=4, help="0 auto, 1 vertex, 2 vwarp, 3 LB, 4 DO")
     ap.add_argument("--quick", action="store_true", help="the headline workload alone (no extra keys, no CPU)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-serial", action="store_true",
+                    help="batch4096: skip the one-graph-per-launch baseline (for profiling the batched launch)")
     ap.add_argument("--copy-calibrate", type=int, default=0, metavar="BYTES",
                     help="only run a plain device copy of BYTES (HBM counter calibration)")
     args = ap.parse_args()
@@ -347,7 +353,8 @@ def main():
                           "GBps_read_plus_write": 2 * args.copy_calibrate / ms / 1e6}), flush=True)
         return
     if args.workload in ("batch4096", "serve"):
-        part = bench_batched(steps=args.steps) if args.workload == "batch4096" else bench_serving()
+        part = (bench_batched(steps=args.steps, serial_baseline=not args.no_serial) if args.workload == "batch4096"
+                else bench_serving())
         print(json.dumps({
             "metric": "batched graphs/s and p99 vs one-graph-per-launch" if args.workload == "batch4096"
                       else "served graphs/s and p50/p99 latency under an HBM budget",
@@ -397,7 +404,7 @@ def main():
         "labels_checked": head["labels_checked"],
     }
     if not args.no_cpu and not args.quick:
-        g = build_graph(args.workload)
+        g = ctx.download()   # the same graph, on the host
         threads = min(16, os.cpu_count() or 1)
         out["cpu_baseline"] = cpu_baseline(g, args.workload, threads)
         log(f"cpu baseline: {out['cpu_baseline']['ms_per_decomposition']:.0f} ms on {threads} threads, "

@@ -489,6 +489,34 @@ class Context:
                                              ctypes.c_double(inter), ctypes.c_uint64(seed), _p32(part)))
         self._adopt(V, self._device_E(), part)
 
+    def gen_batch_mix_dev(self, num_graphs: int, lgmin: int = 8, lgmax: int = 16, dmin: float = 2.0,
+                          dmax: float = 16.0, seed: int = 4096, copy: bool = False):
+        """The batched mix generated on the device as the context's batch;
+        with copy, also returned as a host Batch (for checking)."""
+        h = _BatchHost()
+        _check(lib().scc_gen_batch_mix_dev(self._ctx, ctypes.c_uint32(num_graphs), ctypes.c_uint32(lgmin),
+                                           ctypes.c_uint32(lgmax), ctypes.c_double(dmin), ctypes.c_double(dmax),
+                                           ctypes.c_uint64(seed), ctypes.byref(h) if copy else None))
+        if copy:
+            self._batch = _take_batch(h)
+            return self._batch
+        self._batch = self.batch_info()
+        return None
+
+    def batch_info(self) -> Batch:
+        """The context's batch sizes and offsets (no arrays)."""
+        G = ctypes.c_uint32(0)
+        _check(lib().scc_batch_info(self._ctx, ctypes.byref(G), None, None, None, None))
+        n = G.value
+        V = np.zeros(n, np.uint32)
+        E = np.zeros(n, np.uint32)
+        ro = np.zeros(n + 1, np.uint64)
+        co = np.zeros(n + 1, np.uint64)
+        _check(lib().scc_batch_info(self._ctx, ctypes.byref(G), _p32(V), _p32(E), ro.ctypes.data_as(_u64p),
+                                    co.ctypes.data_as(_u64p)))
+        z = np.zeros(0, np.uint32)
+        return Batch(V, E, ro, co, z, z, z, z)
+
     def _device_E(self) -> int:
         V = ctypes.c_uint32(0)
         E = ctypes.c_uint32(0)

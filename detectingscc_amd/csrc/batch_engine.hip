@@ -2,6 +2,7 @@
 // layout, the persistent launches, the one-graph-per-launch baseline, the
 // per-bucket autotuner and the C ABI for batches.
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -54,47 +55,32 @@ static int bucket_of(uint32_t V, uint32_t E)
     return s * 3 + d;
 }
 
-scc_status BatchEngine::upload(int device, hipStream_t stream, const scc_batch_host *b)
+// Batch layout and device buffers for graphs with V[i] vertices and E[i]
+// arcs, rows of graph i at row_off[i] and columns at col_off[i] in the
+// source; the rows go to 64-aligned slots voff[i].  Buffers grow only.
+scc_status BatchEngine::prepare(int device, hipStream_t stream, uint32_t ng, const uint32_t *Vs, const uint32_t *Es,
+                                const uint64_t *roff, const uint64_t *coff)
 {
-    if (!b || !b->num_graphs || !b->V || !b->E || !b->row_off || !b->col_off || !b->Fr || !b->Br ||
-        !b->Fc || !b->Bc)
-        return fail(SCC_ERR_INVALID, "incomplete batch description");
     dev = device;
     s1 = stream;
     BHIP(hipSetDevice(device));
     if (!stream2) BHIP(hipStreamCreateWithFlags(&stream2, hipStreamNonBlocking));
     ran = false;
-    G = b->num_graphs;
-    V.assign(b->V, b->V + G);
-    E.assign(b->E, b->E + G);
-    row_off.assign(b->row_off, b->row_off + G + 1);
+    G = ng;
+    V.assign(Vs, Vs + G);
+    E.assign(Es, Es + G);
+    row_off.assign(roff, roff + G + 1);
     voff.assign(G + 1, 0);
-    eoff.assign(b->col_off, b->col_off + G + 1);
+    eoff.assign(coff, coff + G + 1);
     nverts = narcs = 0;
     for (uint32_t i = 0; i < G; i++) {
         if (V[i] >= (1u << 30)) return fail(SCC_ERR_INVALID, "graph %u: V too large", i);
         if (eoff[i + 1] - eoff[i] != E[i] || row_off[i + 1] - row_off[i] < (uint64_t)V[i] + 2)
             return fail(SCC_ERR_INVALID, "graph %u: offsets do not match V/E", i);
-        // each graph's rows must be a valid reference-layout CSR
-        const uint32_t *Fr = b->Fr + row_off[i], *Br = b->Br + row_off[i];
-        if (Fr[0] || Fr[1] || Br[0] || Br[1] || Fr[V[i] + 1] != E[i] || Br[V[i] + 1] != E[i])
-            return fail(SCC_ERR_INVALID, "graph %u: row arrays are not in the reference layout", i);
         voff[i + 1] = voff[i] + (((uint64_t)V[i] + 2 + 63) / 64) * 64;
         nverts += V[i];
         narcs += E[i];
     }
-    std::atomic<int> bad{0};
-    parallel_chunks(G, [&](uint64_t lo, uint64_t hi, int) {
-        for (uint64_t i = lo; i < hi; i++) {
-            const uint32_t *Fr = b->Fr + row_off[i], *Br = b->Br + row_off[i];
-            const uint32_t *Fc = b->Fc + eoff[i], *Bc = b->Bc + eoff[i];
-            for (uint32_t v = 0; v <= V[i]; v++)
-                if (Fr[v] > Fr[v + 1] || Br[v] > Br[v + 1]) bad = 1;
-            for (uint32_t k = 0; k < E[i]; k++)
-                if (Fc[k] < 1 || Fc[k] > V[i] || Bc[k] < 1 || Bc[k] > V[i]) bad = 1;
-        }
-    });
-    if (bad) return fail(SCC_ERR_INVALID, "a batch graph has an invalid CSR");
     const uint64_t S = voff[G];
     const uint64_t A = eoff[G];
     // grow-only device buffers: a serving slot reuses them batch after batch
@@ -137,20 +123,12 @@ scc_status BatchEngine::upload(int device, hipStream_t stream, const scc_batch_h
                         (unsigned long long)S, (unsigned long long)A);
         }
     }
-    // rows into their 64-aligned slots, columns as given
-    stage.assign(2 * S, 0);
-    for (int which = 0; which < 2; which++) {
-        const uint32_t *src = which ? b->Br : b->Fr;
-        uint32_t *dst = stage.data() + which * S;
-        parallel_chunks(G, [&](uint64_t lo, uint64_t hi, int) {
-            for (uint64_t i = lo; i < hi; i++)
-                std::memcpy(dst + voff[i], src + row_off[i], sizeof(uint32_t) * ((size_t)V[i] + 2));
-        });
-    }
-    BHIP(hipMemcpyAsync(m_Fr, stage.data(), sizeof(uint32_t) * S, hipMemcpyHostToDevice, s1));
-    BHIP(hipMemcpyAsync(m_Br, stage.data() + S, sizeof(uint32_t) * S, hipMemcpyHostToDevice, s1));
-    BHIP(hipMemcpyAsync(m_Fc, b->Fc, sizeof(uint32_t) * A, hipMemcpyHostToDevice, s1));
-    BHIP(hipMemcpyAsync(m_Bc, b->Bc, sizeof(uint32_t) * A, hipMemcpyHostToDevice, s1));
+    return SCC_OK;
+}
+
+// Metadata to the device and the per-graph tables of the launches.
+scc_status BatchEngine::finish_upload()
+{
     BHIP(hipMemcpyAsync(m_voff, voff.data(), sizeof(uint64_t) * (G + 1), hipMemcpyHostToDevice, s1));
     BHIP(hipMemcpyAsync(m_eoff, eoff.data(), sizeof(uint64_t) * (G + 1), hipMemcpyHostToDevice, s1));
     BHIP(hipMemcpyAsync(m_V, V.data(), sizeof(uint32_t) * G, hipMemcpyHostToDevice, s1));
@@ -168,15 +146,105 @@ scc_status BatchEngine::upload(int device, hipStream_t stream, const scc_batch_h
     bucket.resize(G);
     for (uint32_t i = 0; i < G; i++) bucket[i] = bucket_of(V[i], E[i]);
     int rate_khz = 0;
-    if (hipDeviceGetAttribute(&rate_khz, hipDeviceAttributeWallClockRate, device) != hipSuccess || rate_khz <= 0)
+    if (hipDeviceGetAttribute(&rate_khz, hipDeviceAttributeWallClockRate, dev) != hipSuccess || rate_khz <= 0)
         rate_khz = 100000;
     ticks_per_ms = (double)rate_khz;
     int cus = 0;
-    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess || cus <= 0)
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
         cus = 256;
     num_cus = cus;
     tuned = false;
     return SCC_OK;
+}
+
+scc_status BatchEngine::upload(int device, hipStream_t stream, const scc_batch_host *b)
+{
+    if (!b || !b->num_graphs || !b->V || !b->E || !b->row_off || !b->col_off || !b->Fr || !b->Br ||
+        !b->Fc || !b->Bc)
+        return fail(SCC_ERR_INVALID, "incomplete batch description");
+    // each graph's rows must be a valid reference-layout CSR
+    for (uint32_t i = 0; i < b->num_graphs; i++) {
+        const uint32_t *Fr = b->Fr + b->row_off[i], *Br = b->Br + b->row_off[i];
+        if (b->row_off[i + 1] - b->row_off[i] < (uint64_t)b->V[i] + 2) break;   // prepare() reports it
+        if (Fr[0] || Fr[1] || Br[0] || Br[1] || Fr[b->V[i] + 1] != b->E[i] || Br[b->V[i] + 1] != b->E[i])
+            return fail(SCC_ERR_INVALID, "graph %u: row arrays are not in the reference layout", i);
+    }
+    scc_status st = prepare(device, stream, b->num_graphs, b->V, b->E, b->row_off, b->col_off);
+    if (st != SCC_OK) return st;
+    std::atomic<int> bad{0};
+    parallel_chunks(G, [&](uint64_t lo, uint64_t hi, int) {
+        for (uint64_t i = lo; i < hi; i++) {
+            const uint32_t *Fr = b->Fr + row_off[i], *Br = b->Br + row_off[i];
+            const uint32_t *Fc = b->Fc + eoff[i], *Bc = b->Bc + eoff[i];
+            for (uint32_t v = 0; v <= V[i]; v++)
+                if (Fr[v] > Fr[v + 1] || Br[v] > Br[v + 1]) bad = 1;
+            for (uint32_t k = 0; k < E[i]; k++)
+                if (Fc[k] < 1 || Fc[k] > V[i] || Bc[k] < 1 || Bc[k] > V[i]) bad = 1;
+        }
+    });
+    if (bad) return fail(SCC_ERR_INVALID, "a batch graph has an invalid CSR");
+    const uint64_t S = voff[G], A = eoff[G];
+    // rows into their 64-aligned slots, columns as given
+    stage.assign(2 * S, 0);
+    for (int which = 0; which < 2; which++) {
+        const uint32_t *src = which ? b->Br : b->Fr;
+        uint32_t *dst = stage.data() + which * S;
+        parallel_chunks(G, [&](uint64_t lo, uint64_t hi, int) {
+            for (uint64_t i = lo; i < hi; i++)
+                std::memcpy(dst + voff[i], src + row_off[i], sizeof(uint32_t) * ((size_t)V[i] + 2));
+        });
+    }
+    BHIP(hipMemcpyAsync(m_Fr, stage.data(), sizeof(uint32_t) * S, hipMemcpyHostToDevice, s1));
+    BHIP(hipMemcpyAsync(m_Br, stage.data() + S, sizeof(uint32_t) * S, hipMemcpyHostToDevice, s1));
+    BHIP(hipMemcpyAsync(m_Fc, b->Fc, sizeof(uint32_t) * A, hipMemcpyHostToDevice, s1));
+    BHIP(hipMemcpyAsync(m_Bc, b->Bc, sizeof(uint32_t) * A, hipMemcpyHostToDevice, s1));
+    return finish_upload();
+}
+
+uint32_t BatchEngine::info(uint32_t *Vs, uint32_t *Es, uint64_t *roff, uint64_t *coff) const
+{
+    for (uint32_t i = 0; i < G; i++) {
+        if (Vs) Vs[i] = V[i];
+        if (Es) Es[i] = E[i];
+    }
+    for (uint32_t i = 0; i <= G && G; i++) {
+        if (roff) roff[i] = row_off[i];
+        if (coff) coff[i] = eoff[i];
+    }
+    return G;
+}
+
+// Rows of graph g (V+2 entries at src + roff[g]) into slot voff[g]: one
+// workgroup per graph.
+__global__ void batch_rows_kernel(const uint32_t *src, const uint64_t *roff, const uint64_t *dvoff,
+                                  const uint32_t *Vn, uint32_t G, uint32_t *dst)
+{
+    for (uint32_t g = blockIdx.x; g < G; g += gridDim.x) {
+        const uint32_t n = Vn[g] + 2;
+        for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) dst[dvoff[g] + i] = src[roff[g] + i];
+    }
+}
+
+// A batch already in device memory in the host layout (rows of graph i at
+// row_off[i], V[i] + 2 entries each; columns at col_off[i]), valid by
+// construction (the device generators).
+scc_status BatchEngine::upload_device(int device, hipStream_t stream, uint32_t ng, const uint32_t *Vs,
+                                      const uint32_t *Es, const uint64_t *roff, const uint64_t *coff,
+                                      const uint64_t *d_roff, const uint32_t *dFr, const uint32_t *dBr,
+                                      const uint32_t *dFc, const uint32_t *dBc)
+{
+    scc_status st = prepare(device, stream, ng, Vs, Es, roff, coff);
+    if (st != SCC_OK) return st;
+    BHIP(hipMemcpyAsync(m_voff, voff.data(), sizeof(uint64_t) * (G + 1), hipMemcpyHostToDevice, s1));
+    BHIP(hipMemcpyAsync(m_V, V.data(), sizeof(uint32_t) * G, hipMemcpyHostToDevice, s1));
+    const uint32_t blocks = std::min<uint32_t>(G, 8192);
+    batch_rows_kernel<<<blocks, 256, 0, s1>>>(dFr, d_roff, m_voff, m_V, G, m_Fr);
+    batch_rows_kernel<<<blocks, 256, 0, s1>>>(dBr, d_roff, m_voff, m_V, G, m_Br);
+    if (eoff[G]) {
+        BHIP(hipMemcpyAsync(m_Fc, dFc, sizeof(uint32_t) * eoff[G], hipMemcpyDeviceToDevice, s1));
+        BHIP(hipMemcpyAsync(m_Bc, dBc, sizeof(uint32_t) * eoff[G], hipMemcpyDeviceToDevice, s1));
+    }
+    return finish_upload();
 }
 
 // Launch the graphs listed in `idx` with per-graph cfg words and workgroup
@@ -430,6 +498,16 @@ scc_status BatchEngine::labels(uint32_t *lab, uint32_t *nscc)
 using namespace sccamd;
 
 extern "C" {
+
+scc_status scc_batch_info(scc_ctx *ctx, uint32_t *num_graphs, uint32_t *V, uint32_t *E, uint64_t *row_off,
+                          uint64_t *col_off)
+{
+    clear_error();
+    if (!ctx || !num_graphs) return fail(SCC_ERR_INVALID, "null argument");
+    if (!ctx->batch) return fail(SCC_ERR_NOTREADY, "no batch uploaded");
+    *num_graphs = ctx->batch->info(V, E, row_off, col_off);
+    return SCC_OK;
+}
 
 scc_status scc_batch_upload(scc_ctx *ctx, const scc_batch_host *b)
 {

@@ -472,7 +472,10 @@ scc_status Engine::reach(bool fw, bool bw, bool single, int expand, int W, uint3
     if (!fw && !bw) return SCC_OK;
     if (expand == SCC_EXPAND_DO) return reach_do(fw, bw, single, levels_out);
     do_seeded = false;   // the seeded state is for a direction-optimised search only
-    const int K = check_every > 0 ? check_every : 2;
+    // levels queued past the end of the search are no-ops, so a look at the
+    // counters every 8 levels costs at most 7 empty launches
+    const int K = check_every > 0 ? check_every : 8;
+    zero_ctr(C_BIGF, 6);   // both directions' hub lists start empty
     const uint32_t blocks = bpd(expand == SCC_EXPAND_VWARP ? W : 1);
     uint32_t level = 0;
     for (;;) {

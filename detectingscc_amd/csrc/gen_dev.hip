@@ -12,7 +12,9 @@
 
 #include <rocprim/device/device_radix_sort.hpp>
 
+#include "batch_engine.h"
 #include "engine.h"
+#include "generators.h"
 #include "host_util.h"
 #include "rng.h"
 
@@ -276,6 +278,154 @@ struct ArcBuf {
     }
 };
 
+// ------------------------------------------------------ batch mix ---
+// The 4,096-graph mix on the device.  Every graph's arcs become 64-bit keys
+// (graph << 2b | source << b | target, b bits per vertex id) in one array
+// (and the reverse keys in another); ER draws that are self-loops become
+// the largest key, which sorts after every real one.  Two radix sorts then
+// give every graph's columns in the reference order, and binary searches
+// over the sorted keys give the graphs' column offsets and row arrays.
+struct MixJob {
+    uint64_t start;     // arc jobs: first item in the dense numbering of all arc-job items
+    uint64_t pos;       // position of the job's first item in the key arrays
+    uint64_t count;     // items
+    uint64_t seed;
+    uint32_t g, V;      // graph and its vertex count
+    uint32_t kind;      // 0 ER draws, 1 R-MAT arcs, 2 one planted cycle
+    uint32_t extra;     // cycle: its index j
+    RmatParams P;       // R-MAT only
+    SeededPerm perm;    // R-MAT only
+};
+
+__device__ __forceinline__ uint64_t mix_key(uint64_t g, uint32_t u, uint32_t v, uint32_t b)
+{
+    return (g << (2 * b)) | ((uint64_t)u << b) | v;
+}
+
+// Job of item x: the last job whose start is <= x (jobs sorted by start).
+__device__ __forceinline__ uint32_t job_of(const MixJob *jobs, uint32_t njobs, uint64_t x)
+{
+    uint32_t lo = 0, hi = njobs - 1;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi + 1) >> 1;
+        if (jobs[mid].start <= x) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+// ER draws and R-MAT arcs: one item per thread.
+__global__ void mix_arcs_kernel(const MixJob *jobs, uint32_t njobs, uint64_t total, uint32_t b, uint64_t *fwd,
+                                uint64_t *rev)
+{
+    for (uint64_t x = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; x < total;
+         x += (uint64_t)gridDim.x * blockDim.x) {
+        const MixJob &J = jobs[job_of(jobs, njobs, x)];
+        const uint64_t k = x - J.start, at = J.pos + k;
+        uint32_t u, v;
+        if (J.kind == 0) {
+            u = 1 + (uint32_t)below(rnd(J.seed, 1, 2 * k), J.V);
+            v = 1 + (uint32_t)below(rnd(J.seed, 1, 2 * k + 1), J.V);
+        } else {
+            rmat_arc(J.P, J.seed, k, u, v);
+            u = perm_at(J.perm, u) + 1;
+            v = perm_at(J.perm, v) + 1;
+        }
+        const bool keep = u != v || J.kind != 0;   // ER drops self-loops; R-MAT keeps them
+        fwd[at] = keep ? mix_key(J.g, u, v, b) : ~0ull;
+        rev[at] = keep ? mix_key(J.g, v, u, b) : ~0ull;
+    }
+}
+
+// Planted cycles: one thread per cycle; its vertices are collected in
+// `tmp` at the cycle's items, a linear scan rejecting repeats.
+__global__ void mix_cycles_kernel(const MixJob *jobs, uint32_t njobs, uint32_t b, uint32_t *tmp, uint64_t *fwd,
+                                  uint64_t *rev)
+{
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < njobs; i += gridDim.x * blockDim.x) {
+        const MixJob &J = jobs[i];
+        const uint32_t L = (uint32_t)J.count;
+        uint32_t *cyc = tmp + J.pos;
+        uint32_t have = 0;
+        for (uint64_t t = 0; have < L; t++) {
+            const uint32_t x = 1 + (uint32_t)below(rnd(J.seed, 3 + (uint64_t)J.extra, t), J.V);
+            bool seen = false;
+            for (uint32_t q = 0; q < have && !seen; q++) seen = cyc[q] == x;
+            if (!seen) cyc[have++] = x;
+        }
+        for (uint32_t q = 0; q < L; q++) {
+            const uint32_t u = cyc[q], v = cyc[(q + 1) % L];
+            fwd[J.pos + q] = mix_key(J.g, u, v, b);
+            rev[J.pos + q] = mix_key(J.g, v, u, b);
+        }
+    }
+}
+
+__device__ __forceinline__ uint64_t lower_bound_key(const uint64_t *keys, uint64_t lo, uint64_t hi, uint64_t key)
+{
+    while (lo < hi) {
+        const uint64_t mid = (lo + hi) >> 1;
+        if (keys[mid] < key) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// start[g] = the first sorted key of graph g (g = 0..G; start[G] = the
+// number of real arcs).
+__global__ void mix_starts_kernel(const uint64_t *keys, uint64_t n, uint32_t G, uint32_t b, uint64_t *start)
+{
+    for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g <= G; g += gridDim.x * blockDim.x)
+        start[g] = lower_bound_key(keys, 0, n, (uint64_t)g << (2 * b));
+}
+
+// Row arrays of every graph (V_g + 2 entries at roff[g], offsets local to
+// the graph's columns) and the columns themselves.
+__global__ void mix_rows_kernel(const uint64_t *keys, const uint64_t *start, const uint64_t *roff, uint32_t G,
+                                uint32_t b, uint32_t *R)
+{
+    const uint64_t total = roff[G];
+    for (uint64_t x = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; x < total;
+         x += (uint64_t)gridDim.x * blockDim.x) {
+        uint32_t lo = 0, hi = G - 1;
+        while (lo < hi) {
+            const uint32_t mid = (lo + hi + 1) >> 1;
+            if (roff[mid] <= x) lo = mid;
+            else hi = mid - 1;
+        }
+        const uint32_t g = lo, u = (uint32_t)(x - roff[g]);
+        R[x] = (uint32_t)(lower_bound_key(keys, start[g], start[g + 1], mix_key(g, u, 0, b)) - start[g]);
+    }
+}
+
+__global__ void mix_cols_kernel(const uint64_t *keys, uint64_t n, uint32_t b, uint32_t *C)
+{
+    const uint64_t mask = (1ull << b) - 1;
+    for (uint64_t x = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; x < n; x += (uint64_t)gridDim.x * blockDim.x)
+        C[x] = (uint32_t)(keys[x] & mask);
+}
+
+// Device buffers of one generation, freed on scope exit.
+struct DevBufs {
+    std::vector<void *> p;
+    ~DevBufs()
+    {
+        for (void *x : p)
+            if (x) (void)hipFree(x);
+    }
+    template <class T> scc_status get(T **out, size_t n)
+    {
+        void *x = nullptr;
+        if (hipMalloc(&x, sizeof(T) * std::max<size_t>(n, 1)) != hipSuccess) {
+            (void)hipGetLastError();
+            return fail(SCC_ERR_NOMEM, "Error: Not enough memory on GPU for the batch generation");
+        }
+        p.push_back(x);
+        *out = static_cast<T *>(x);
+        return SCC_OK;
+    }
+};
+
 } // namespace sccamd
 
 using namespace sccamd;
@@ -400,6 +550,160 @@ scc_status scc_gen_communities_dev(scc_ctx *ctx, uint32_t V, uint32_t K, double 
     if (s == SCC_OK) s = ctx->eng.upload_partition(hpart.data());
     if (s == SCC_OK && part) std::memcpy(part, hpart.data(), sizeof(uint32_t) * hpart.size());
     return s;
+}
+
+scc_status scc_gen_batch_mix_dev(scc_ctx *ctx, uint32_t num_graphs, uint32_t lgmin, uint32_t lgmax, double dmin,
+                                 double dmax, uint64_t seed, scc_batch_host *copy)
+{
+    CTX_DEV(ctx);
+    if (!num_graphs || lgmin < 1 || lgmax < lgmin || lgmax > 24 || dmin < 0 || dmax < dmin)
+        return fail(SCC_ERR_INVALID, "bad batch-mix arguments");
+    const uint32_t G = num_graphs;
+    hipStream_t st = ctx->eng.stream;
+    // per-graph draws on the host (they use pow and rounding, which the
+    // device need not reproduce bit for bit); jobs for the device
+    std::vector<uint32_t> V(G);
+    std::vector<MixJob> arcs, cycles;
+    uint64_t n = 0;
+    uint32_t vmax = 2;
+    for (uint32_t i = 0; i < G; i++) {
+        const MixParams m = mix_params(seed, i, lgmin, lgmax, dmin, dmax);
+        V[i] = m.V;
+        vmax = std::max(vmax, m.V);
+        MixJob J;
+        std::memset(&J, 0, sizeof J);
+        J.g = i;
+        J.V = m.V;
+        if (m.kind == 0 || m.kind == 1) {
+            J.kind = m.kind;
+            J.pos = n;
+            J.count = m.E;
+            J.seed = m.gs;
+            if (m.kind == 1) {
+                uint32_t scale = 1;
+                while ((1ull << scale) < m.V) scale++;
+                J.P = rmat_params(scale, m.V, m.E, 0.57, 0.19, 0.19);
+                J.perm = make_perm(m.V, m.gs);
+            }
+            if (J.count) arcs.push_back(J);
+            n += m.E;
+        } else {
+            uint64_t have = 0;
+            const uint32_t nc = mix_cycles(m);
+            for (uint32_t j = 0; j < nc; j++) {
+                MixJob C = J;
+                C.kind = 2;
+                C.pos = n;
+                C.count = mix_cycle_len(m, j);
+                C.seed = m.gs;
+                C.extra = j;
+                cycles.push_back(C);
+                n += C.count;
+                have += C.count;
+            }
+            if (have < m.E) {
+                J.kind = 0;
+                J.pos = n;
+                J.count = m.E - have;
+                J.seed = m.gs ^ kMixFillSalt;
+                arcs.push_back(J);
+                n += J.count;
+            }
+        }
+    }
+    uint32_t b = 1;
+    while ((1ull << b) <= vmax) b++;
+    uint32_t gbits = 1;
+    while ((1ull << gbits) <= G) gbits++;
+    if (2 * b + gbits > 63) return fail(SCC_ERR_INVALID, "batch too large for 64-bit arc keys");
+    if (n > 0xFFFFFFFFull) return fail(SCC_ERR_INVALID, "batch has too many arcs");
+    std::vector<uint64_t> roff((size_t)G + 1, 0);
+    for (uint32_t i = 0; i < G; i++) roff[i + 1] = roff[i] + V[i] + 2;
+    DevBufs B;
+    uint64_t *fwd, *rev, *fout, *rout, *dstart, *droff;
+    uint32_t *tmp, *Fr, *Br, *Fc, *Bc;
+    MixJob *djobs, *dcyc;
+    scc_status s;
+    const size_t N = std::max<uint64_t>(n, 1);
+    if ((s = B.get(&fwd, N)) || (s = B.get(&rev, N)) || (s = B.get(&fout, N)) || (s = B.get(&rout, N)) ||
+        (s = B.get(&tmp, N)) || (s = B.get(&dstart, (size_t)G + 1)) || (s = B.get(&droff, (size_t)G + 1)) ||
+        (s = B.get(&Fr, roff[G])) || (s = B.get(&Br, roff[G])) || (s = B.get(&djobs, arcs.size())) ||
+        (s = B.get(&dcyc, cycles.size())))
+        return s;
+    // the arc jobs' items numbered densely in job order
+    uint64_t arc_items = 0;
+    for (auto &J : arcs) {
+        J.start = arc_items;
+        arc_items += J.count;
+    }
+    if (!arcs.empty())
+        HIP_TRY(hipMemcpyAsync(djobs, arcs.data(), sizeof(MixJob) * arcs.size(), hipMemcpyHostToDevice, st));
+    if (!cycles.empty())
+        HIP_TRY(hipMemcpyAsync(dcyc, cycles.data(), sizeof(MixJob) * cycles.size(), hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(droff, roff.data(), sizeof(uint64_t) * (G + 1), hipMemcpyHostToDevice, st));
+    if (arc_items)
+        mix_arcs_kernel<<<gen_blocks(arc_items), GEN_BLOCK, 0, st>>>(djobs, (uint32_t)arcs.size(), arc_items, b,
+                                                                    fwd, rev);
+    if (!cycles.empty())
+        mix_cycles_kernel<<<gen_blocks(cycles.size()), 64, 0, st>>>(dcyc, (uint32_t)cycles.size(), b, tmp, fwd,
+                                                                    rev);
+    // sort both key arrays
+    const uint32_t bits = 2 * b + gbits;
+    size_t tmp_bytes = 0;
+    HIP_TRY(rocprim::radix_sort_keys(nullptr, tmp_bytes, fwd, fout, (unsigned int)N, 0u, bits, st));
+    void *sort_tmp;
+    if ((s = B.get((uint8_t **)&sort_tmp, tmp_bytes))) return s;
+    if (n) {
+        // valid keys are below 2^bits; a dropped draw's all-ones key stays the largest
+        HIP_TRY(rocprim::radix_sort_keys(sort_tmp, tmp_bytes, fwd, fout, (unsigned int)n, 0u, bits, st));
+        HIP_TRY(rocprim::radix_sort_keys(sort_tmp, tmp_bytes, rev, rout, (unsigned int)n, 0u, bits, st));
+    }
+    mix_starts_kernel<<<gen_blocks((uint64_t)G + 1), GEN_BLOCK, 0, st>>>(fout, n, G, b, dstart);
+    std::vector<uint64_t> start((size_t)G + 1);
+    HIP_TRY(hipMemcpyAsync(start.data(), dstart, sizeof(uint64_t) * (G + 1), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    const uint64_t A = start[G];
+    std::vector<uint32_t> E(G);
+    for (uint32_t i = 0; i < G; i++) E[i] = (uint32_t)(start[i + 1] - start[i]);
+    if ((s = B.get(&Fc, A)) || (s = B.get(&Bc, A))) return s;
+    mix_rows_kernel<<<gen_blocks(roff[G]), GEN_BLOCK, 0, st>>>(fout, dstart, droff, G, b, Fr);
+    mix_cols_kernel<<<gen_blocks(A), GEN_BLOCK, 0, st>>>(fout, A, b, Fc);
+    // the reverse keys' graph starts are the same (each graph has the same arcs)
+    mix_rows_kernel<<<gen_blocks(roff[G]), GEN_BLOCK, 0, st>>>(rout, dstart, droff, G, b, Br);
+    mix_cols_kernel<<<gen_blocks(A), GEN_BLOCK, 0, st>>>(rout, A, b, Bc);
+    if (!ctx->batch) ctx->batch.reset(new (std::nothrow) BatchEngine());
+    if (!ctx->batch) return fail(SCC_ERR_NOMEM, "host allocation failed");
+    s = ctx->batch->upload_device(ctx->eng.device, st, G, V.data(), E.data(), roff.data(), start.data(), droff, Fr,
+                                  Br, Fc, Bc);
+    if (s != SCC_OK || !copy) return s;
+    // a host copy in the generators' layout (for checking against them)
+    std::memset(copy, 0, sizeof *copy);
+    copy->num_graphs = G;
+    copy->V = xalloc<uint32_t>(G);
+    copy->E = xalloc<uint32_t>(G);
+    copy->row_off = xalloc<uint64_t>((size_t)G + 1);
+    copy->col_off = xalloc<uint64_t>((size_t)G + 1);
+    copy->Fr = xalloc<uint32_t>(roff[G]);
+    copy->Br = xalloc<uint32_t>(roff[G]);
+    copy->Fc = xalloc<uint32_t>(A);
+    copy->Bc = xalloc<uint32_t>(A);
+    if (!copy->V || !copy->E || !copy->row_off || !copy->col_off || !copy->Fr || !copy->Br || !copy->Fc ||
+        !copy->Bc) {
+        scc_batch_host_free(copy);
+        return fail(SCC_ERR_NOMEM, "host allocation of the batch copy failed");
+    }
+    std::memcpy(copy->V, V.data(), sizeof(uint32_t) * G);
+    std::memcpy(copy->E, E.data(), sizeof(uint32_t) * G);
+    std::memcpy(copy->row_off, roff.data(), sizeof(uint64_t) * (G + 1));
+    std::memcpy(copy->col_off, start.data(), sizeof(uint64_t) * (G + 1));
+    HIP_TRY(hipMemcpyAsync(copy->Fr, Fr, sizeof(uint32_t) * roff[G], hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(copy->Br, Br, sizeof(uint32_t) * roff[G], hipMemcpyDeviceToHost, st));
+    if (A) {
+        HIP_TRY(hipMemcpyAsync(copy->Fc, Fc, sizeof(uint32_t) * A, hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipMemcpyAsync(copy->Bc, Bc, sizeof(uint32_t) * A, hipMemcpyDeviceToHost, st));
+    }
+    HIP_TRY(hipStreamSynchronize(st));
+    return SCC_OK;
 }
 
 scc_status scc_graph_size(scc_ctx *ctx, uint32_t *V, uint32_t *E)

@@ -102,17 +102,37 @@ void rmat_arcs(uint32_t scale, uint32_t vcap, uint64_t E, double a, double b, do
 // 20% planted directed cycles of length 2..512 carrying about half of the
 // arcs plus ER arcs for the rest.  Single-threaded: callers run many
 // graphs in parallel.
-MixGraph mix_graph(uint64_t seed, uint32_t index, uint32_t lgmin, uint32_t lgmax, double dmin, double dmax)
+MixParams mix_params(uint64_t seed, uint32_t index, uint32_t lgmin, uint32_t lgmax, double dmin, double dmax)
 {
-    MixGraph m;
-    const uint64_t gs = mix64(seed * 0x9E3779B97F4A7C15ull + index + 1);
-    const double lg = lgmin + unit(rnd(gs, 40, 0)) * (double)(lgmax - lgmin);
+    MixParams m;
+    m.gs = mix64(seed * 0x9E3779B97F4A7C15ull + index + 1);
+    const double lg = lgmin + unit(rnd(m.gs, 40, 0)) * (double)(lgmax - lgmin);
     m.V = (uint32_t)std::llround(std::pow(2.0, lg));
     if (m.V < 2) m.V = 2;
-    const double deg = dmin + unit(rnd(gs, 40, 1)) * (dmax - dmin);
-    const uint64_t E = (uint64_t)std::llround(deg * m.V);
-    const double t = unit(rnd(gs, 40, 2));
+    const double deg = dmin + unit(rnd(m.gs, 40, 1)) * (dmax - dmin);
+    m.E = (uint64_t)std::llround(deg * m.V);
+    const double t = unit(rnd(m.gs, 40, 2));
     m.kind = t < 0.4 ? 0 : (t < 0.8 ? 1 : 2);
+    return m;
+}
+
+// About half of a planted-cycle graph's arcs are on cycles of mean length
+// 257 (lengths uniform in 2..512, capped at V).
+uint32_t mix_cycles(const MixParams &m) { return (uint32_t)std::max<uint64_t>(1, m.E / (2 * 257)); }
+
+uint32_t mix_cycle_len(const MixParams &m, uint32_t j)
+{
+    const uint32_t cmax = std::min<uint32_t>(512, m.V);   // planted_cycles' clamp (V >= 2)
+    return 2 + (uint32_t)below(rnd(m.gs, 2, j), (uint64_t)(cmax - 2 + 1));
+}
+
+MixGraph mix_graph(uint64_t seed, uint32_t index, uint32_t lgmin, uint32_t lgmax, double dmin, double dmax)
+{
+    const MixParams p = mix_params(seed, index, lgmin, lgmax, dmin, dmax);
+    MixGraph m;
+    m.V = p.V;
+    m.kind = p.kind;
+    const uint64_t gs = p.gs, E = p.E;
     if (m.kind == 0) {
         er_arcs(m.V, E, gs, m.s, m.d, 1);
     } else if (m.kind == 1) {
@@ -120,10 +140,9 @@ MixGraph mix_graph(uint64_t seed, uint32_t index, uint32_t lgmin, uint32_t lgmax
         while ((1ull << scale) < m.V) scale++;
         rmat_arcs(scale, m.V, E, 0.57, 0.19, 0.19, gs, true, m.s, m.d, 1);
     } else {
-        const uint32_t cycles = (uint32_t)std::max<uint64_t>(1, E / (2 * 257));
-        planted_cycles(m.V, cycles, 2, 512, gs, m.s, m.d);
+        planted_cycles(m.V, mix_cycles(p), 2, 512, gs, m.s, m.d);
         const uint64_t have = m.s.size();
-        if (have < E) er_arcs(m.V, E - have, gs ^ 0x5bd1e995ull, m.s, m.d, 1);
+        if (have < E) er_arcs(m.V, E - have, gs ^ kMixFillSalt, m.s, m.d, 1);
     }
     return m;
 }

@@ -21,6 +21,21 @@ void grid_arcs(uint32_t side, double p, uint64_t seed, std::vector<uint32_t> &s,
 void csr_build_serial(uint32_t V, uint64_t E, const uint32_t *src, const uint32_t *dst, uint32_t *Fr,
                       uint32_t *Fc, uint32_t *Br, uint32_t *Bc);
 
+// The per-graph draws of the batched mix that fix a graph's size and kind
+// (graph seed gs, V, target arc count E, kind), shared by the host and
+// device generators so both build the same graphs.
+struct MixParams {
+    uint64_t gs;
+    uint32_t V;
+    uint64_t E;
+    uint8_t kind;
+};
+MixParams mix_params(uint64_t seed, uint32_t index, uint32_t lgmin, uint32_t lgmax, double dmin, double dmax);
+// planted-cycle graphs: cycle count and each cycle's length
+uint32_t mix_cycles(const MixParams &m);
+uint32_t mix_cycle_len(const MixParams &m, uint32_t j);
+constexpr uint64_t kMixFillSalt = 0x5bd1e995ull;   // seed of the ER fill of a planted-cycle graph: gs ^ salt
+
 // One graph of the batched mix: kind 0 ER, 1 R-MAT, 2 planted cycles + ER.
 struct MixGraph {
     uint32_t V;

@@ -478,25 +478,38 @@ template <int BS> __device__ uint32_t g_canon(Lds &L, const G1 &g)
 
 } // namespace
 
-// Words of one visited bitmap held in LDS: graphs of up to 2^16 vertices
-// keep both bitmaps there, so probes and claims are LDS operations (a
-// device-wide atomic costs a share of a chip-wide rate of about 10 G/s);
-// larger graphs use their slots in global memory.  The views are generic
-// pointers, so the phase code is the same either way.
-constexpr uint32_t BVIS_LDS_WORDS = 2 * ((65536 + 64) / 64);
-// Class words of graphs of up to 2^15 vertices live in LDS as well (128 KB:
-// with the bitmaps about 145 KB of the 160 KB a workgroup may hold; the
-// kernel's registers allow one 1,024-lane workgroup per CU anyway), so the
+// LDS tiers.  A graph whose visited bitmaps fit `vis_words` words each keeps
+// them in LDS, so probes and claims are LDS operations (a device-wide
+// atomic costs a share of a chip-wide rate of about 10 G/s); a graph whose
+// class words fit `cls_words` keeps those in LDS as well, so the
 // neighbour-class tests of the trims, searches and colouring stay on the
-// CU.  Nothing outside the graph's pipeline reads them.
-constexpr uint32_t BCLS_LDS_WORDS = (1u << 15) + 2;
+// CU.  Larger graphs use their slots in global memory (the views are
+// generic pointers, so the phase code is the same either way).  The tiers
+// shrink with the workgroup so that smaller workgroups share a CU:
+//   1,024 lanes: bitmaps up to 2^16 vertices, classes up to 2^15 (about
+//                145 KB: one workgroup per CU, which its registers allow)
+//     512 lanes: bitmaps up to 2^15, classes up to 2^14 (about 74 KB: two)
+//     256 lanes: bitmaps up to 2^15, classes up to 2^13 (about 41 KB: three)
+template <int BS> struct BTier;
+template <> struct BTier<1024> {
+    static constexpr uint32_t vis_words = 2 * ((65536 + 64) / 64);
+    static constexpr uint32_t cls_words = (1u << 15) + 2;
+};
+template <> struct BTier<512> {
+    static constexpr uint32_t vis_words = 2 * ((32768 + 64) / 64);
+    static constexpr uint32_t cls_words = (1u << 14) + 2;
+};
+template <> struct BTier<256> {
+    static constexpr uint32_t vis_words = 2 * ((32768 + 64) / 64);
+    static constexpr uint32_t cls_words = (1u << 13) + 2;
+};
 
 template <int BS>
 __global__ __launch_bounds__(BS) void batch_scc_kernel(BatchDev B, uint32_t first, uint32_t count)
 {
     __shared__ Lds L;
-    __shared__ uint32_t s_vis[2][BVIS_LDS_WORDS];
-    __shared__ uint32_t s_cls[BCLS_LDS_WORDS];
+    __shared__ uint32_t s_vis[2][BTier<BS>::vis_words];
+    __shared__ uint32_t s_cls[BTier<BS>::cls_words];
     for (;;) {
         __syncthreads();
         if (threadIdx.x == 0) L.graph = atomicAdd(B.next, 1u);
@@ -511,11 +524,11 @@ __global__ __launch_bounds__(BS) void batch_scc_kernel(BatchDev B, uint32_t firs
         g.Br = B.Br + vo;
         g.Fc = B.Fc + eo;
         g.Bc = B.Bc + eo;
-        g.cls = g.V + 1 <= BCLS_LDS_WORDS ? s_cls : B.cls + vo;
+        g.cls = g.V + 1 <= BTier<BS>::cls_words ? s_cls : B.cls + vo;
         g.lab = B.lab + vo;
         g.aux = B.aux + vo;
         // g_init_vis writes 2 * ((V + 64) / 64) words of each bitmap
-        const bool lds_vis = 2 * ((g.V + 64) / 64) <= BVIS_LDS_WORDS;
+        const bool lds_vis = 2 * ((g.V + 64) / 64) <= BTier<BS>::vis_words;
         g.fvis = lds_vis ? s_vis[0] : B.fvis + vo / 32;
         g.bvis = lds_vis ? s_vis[1] : B.bvis + vo / 32;
         g.piv = B.piv + 3 * vo;

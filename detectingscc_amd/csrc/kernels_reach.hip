@@ -48,6 +48,15 @@ __device__ __forceinline__ void push_wave(bool claimed, uint32_t w, uint32_t *qo
 
 // W lanes per frontier vertex; W = 1 is the vertex-per-thread mapping of
 // fwd/bwd, W > 1 the virtual warp of fwd_warp<W> (lanes stride the list).
+// A frontier vertex of degree >= LB_BIG (a hub: one lane, or one virtual
+// warp, would walk its whole list while the rest of the grid idles) is
+// instead listed for the next level, whose launch first cuts the listed
+// hubs into LB_CHUNK-arc chunks over all its lanes (the load-balanced
+// mapping's hub lists).  The hub's neighbours join one level late, which
+// leaves every reachable set unchanged.
+constexpr uint32_t VW_HUB = LB_BIG;   // the hub lists hold at most E / LB_BIG entries
+constexpr uint32_t VW_CHUNK = 512;   // LB_CHUNK
+
 template <int W, bool SINGLE>
 __global__ __launch_bounds__(256) void expand_vw_kernel(LevelArgs a, uint32_t bpd)
 {
@@ -61,18 +70,51 @@ __global__ __launch_bounds__(256) void expand_vw_kernel(LevelArgs a, uint32_t bp
     // the counters' own words
     const uint32_t n = D.cnt[CT(L % 3)];
     uint32_t *cnt_out = &D.cnt[CT((L + 1) % 3)];
-    if (bid == 0) seg_clear(&D.cnt[CT((L + 2) % 3)], &D.cslot[CT(((L + 2) % 3) * kSpread)]);
+    if (bid == 0) {
+        seg_clear(&D.cnt[CT((L + 2) % 3)], &D.cslot[CT(((L + 2) % 3) * kSpread)]);
+        if (threadIdx.x == 0) *D.bigc_zero = 0ull;
+    }
+    const uint32_t *__restrict__ R = D.R;
+    const uint32_t *__restrict__ C = D.C;
+    // hubs of the previous level: chunk gch belongs to the item whose chunk
+    // offset is the last one <= gch
+    const unsigned long long packed = *D.pbigc;
+    const uint32_t items = (uint32_t)(packed >> 32), chunks = (uint32_t)packed;
+    for (uint32_t gch = bid; gch < chunks; gch += bpd) {
+        uint32_t lo = 0, hi = items - 1;
+        while (lo < hi) {
+            const uint32_t mid = (lo + hi + 1) >> 1;
+            if (D.pbigoff[mid] <= gch) lo = mid;
+            else hi = mid - 1;
+        }
+        const uint32_t v = D.pbig[lo];
+        const uint32_t c = SINGLE ? 0u : a.cls[v];
+        const uint32_t b = R[v] + (gch - D.pbigoff[lo]) * VW_CHUNK;
+        const uint32_t e = min(R[v + 1], b + VW_CHUNK);
+        for (uint32_t j = b + threadIdx.x; j < e; j += blockDim.x) {
+            const uint32_t w = C[j];
+            push_wave(try_claim<SINGLE>(D.vis, a.cls, w, c), w, D.qout, cnt_out);
+        }
+    }
     if (n == 0) return;
     const uint32_t g = (bid * blockDim.x + threadIdx.x) / W;
     const uint32_t ng = bpd * blockDim.x / W;
     const uint32_t sub = threadIdx.x % W;
-    const uint32_t *__restrict__ R = D.R;
-    const uint32_t *__restrict__ C = D.C;
     for (uint32_t i = g; i < n; i += ng) {
         const uint32_t v = D.qin[i];
         const uint32_t c = SINGLE ? 0u : a.cls[v];
-        const uint32_t e = R[v + 1];
-        for (uint32_t j = R[v] + sub; j < e; j += W) {
+        const uint32_t b = R[v], e = R[v + 1];
+        if (e - b >= VW_HUB) {
+            if (sub == 0) {
+                const unsigned long long nch = (e - b + VW_CHUNK - 1) / VW_CHUNK;
+                const unsigned long long old = atomicAdd(D.bigc, (1ull << 32) | nch);
+                const uint32_t item = (uint32_t)(old >> 32);
+                D.big[item] = v;
+                D.bigoff[item] = (uint32_t)old;
+            }
+            continue;
+        }
+        for (uint32_t j = b + sub; j < e; j += W) {
             const uint32_t w = C[j];
             const bool claimed = try_claim<SINGLE>(D.vis, a.cls, w, c);
             push_wave(claimed, w, D.qout, cnt_out);
@@ -505,16 +547,25 @@ __global__ __launch_bounds__(256) void expand_do_kernel(DoArgs a, uint32_t bpd)
 // for the device-wide levels that follow: the frontier it stops at is level
 // L0's input queue and count, and the search state of slot (L0-1)&1 (slot
 // 0 for L0 = 0) is the one the last expanded level would have left.  It
-// stops when the frontier is empty, when the next frontier exceeds NL_CAP
-// (the claims past NL_CAP go straight to the global queue), or when a
-// level after the first would scan more than NL_ARCS arcs.  It does nothing
+// stops when the frontier is empty, when the next frontier exceeds NL_SPILL
+// (claims past NL_CAP go straight to the global queue), or when a level
+// after the first would scan more than NL_ARCS arcs.  It does nothing
 // unless level L0 would run top-down from a top-down level's queue with no
 // hubs pending and at most NL_CAP vertices, so the host may queue it behind
 // levels whose outcome it has not seen.  At least one level is expanded
 // whenever it runs, and every level either claims a vertex or ends the
 // loop, so it ends.
+//
+// One CU's scattered loads and claims limit a level to about 12 ns per
+// frontier vertex on top of a 2.6 us chain (measured: a 20,000-vertex
+// cycle, one vertex per level, 2.6 us per level against 6.2 us device-wide;
+// the 4096^2 grid's frontiers of ~2,000 vertices, 27 us against 7.2 us), so
+// the host starts it on frontiers of at most NL_ENTER vertices and it hands
+// back beyond NL_SPILL.
 constexpr uint32_t NL_THREADS = 1024;
 constexpr uint32_t NL_CAP = 6144;                 // frontier entries held in LDS
+constexpr uint32_t NL_ENTER = 256;                // host: start on frontiers this small
+constexpr uint32_t NL_SPILL = 512;                // hand back when the next frontier is larger
 constexpr uint32_t NL_PER = NL_CAP / NL_THREADS;  // frontier entries per lane
 constexpr uint32_t NL_ARCS = 32768;               // larger levels go back to the device-wide kernel
 constexpr uint32_t NL_K = 8;                      // arcs per lane per round
@@ -637,15 +688,20 @@ __global__ __launch_bounds__(NL_THREADS) void expand_narrow_kernel(DoArgs a)
                     if (!SINGLE) wcl[k] = s_cls[o];
                 }
             }
+            // every claim of the lane in flight at once, then the appends
+            uint32_t old[NL_K];
 #pragma unroll
             for (uint32_t k = 0; k < NL_K; k++) {
                 const uint32_t j = j0 + k;
-                bool claimed = false;
-                if (j < T && (SINGLE || a.cls[w[k]] == wcl[k])) {
-                    const uint32_t bit = 1u << (w[k] & 31);
-                    claimed = !(__hip_atomic_fetch_or(&D.vis[vword(w[k])], bit, __ATOMIC_RELAXED,
-                                                      __HIP_MEMORY_SCOPE_WORKGROUP) & bit);
-                }
+                old[k] = ~0u;
+                if (j < T && (SINGLE || a.cls[w[k]] == wcl[k]))
+                    old[k] = __hip_atomic_fetch_or(&D.vis[vword(w[k])], 1u << (w[k] & 31), __ATOMIC_RELAXED,
+                                                   __HIP_MEMORY_SCOPE_WORKGROUP);
+            }
+#pragma unroll
+            for (uint32_t k = 0; k < NL_K; k++) {
+                const uint32_t j = j0 + k;
+                const bool claimed = !(old[k] & (1u << (w[k] & 31)));
                 const uint64_t m = __ballot(claimed);
                 if (m) {
                     const int leader = __ffsll((unsigned long long)m) - 1;
@@ -672,8 +728,8 @@ __global__ __launch_bounds__(NL_THREADS) void expand_narrow_kernel(DoArgs a)
         visited += n2;
         n = n2;
         cur ^= 1;
-        if (n > NL_CAP) {   // the LDS part joins the claims already in the global queue
-            for (uint32_t i = t; i < NL_CAP; i += NL_THREADS) q[i] = s_q[cur][i];
+        if (n > NL_SPILL) {   // the LDS part joins any claims already in the global queue
+            for (uint32_t i = t; i < min(n, NL_CAP); i += NL_THREADS) q[i] = s_q[cur][i];
             spilled = true;
             break;
         }
@@ -730,7 +786,7 @@ void k_narrow_levels(hipStream_t s, const DoArgs &a, bool single_class)
         expand_narrow_kernel<false, false><<<a.ndirs, NL_THREADS, 0, s>>>(a);
 }
 
-uint32_t narrow_cap() { return NL_CAP; }
+uint32_t narrow_cap() { return NL_ENTER; }
 
 uint32_t do_blocks(uint32_t V, int ndirs)
 {

@@ -427,7 +427,8 @@ void k_do_begin(hipStream_t s, uint32_t *ctr, int ndirs);
 void k_do_level(hipStream_t s, const DoArgs &a, bool single_class, uint32_t blocks_per_dir);
 // Narrow-frontier levels (one workgroup per direction) from level a.level,
 // leaving that level's input rewritten; a no-op unless that level would run
-// top-down on at most narrow_cap() vertices with no hubs pending
+// top-down from a small frontier with no hubs pending.  narrow_cap(): the
+// largest frontier the host starts it on
 void k_narrow_levels(hipStream_t s, const DoArgs &a, bool single_class);
 uint32_t narrow_cap();
 uint32_t do_blocks(uint32_t V, int ndirs);

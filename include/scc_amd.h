@@ -362,6 +362,16 @@ scc_status scc_gen_rmat_dev(scc_ctx *ctx, uint32_t scale, uint64_t E, double a, 
 scc_status scc_gen_grid_dev(scc_ctx *ctx, uint32_t side, double p, uint64_t seed);
 scc_status scc_gen_communities_dev(scc_ctx *ctx, uint32_t V, uint32_t K, double deg, double inter,
                                    uint64_t seed, uint32_t *part);
+/* The batched mix of scc_gen_batch_mix generated on the device and
+ * installed as the context's batch as scc_batch_upload would; with `copy`
+ * not NULL the batch is also copied to the host in scc_gen_batch_mix's
+ * layout (free with scc_batch_host_free). */
+scc_status scc_gen_batch_mix_dev(scc_ctx *ctx, uint32_t num_graphs, uint32_t lgmin, uint32_t lgmax, double dmin,
+                                 double dmax, uint64_t seed, scc_batch_host *copy);
+/* The context's batch: number of graphs (call with NULL arrays first),
+ * then V, E and the row/column offsets of scc_gen_batch_mix's layout. */
+scc_status scc_batch_info(scc_ctx *ctx, uint32_t *num_graphs, uint32_t *V, uint32_t *E, uint64_t *row_off,
+                          uint64_t *col_off);
 /* Size of the context's graph, and a host copy of it (arrays from malloc). */
 scc_status scc_graph_size(scc_ctx *ctx, uint32_t *V, uint32_t *E);
 scc_status scc_graph_download(scc_ctx *ctx, scc_host_graph *out);

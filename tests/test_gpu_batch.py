@@ -84,6 +84,50 @@ def test_batch_edge_graphs(ctx):
         _check_batch(ctx, b, want, f"edge {cfg}")
 
 
+TIER_SIZES = (8190, 8193, 16383, 16386, 32767, 33000, 40000, 65000, 65536)
+
+
+def _tier_graphs(dags):
+    """Graphs at and around the LDS tier boundaries of every workgroup size
+    (class words in LDS up to 2^13, 2^14 or 2^15 vertices, visited bitmaps
+    up to 2^15 or 2^16): ER with planted cycles (non-trivial SCCs), or
+    layered DAGs (all singletons; 8 layers, arcs only to later layers, so
+    trim-1 peels one in at most 8 passes -- a deep DAG makes the colouring
+    of vSlota quadratic, as in the reference)."""
+    rng = np.random.default_rng(2027)
+    gs = []
+    for V in TIER_SIZES:
+        if not dags:
+            gs.append(scc.gen_er(V, 3 * V, cycles=8, cmin=2, cmax=512, seed=V))
+            continue
+        src = rng.integers(1, V + 1, 4 * V)
+        dst = rng.integers(1, V + 1, 4 * V)
+        layer = lambda x: (x - 1) * 8 // V
+        keep = layer(src) < layer(dst)
+        gs.append(scc.from_arcs(V, src[keep], dst[keep]))
+    return gs
+
+
+@pytest.mark.parametrize("block", [256, 512, 1024])
+def test_batch_lds_tiers(ctx, block):
+    for dags, algos in ((False, ("g", "h")), (True, ("g",))):
+        gs = _tier_graphs(dags)
+        b = scc.batch_from_graphs(gs)
+        want = [orc.tarjan(g) for g in gs]
+        ctx.batch_upload(b)
+        for algo in algos:
+            ctx.batch_run(algo=algo, trim1=True, trim2=algo == "g", lanes=16, block=block)
+            _check_batch(ctx, b, want, f"tiers block {block} -a {algo} dags={dags}")
+    gs = _tier_graphs(False)
+    want = [orc.tarjan(g) for g in gs]
+    # the single-graph engine agrees graph by graph
+    with scc.Context(0) as one:
+        for g, (lab, n) in zip(gs[:4], want[:4]):
+            one.upload(g)
+            st = one.run(algo="g")
+            assert st.num_sccs == n and np.array_equal(one.labels(), lab)
+
+
 def test_batch_serial_baseline(ctx, mix):
     b, want = mix
     ctx.batch_upload(b)

@@ -66,3 +66,23 @@ def test_device_graph_decomposes(ctx):
     st = ctx.run(algo="g", expand=scc.EXPAND_DO)
     want, n = orc.tarjan(scc.gen_rmat(16, seed=16))
     assert st.num_sccs == n and np.array_equal(ctx.labels(), want)
+
+
+@pytest.mark.parametrize("graphs,lgmin,lgmax,seed", [(96, 6, 11, 20261), (4096, 8, 16, 4096)])
+def test_batch_mix(ctx, graphs, lgmin, lgmax, seed):
+    """The batched mix (40% ER, 40% R-MAT, 20% planted cycles + ER fill)
+    generated on the device equals the host generator's, array by array,
+    and the device-built batch decomposes like the host-built one."""
+    dev = ctx.gen_batch_mix_dev(graphs, lgmin, lgmax, 2.0 if graphs == 4096 else 1.5,
+                                16.0 if graphs == 4096 else 14.0, seed=seed, copy=True)
+    host = scc.gen_batch_mix(graphs, lgmin, lgmax, 2.0 if graphs == 4096 else 1.5,
+                             16.0 if graphs == 4096 else 14.0, seed=seed)
+    for a in ("V", "E", "row_off", "col_off", "Fr", "Fc", "Br", "Bc"):
+        assert np.array_equal(getattr(dev, a), getattr(host, a)), f"{graphs}-graph mix: {a} differs"
+    if graphs == 96:
+        from oracle import pyoracle as orc
+        ctx.batch_run(algo="g", lanes=16, block=1024)
+        labs, nscc = ctx.batch_labels()
+        for i in range(graphs):
+            want, n = orc.tarjan(host.graph(i))
+            assert nscc[i] == n and np.array_equal(labs[i], want), f"graph {i}"
