@@ -29,7 +29,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 CXX = os.environ.get("CXX", "g++")
 
 HIP_SOURCES = ["kernels_sweep.hip", "kernels_reach.hip", "kernels_merge.hip", "kernels_batch.hip",
-               "engine.hip", "batch_engine.hip", "server.hip", "capi.hip", "measure.hip", "gen_dev.hip"]
+               "engine.hip", "batch_engine.hip", "server.hip", "capi.hip", "measure.hip", "noalloc.hip", "gen_dev.hip"]
 HOST_SOURCES = ["graph_io.cpp", "generators.cpp", "dropin.cpp"]
 
 HIP_FLAGS = [

@@ -11,6 +11,9 @@ struct scc_ctx;
 namespace sccamd {
 
 const char *phase_name(int p);
+// Looks up this device's address of the allocation-free entry point's
+// scratch array (noalloc.hip); called once per device at context creation.
+scc_status noalloc_init(int device);
 
 class Engine {
   public:

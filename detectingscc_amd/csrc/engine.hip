@@ -83,6 +83,8 @@ scc_status Engine::init(int dev)
     if (dev < 0 || dev >= n) return fail(SCC_ERR_INVALID, "device %d out of range (0..%d)", dev, n - 1);
     device = dev;
     HIP_TRY(hipSetDevice(dev));
+    scc_status ns = noalloc_init(dev);
+    if (ns != SCC_OK) return ns;
     HIP_TRY(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
     HIP_TRY(hipHostMalloc((void **)&h_ctr, CT(kCtrLines) * sizeof(uint32_t), hipHostMallocDefault));
     HIP_TRY(hipMalloc((void **)&st.ctr, CT(kCtrLines) * sizeof(uint32_t)));

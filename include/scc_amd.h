@@ -223,6 +223,18 @@ scc_status scc_solve(scc_ctx *ctx, const scc_host_graph *g, const uint32_t *part
  * counterpart there. */
 scc_status scc_histogram(scc_ctx *ctx, uint64_t *pairs, uint32_t cap, uint32_t *npairs, double *ms);
 
+/* Allocation-free variant of scc_histogram.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Same outputs as scc_histogram (pairs ascending by size, up
+ * to `cap` of them; *npairs = distinct sizes), no timing.  It calls no
+ * hipMalloc, hipFree or other allocating routine: its staging lives in a
+ * static device array of 262,400 bytes per GPU (held until the process
+ * ends), sized for at most 16,384 distinct SCC sizes, so graphs of more
+ * than 134,225,920 vertices are rejected with SCC_ERR_INVALID.  Every call
+ * on a GPU shares that array: calls that overlap on one GPU race. */
+scc_status scc_histogram_noalloc(scc_ctx *ctx, uint64_t *pairs, uint32_t cap, uint32_t *npairs);
+
 /* SCC size histogram from labels: pairs (size, count) ascending by size,
  * at most `cap` pairs; *npairs receives the number of distinct sizes. */
 scc_status scc_size_histogram(uint32_t V, const uint32_t *lab, uint64_t *pairs, uint32_t cap,
