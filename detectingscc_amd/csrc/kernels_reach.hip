@@ -138,6 +138,9 @@ constexpr uint32_t LB_CHUNK = 512;
 constexpr uint32_t LB_SLICES = 16;   // most workgroups sharing one tile
 constexpr uint32_t LB_DIRECT_N = 8192;   // frontier and mean degree of claims
 constexpr uint32_t LB_DIRECT_DEG = 4;    // without a visited-word read first
+constexpr uint32_t LB_HOPS = 64;         // local continuation: hops per workgroup and launch
+constexpr uint32_t HOP_LANE_DEG = 4;     // a continuation hop runs a vertex per lane up to this degree
+static_assert(LB_TILE * HOP_LANE_DEG <= QBUF, "a lane hop's claims fit the LDS queue");
 
 // Exclusive scan of x over the 256-lane workgroup; returns the prefix and
 // writes the total to *total.  `wsum` is LDS scratch of 4 words.
@@ -287,6 +290,124 @@ __device__ __forceinline__ void lb_level(const DirArgs &D, const uint32_t *cls, 
             lq_flush(q, cnt_out, D.qout, false);
         }
         __syncthreads();   // LDS reuse by the next tile
+    }
+    // Local continuation.  In a narrow single-class level (a high-diameter
+    // graph's search) each workgroup goes on expanding the vertices it has
+    // just claimed, for up to LB_HOPS more hops, while they fit one tile and
+    // carry at most LB_DIRECT_DEG * LB_TILE arcs; what it claims in the last
+    // hop, or when it stops early, goes to the next level's queue as usual.
+    // Reachability does not depend on the order of expansion, and each
+    // vertex is still claimed once, so the reachable sets are unchanged;
+    // the search needs far fewer launches (each a chain of global round
+    // trips) when the frontier stays narrow for thousands of levels.
+    //
+    // A hop whose vertices all have at most HOP_LANE_DEG arcs (a grid, a
+    // sparse tail) runs one vertex per lane: no scan, and each claim's row
+    // offsets are loaded beside its claim atomic and kept in LDS with the
+    // queue entry, so a hop is two dependent global round trips (column,
+    // then claim and rows) instead of three.
+    if (SINGLE && n <= LB_DIRECT_N) {
+        uint32_t *s_hb = s_off, *s_he = s_beg;   // rows of queue entries < LB_TILE
+        uint32_t *const vis = D.vis;
+        bool rows = false;                        // entries carry their rows
+        for (uint32_t hop = 0; hop < LB_HOPS; hop++) {
+            __syncthreads();
+            const uint32_t m = *q.n;
+            if (m == 0 || m > LB_TILE) break;
+            const uint32_t t = threadIdx.x;
+            const uint32_t v = t < m ? q.buf[t] : 0u;
+            uint32_t beg = 0, end = 0;
+            if (t < m) {
+                beg = rows ? s_hb[t] : R[v];
+                end = rows ? s_he[t] : R[v + 1];
+            }
+            const uint32_t deg = end - beg;
+            // every lane has read its entry and rows once this barrier passes
+            if (!__syncthreads_or(deg > HOP_LANE_DEG)) {
+                if (t == 0) *q.n = 0;
+                if (COUNT) {
+                    wc.n[W_TDV] += t < m;
+                    wc.n[W_TDE] += deg;
+                }
+                __syncthreads();
+                // Straight-line per lane with arcs: every load and claim is
+                // issued before any result is waited for (a claim per branch
+                // would wait for the previous one).  Slots past the degree
+                // repeat the last arc with an empty mask, which changes
+                // nothing and claims nothing.
+                uint32_t w[HOP_LANE_DEG] = {}, rb[HOP_LANE_DEG] = {}, re[HOP_LANE_DEG] = {};
+                uint32_t old[HOP_LANE_DEG] = {};
+                bool claimed[HOP_LANE_DEG];
+                if (deg > 0) {
+#pragma unroll
+                    for (uint32_t k = 0; k < HOP_LANE_DEG; k++) w[k] = C[beg + min(k, deg - 1)];
+#pragma unroll
+                    for (uint32_t k = 0; k < HOP_LANE_DEG; k++)
+                        old[k] = atomicOr(&vis[vword(w[k])], k < deg ? 1u << (w[k] & 31) : 0u);
+#pragma unroll
+                    for (uint32_t k = 0; k < HOP_LANE_DEG; k++) {
+                        rb[k] = R[w[k]];
+                        re[k] = R[w[k] + 1];
+                    }
+                }
+#pragma unroll
+                for (uint32_t k = 0; k < HOP_LANE_DEG; k++)
+                    claimed[k] = k < deg && !(old[k] & (1u << (w[k] & 31)));
+#pragma unroll
+                for (uint32_t k = 0; k < HOP_LANE_DEG; k++) {
+                    if (COUNT) wc.n[W_CLAIM] += claimed[k];
+                    const uint64_t act = __ballot(1), mk = __ballot(claimed[k]);
+                    if (!mk) continue;
+                    const int leader = __ffsll((unsigned long long)act) - 1;
+                    uint32_t base = 0;
+                    if ((int)wlane() == leader) base = atomicAdd(q.n, (uint32_t)__popcll(mk));
+                    base = __shfl(base, leader);
+                    if (claimed[k]) {
+                        const uint32_t pos = base + (uint32_t)__popcll(mk & ((1ull << wlane()) - 1));
+                        q.buf[pos] = w[k];   // pos < LB_TILE * HOP_LANE_DEG <= QBUF
+                        if (pos < LB_TILE) {
+                            s_hb[pos] = rb[k];
+                            s_he[pos] = re[k];
+                        }
+                    }
+                }
+                rows = true;
+                continue;
+            }
+            uint32_t total;
+            const uint32_t off = block_exscan256(deg, s_wsum, &total);
+            if (total > LB_DIRECT_DEG * LB_TILE) break;   // uniform: the queue is untouched
+            s_off[t] = off;
+            s_beg[t] = beg;
+            if (t == 0) s_off[LB_TILE] = total;
+            __syncthreads();   // every lane holds its entry before the queue is reused
+            if (t == 0) *q.n = 0;
+            if (COUNT) wc.n[W_TDV] += t < m;
+            __syncthreads();
+            for (uint32_t j0 = 0; j0 < total; j0 += LB_TILE) {
+                const uint32_t j = j0 + t;
+                bool claimed = false;
+                uint32_t w = 0;
+                if (j < total) {
+                    uint32_t lo = 0, hi = LB_TILE - 1;
+                    while (lo < hi) {
+                        const uint32_t mid = (lo + hi + 1) >> 1;
+                        if (s_off[mid] <= j) lo = mid;
+                        else hi = mid - 1;
+                    }
+                    w = C[s_beg[lo] + (j - s_off[lo])];
+                    const uint32_t bit = 1u << (w & 31);
+                    claimed = !(atomicOr(&D.vis[vword(w)], bit) & bit);
+                    if (COUNT) {
+                        wc.n[W_TDE]++;
+                        wc.n[W_CLAIM] += claimed;
+                    }
+                }
+                lq_push(q, claimed, w);
+                lq_flush(q, cnt_out, D.qout, false);
+            }
+            rows = false;
+        }
     }
     lq_flush(q, cnt_out, D.qout, true);
 }
