@@ -90,7 +90,10 @@ def profiled_traffic(workload):
         return None, None
     with open(p) as f:
         d = json.load(f)
-    return d.get("hbm_bytes_per_search"), f"profiles/reach_traffic_{workload}.md ({d.get('command', '')})"
+    if d.get("workload") != WORKLOADS.get(workload):   # a profile of another configuration
+        return None, None
+    return d.get("hbm_bytes_per_search"), (f"profiles/reach_traffic_{workload}.md: rocprofv3 FETCH_SIZE + "
+                                           "WRITE_SIZE over this workload's first search, copy-calibrated")
 
 
 def host_info():
@@ -178,6 +181,11 @@ def bench_single(ctx, name, steps, warmup, algo="g", warp=1, expand=4, phases=Tr
         "bw": [counted.first_bw_vertices, counted.first_bw_arcs],
         "histogram_device_ms": hist_ms, "distinct_scc_sizes": len(hist),
     }
+    traffic, src = profiled_traffic(name)
+    if traffic:
+        out["search_traffic_bytes"] = traffic
+        out["search_traffic_GBps"] = traffic / (fr_ms * 1e6) if fr_ms > 0 else None
+        out["search_traffic_source"] = src
     if phases:
         phased = [ctx.run(**dict(kw, phase_times=True)) for _ in range(max(1, min(steps, 5)))]
         ph = {k: sum(s.ms_phase[k] for s in phased) / len(phased) for k in scc.PHASES}
