@@ -107,6 +107,44 @@ template <int BS> __device__ void g_init_vis(const G1 &g)
     }
 }
 
+// Per-vertex list walks: a step loads GK columns and then issues every
+// probe of the step before using any (indexes past the end repeat the last
+// arc, which changes no minimum, maximum or existence test).  A walk of
+// one arc per step waits for two dependent loads per arc.
+constexpr uint32_t GK = 4;
+
+// whether some arc of col[i..e) leads to a vertex of class c
+__device__ __forceinline__ bool g_any_cls(const uint32_t *C, uint32_t i, uint32_t e, const uint32_t *cls,
+                                          uint32_t c)
+{
+    bool hit = false;
+    for (; !hit && i < e; i += GK) {
+        uint32_t w[GK];
+#pragma unroll
+        for (uint32_t k = 0; k < GK; k++) w[k] = C[min(i + k, e - 1)];
+#pragma unroll
+        for (uint32_t k = 0; k < GK; k++) hit |= cls[w[k]] == c;
+    }
+    return hit;
+}
+
+// m combined with a[w] over the arcs col[i..e): the largest (MAX) or the smallest
+template <bool MAX>
+__device__ __forceinline__ uint32_t g_fold(const uint32_t *C, uint32_t i, uint32_t e, const uint32_t *a, uint32_t m)
+{
+    for (; i < e; i += GK) {
+        uint32_t w[GK];
+#pragma unroll
+        for (uint32_t k = 0; k < GK; k++) w[k] = C[min(i + k, e - 1)];
+#pragma unroll
+        for (uint32_t k = 0; k < GK; k++) {
+            const uint32_t x = a[w[k]];
+            m = MAX ? (x > m ? x : m) : (x < m ? x : m);
+        }
+    }
+    return m;
+}
+
 // trim-1 (scc_kernels.cu:187-233) to its fixpoint or one pass
 template <int BS> __device__ uint32_t g_trim1(Lds &L, const G1 &g, bool fixpoint)
 {
@@ -116,14 +154,8 @@ template <int BS> __device__ uint32_t g_trim1(Lds &L, const G1 &g, bool fixpoint
         for (uint32_t v = threadIdx.x + 1; v <= g.V; v += BS) {
             const uint32_t c = g.cls[v];
             if (c == DEAD) continue;
-            bool keep = false;
-            for (uint32_t i = g.Br[v], e = g.Br[v + 1]; i < e; i++)
-                if (g.cls[g.Bc[i]] == c) { keep = true; break; }
-            if (keep) {
-                keep = false;
-                for (uint32_t i = g.Fr[v], e = g.Fr[v + 1]; i < e; i++)
-                    if (g.cls[g.Fc[i]] == c) { keep = true; break; }
-            }
+            bool keep = g_any_cls(g.Bc, g.Br[v], g.Br[v + 1], g.cls, c);
+            if (keep) keep = g_any_cls(g.Fc, g.Fr[v], g.Fr[v + 1], g.cls, c);
             if (!keep) {
                 g.lab[v] = v | LAB_FINAL;
                 __hip_atomic_store(&g.cls[v], DEAD, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
@@ -204,17 +236,43 @@ template <int BS> __device__ void g_reach(Lds &L, const G1 &g, bool do_bw, bool 
             const uint32_t *qin = g.queue(2 * d + in);
             uint32_t *qout = g.queue(2 * d + (in ^ 1));
             uint32_t *cnt = &L.cnt[d ? 3 : 1];
-            for (uint32_t i = grp; i < n; i += ngrp) {
-                const uint32_t v = qin[i];
-                const uint32_t c = single ? 0u : g.cls[v];
-                const uint32_t e = R[v + 1];
-                for (uint32_t j = R[v] + sub; j < e; j += W) {
-                    const uint32_t w = C[j];
-                    const uint32_t bit = 1u << (w & 31);
-                    bool claimed = false;
-                    if (!(vis[w >> 5] & bit) && (single || g.cls[w] == c))
-                        claimed = !(atomicOr(&vis[w >> 5], bit) & bit);
-                    lpush(claimed, w, qout, cnt);
+            // two frontier vertices per lane group in flight: their queue
+            // entries, rows, columns and probes are loaded side by side
+            // (vertex 0, the dummy, stands for "none": its row is empty)
+            for (uint32_t i0 = grp; i0 < n; i0 += 2 * ngrp) {
+                uint32_t v[2], b[2], e[2], c[2];
+#pragma unroll
+                for (uint32_t u = 0; u < 2; u++) v[u] = i0 + u * ngrp < n ? qin[i0 + u * ngrp] : 0u;
+#pragma unroll
+                for (uint32_t u = 0; u < 2; u++) {
+                    b[u] = R[v[u]];
+                    e[u] = R[v[u] + 1];
+                    c[u] = single ? 0u : g.cls[v[u]];
+                }
+                const uint32_t d0 = e[0] - b[0], d1 = e[1] - b[1];
+                const uint32_t dm = d0 > d1 ? d0 : d1;
+                for (uint32_t j = sub; j < dm; j += W) {
+                    uint32_t w[2] = {0u, 0u};
+                    bool ok[2] = {j < d0, j < d1};
+#pragma unroll
+                    for (uint32_t u = 0; u < 2; u++)
+                        if (ok[u]) w[u] = C[b[u] + j];
+                    uint32_t word[2] = {~0u, ~0u}, wc[2] = {0u, 0u};
+#pragma unroll
+                    for (uint32_t u = 0; u < 2; u++)
+                        if (ok[u]) {
+                            word[u] = vis[w[u] >> 5];
+                            if (!single) wc[u] = g.cls[w[u]];
+                        }
+                    bool claimed[2] = {false, false};
+#pragma unroll
+                    for (uint32_t u = 0; u < 2; u++) {
+                        const uint32_t bit = 1u << (w[u] & 31);
+                        if (ok[u] && !(word[u] & bit) && (single || wc[u] == c[u]))
+                            claimed[u] = !(atomicOr(&vis[w[u] >> 5], bit) & bit);
+                    }
+#pragma unroll
+                    for (uint32_t u = 0; u < 2; u++) lpush(claimed[u], w[u], qout, cnt);
                 }
             }
         }
@@ -348,11 +406,7 @@ template <int BS> __device__ void g_minlabel(Lds &L, const G1 &g)
         for (uint32_t v = threadIdx.x + 1; v <= g.V; v += BS) {
             const uint32_t m0 = g.cls[v];
             if (m0 == DEAD) continue;
-            uint32_t m = m0;
-            for (uint32_t i = g.Fr[v], e = g.Fr[v + 1]; i < e; i++) {
-                const uint32_t x = g.cls[g.Fc[i]];
-                m = x < m ? x : m;
-            }
+            uint32_t m = g_fold<false>(g.Fc, g.Fr[v], g.Fr[v + 1], g.cls, m0);
             for (int it = 0; it < 32; it++) {
                 const uint32_t x = g.cls[m];
                 if (x >= m) break;
@@ -385,11 +439,7 @@ template <int BS> __device__ void g_color_loop(Lds &L, const G1 &g, uint32_t W, 
             for (uint32_t v = threadIdx.x + 1; v <= g.V; v += BS) {
                 const uint32_t m0 = g.aux[v];
                 if (!m0) continue;
-                uint32_t m = m0;
-                for (uint32_t i = g.Fr[v], e = g.Fr[v + 1]; i < e; i++) {
-                    const uint32_t x = g.aux[g.Fc[i]];
-                    m = x > m ? x : m;
-                }
+                uint32_t m = g_fold<true>(g.Fc, g.Fr[v], g.Fr[v + 1], g.aux, m0);
                 for (int it = 0; it < 32; it++) {
                     const uint32_t x = g.aux[m];
                     if (x <= m) break;
