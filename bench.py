@@ -197,20 +197,32 @@ def bench_single(ctx, name, steps, warmup, algo="g", warp=1, expand=4, phases=Tr
 def bench_matrix(ctx, g):
     """The 64-run matrix on the uploaded R-MAT scale 22 graph: {g, h} x
     trims, {x, y} x W in {1..64} x trims, each with the reference's own
-    mapping (vertex per lane; virtual warp of W lanes for the first search).
-    Device ms of one decomposition each (after one warm-up run)."""
-    res = {}
+    mapping (vertex per lane; virtual warp of W lanes for the first search),
+    then the 16 (algorithm, trims) runs with the automatic expansion a
+    drop-in caller gets (direction-optimised; W does not apply).  Device ms
+    of one decomposition each (after one warm-up run)."""
+    import detectingscc_amd as scc
     n0 = None
+
+    def one(a, w, p, q, expand):
+        nonlocal n0
+        ctx.run(algo=a, warp=w, trim1=p, trim2=q, expand=expand, sizes=False)
+        st = ctx.run(algo=a, warp=w, trim1=p, trim2=q, expand=expand, sizes=False)
+        n0 = st.num_sccs if n0 is None else n0
+        assert st.num_sccs == n0, (a, w, p, q, expand, st.num_sccs, n0)
+        return round(st.ms_scc, 3)
+
     runs = ([(a, 1, p, q) for a in "gh" for p in (1, 0) for q in (1, 0)] +
             [(a, w, p, q) for a in "xy" for w in (1, 2, 4, 8, 16, 32, 64) for p in (1, 0) for q in (1, 0)])
-    for a, w, p, q in runs:
-        ctx.run(algo=a, warp=w, trim1=p, trim2=q, sizes=False)
-        st = ctx.run(algo=a, warp=w, trim1=p, trim2=q, sizes=False)
-        n0 = st.num_sccs if n0 is None else n0
-        assert st.num_sccs == n0, (a, w, p, q, st.num_sccs, n0)
-        res[f"-a {a} -w {w} -p {p} -q {q}"] = round(st.ms_scc, 3)
+    res = {f"-a {a} -w {w} -p {p} -q {q}": one(a, w, p, q, scc.EXPAND_VWARP if a in "xy" else scc.EXPAND_VERTEX)
+           for a, w, p, q in runs}
+    dflt = {f"-a {a} -p {p} -q {q}": one(a, 1, p, q, scc.EXPAND_AUTO)
+            for a in "ghxy" for p in (1, 0) for q in (1, 0)}
     return {"ms": res, "edges_per_s_best": g.E / (min(res.values()) * 1e-3),
-            "edges_per_s_worst": g.E / (max(res.values()) * 1e-3)}
+            "edges_per_s_worst": g.E / (max(res.values()) * 1e-3),
+            "default_expansion_ms": dflt,
+            "default_expansion_edges_per_s_best": g.E / (min(dflt.values()) * 1e-3),
+            "default_expansion_edges_per_s_worst": g.E / (max(dflt.values()) * 1e-3)}
 
 
 def bench_batched(graphs=4096, steps=3, sample=1024, serial_baseline=True):

@@ -533,19 +533,57 @@ __device__ __forceinline__ void bu_level(const DirArgs &D, const uint32_t *__res
             c[h] = (SINGLE || !cand[h]) ? 0u : cls[u[h]];
             pf[h] = cand[h] ? PF[u[h]] : 0u;
         }
+        // Every chunk's parent probe is issued before any result is used,
+        // and then every chunk's row offsets, without per-lane branches: a
+        // lane with nothing to load reads vertex 0's entries (the dummy:
+        // group 0 of the bitmap, rows 0 and 1), which are always in bounds.
+        // A wave with nothing to probe (or no row to read) skips the loads.
+        // With a probe and its rows per chunk in one branch, each load
+        // waited for the one before it.
+        bool anyp = false;
+#pragma unroll
+        for (uint32_t h = 0; h < BU_CHUNKS; h++) anyp |= pf[h] != 0;
+        uint64_t pw[BU_CHUNKS] = {};
+        if (__ballot(anyp)) {
+#pragma unroll
+            for (uint32_t h = 0; h < BU_CHUNKS; h++) pw[h] = reinterpret_cast<const uint64_t *>(D.vis)[pf[h] >> 5];
+        }
+        bool more[BU_CHUNKS];
 #pragma unroll
         for (uint32_t h = 0; h < BU_CHUNKS; h++) {
-            found[h] = pf[h] && visited_parent<SINGLE>(D.vis, cls, pf[h], c[h]);
+            // visited in this search (visited-or-finished and not finished),
+            // and of the candidate's class
+            const bool vp = ((uint32_t)pw[h] & ~(uint32_t)(pw[h] >> 32)) & (1u << (pf[h] & 31));
+            found[h] = pf[h] && vp && (SINGLE || cls[pf[h]] == c[h]);
+            more[h] = pf[h] && !found[h];
+        }
+        bool anym = false;
+#pragma unroll
+        for (uint32_t h = 0; h < BU_CHUNKS; h++) {
+            b[h] = e[h] = 0;
+            anym |= more[h];
+        }
+        if (__ballot(anym)) {
+#pragma unroll
+            for (uint32_t h = 0; h < BU_CHUNKS; h++) {
+                const uint32_t r = more[h] ? u[h] : 0u;
+                b[h] = PR[r];
+                e[h] = PR[r + 1];
+            }
+#pragma unroll
+            for (uint32_t h = 0; h < BU_CHUNKS; h++) {
+                if (!more[h]) b[h] = e[h] = 0;
+            }
+        }
+#pragma unroll
+        for (uint32_t h = 0; h < BU_CHUNKS; h++) {
             hits[h] = __ballot(found[h]);
-            const bool more = pf[h] && !found[h];
-            b[h] = more ? PR[u[h]] : 0u;
-            e[h] = more ? PR[u[h] + 1] : 0u;
-            need_long[h] = more && e[h] - b[h] > BU_SHORT;
+            need_long[h] = more[h] && e[h] - b[h] > BU_SHORT;
             if (COUNT) {
                 wc.n[W_BUS] += u[h] <= V;
                 wc.n[W_BUC] += cand[h];
                 wc.n[W_BUF] += pf[h] != 0;
-                wc.n[W_BUR] += more;
+                wc.n[W_BUR] += more[h];
                 wc.n[W_CLAIM] += found[h];
             }
         }

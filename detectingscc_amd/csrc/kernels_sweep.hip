@@ -172,6 +172,27 @@ __device__ __forceinline__ bool same_live(const uint32_t *cls, const uint32_t *l
     return cls[w] == c;
 }
 
+// Whether any of the arcs col[i..e) leads to a live same-class vertex.  A
+// walk step loads TRIM_SCAN columns and then probes them all (indexes past
+// the end repeat the last arc), so a vertex whose neighbours were mostly
+// removed does not hold its wave for two dependent loads per arc.
+constexpr uint32_t TRIM_SCAN = 4;
+
+template <bool BITS>
+__device__ __forceinline__ bool any_live(const uint32_t *__restrict__ col, uint32_t i, uint32_t e,
+                                         const uint32_t *cls, const uint32_t *live, uint32_t c)
+{
+    bool hit = false;
+    for (; !hit && i < e; i += TRIM_SCAN) {
+        uint32_t w[TRIM_SCAN];
+#pragma unroll
+        for (uint32_t k = 0; k < TRIM_SCAN; k++) w[k] = col[min(i + k, e - 1)];
+#pragma unroll
+        for (uint32_t k = 0; k < TRIM_SCAN; k++) hit |= same_live<BITS>(cls, live, w[k], c);
+    }
+    return hit;
+}
+
 // v (live, class c) keeps a live same-class in-neighbour and out-neighbour.
 // The first arc each way is probed first: pin = B1[v] and pout = F1[v],
 // loaded with the row offsets (coalesced, one dependent step to the
@@ -185,13 +206,12 @@ __device__ __forceinline__ bool trim1_keep_at(const uint32_t *__restrict__ Fr, c
                                               const uint32_t *cls, const uint32_t *live, uint32_t c, uint32_t v,
                                               uint32_t pin, uint32_t pout)
 {
-    bool kin = pin && same_live<BITS>(cls, live, pin, c);
-    bool kout = pout && same_live<BITS>(cls, live, pout, c);
-    if (!kin && pin)
-        for (uint32_t i = Br[v] + 1, be = Br[v + 1]; !kin && i < be; i++) kin = same_live<BITS>(cls, live, Bc[i], c);
-    if (kin && !kout && pout)
-        for (uint32_t i = Fr[v] + 1, fe = Fr[v + 1]; !kout && i < fe; i++)
-            kout = same_live<BITS>(cls, live, Fc[i], c);
+    // vertex 0 (the first neighbour of an empty list) is dead in the class
+    // words and the live bitmap, so both probes go out unconditionally
+    bool kin = same_live<BITS>(cls, live, pin, c);
+    bool kout = same_live<BITS>(cls, live, pout, c);
+    if (!kin && pin) kin = any_live<BITS>(Bc, Br[v] + 1, Br[v + 1], cls, live, c);
+    if (kin && !kout && pout) kout = any_live<BITS>(Fc, Fr[v] + 1, Fr[v + 1], cls, live, c);
     return kin && kout;
 }
 
@@ -274,6 +294,11 @@ __global__ __launch_bounds__(256) void trim1_kernel(const uint32_t *__restrict__
         for (uint32_t base = (blockIdx.x * 4 + wv) * 64; base <= V; base += gridDim.x * 256) {
             if (nq + 64 > TRIM_WQ) wave_flush(wq, nq, cnt_out, qout, qcap, anom);
             const uint32_t v = base + lane();   // vertex 0, the dummy, is dead
+            // the first neighbours are loaded beside the live word or class
+            // (coalesced; a dead vertex's entries are not used, and past V
+            // the lane reads vertex 0's)
+            const uint32_t vv = v <= V ? v : 0u;
+            const uint32_t pin = B1[vv], pout = F1[vv];
             uint32_t word = 0, c = DEAD;
             if (v <= V) {
                 if (BITS) {
@@ -285,7 +310,7 @@ __global__ __launch_bounds__(256) void trim1_kernel(const uint32_t *__restrict__
             }
             bool gone = false;
             if (c != DEAD) {
-                if (!trim1_keep_at<BITS>(Fr, Fc, Br, Bc, cls, live, c, v, B1[v], F1[v])) {
+                if (!trim1_keep_at<BITS>(Fr, Fc, Br, Bc, cls, live, c, v, pin, pout)) {
                     if (!lazy) cls[v] = DEAD;
                     gone = true;
                 }
@@ -393,14 +418,26 @@ __global__ __launch_bounds__(256) void trim1_degree_kernel(const uint32_t *__res
             bb[h] = in ? Br[v[h]] : 0u;
             be[h] = in ? Br[v[h] + 1] : 0u;
         }
+        // every chunk's first-neighbour gathers are issued before any is
+        // stored (a gather and its store per branch would wait for each)
+        uint32_t f1[TRIM_DEG_CHUNKS], b1[TRIM_DEG_CHUNKS];
 #pragma unroll
         for (uint32_t h = 0; h < TRIM_DEG_CHUNKS; h++) {
             // 0: no out-arc or no in-arc (vertex 0, the dummy, has neither)
             const uint64_t p = (uint64_t)(fe[h] - fb[h]) * (uint64_t)(be[h] - bb[h]);
             dp[h] = p > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)p;
+            f1[h] = 0;
+            b1[h] = 0;
             if (v[h] >= 1 && dp[h]) {
-                F1[v[h]] = Fc[fb[h]];
-                B1[v[h]] = Bc[bb[h]];
+                f1[h] = Fc[fb[h]];
+                b1[h] = Bc[bb[h]];
+            }
+        }
+#pragma unroll
+        for (uint32_t h = 0; h < TRIM_DEG_CHUNKS; h++) {
+            if (v[h] >= 1 && dp[h]) {
+                F1[v[h]] = f1[h];
+                B1[v[h]] = b1[h];
                 DP[v[h]] = dp[h];
             }
         }

@@ -127,7 +127,7 @@ typedef struct scc_ctx scc_ctx;
 #define SCC_ALGO_DETECT 'd' /* detectSCC scc_WCC.cu:1000-1617 */
 
 /* Frontier expansion strategy of the reachability kernels. */
-#define SCC_EXPAND_AUTO   0 /* algorithm default: vertex (g,h), vwarp (x,y) */
+#define SCC_EXPAND_AUTO   0 /* direction-optimising (SCC_EXPAND_DO) for every algorithm */
 #define SCC_EXPAND_VERTEX 1 /* one lane per frontier vertex (fwd/bwd, scc_kernels.cu:360-445) */
 #define SCC_EXPAND_VWARP  2 /* W lanes per vertex (fwd_warp/bwd_warp, scc_kernels.h:46-105) */
 #define SCC_EXPAND_LB     3 /* workgroup scan-based gather + hub splitting */

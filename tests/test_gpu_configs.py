@@ -97,13 +97,30 @@ C2_RUNS = ([(a, 1, p, q) for a in "gh" for p, q in TRIMS] +
 assert len(C2_RUNS) == 64
 
 
+def ref_mapping(algo):
+    """The reference's own frontier mapping for an algorithm: a W-lane
+    virtual warp for -a x / -a y, a vertex per lane otherwise."""
+    return scc.EXPAND_VWARP if algo in "xy" else scc.EXPAND_VERTEX
+
+
 @pytest.mark.parametrize("algo,warp,p,q", C2_RUNS)
 def test_rmat22_matrix(ctx, rmat22, algo, warp, p, q):
     g, t = rmat22
     if ctx.graph is not g:
         ctx.upload(g)
-    st = ctx.run(algo=algo, warp=warp, trim1=p, trim2=q)
+    st = ctx.run(algo=algo, warp=warp, trim1=p, trim2=q, expand=ref_mapping(algo))
     t.check(ctx.labels(), st, f"R-MAT 22 -a {algo} -w {warp} -p {p} -q {q}", ctx=ctx)
+
+
+@pytest.mark.parametrize("algo", "ghxy")
+@pytest.mark.parametrize("p,q", TRIMS)
+def test_rmat22_default_expansion(ctx, rmat22, algo, p, q):
+    """What a drop-in caller gets: the automatic expansion (direction-optimised)."""
+    g, t = rmat22
+    if ctx.graph is not g:
+        ctx.upload(g)
+    st = ctx.run(algo=algo, warp=8, trim1=p, trim2=q, expand=scc.EXPAND_AUTO)
+    t.check(ctx.labels(), st, f"R-MAT 22 -a {algo} -p {p} -q {q} default expansion", ctx=ctx)
 
 
 def test_rmat22_direction_optimised_unsized(ctx, rmat22):
@@ -123,8 +140,9 @@ def test_rmat24(ctx):
     g = scc.gen_rmat(24, seed=24)
     t = Truth(g)
     ctx.upload(g)
-    for algo, w, expand, sizes in [("g", 1, scc.EXPAND_DO, False), ("g", 1, scc.EXPAND_AUTO, True),
-                                   ("h", 1, scc.EXPAND_AUTO, True), ("x", 32, scc.EXPAND_AUTO, True)]:
+    for algo, w, expand, sizes in [("g", 1, scc.EXPAND_DO, False), ("g", 1, scc.EXPAND_VERTEX, True),
+                                   ("h", 1, scc.EXPAND_VERTEX, True), ("x", 32, scc.EXPAND_VWARP, True),
+                                   ("h", 1, scc.EXPAND_AUTO, True), ("y", 8, scc.EXPAND_AUTO, False)]:
         st = ctx.run(algo=algo, warp=w, expand=expand, sizes=sizes)
         t.check(ctx.labels(), st, f"R-MAT 24 -a {algo} -w {w} expand {expand}", sized=sizes, ctx=ctx)
 
@@ -133,8 +151,8 @@ def test_grid4096(ctx):
     g = scc.gen_grid(4096, 0.7, seed=4096)
     t = Truth(g)
     ctx.upload(g)
-    for algo, w, expand in [("g", 1, scc.EXPAND_DO), ("g", 1, scc.EXPAND_AUTO), ("x", 4, scc.EXPAND_AUTO),
-                            ("h", 1, scc.EXPAND_AUTO)]:
+    for algo, w, expand in [("g", 1, scc.EXPAND_DO), ("g", 1, scc.EXPAND_VERTEX), ("x", 4, scc.EXPAND_VWARP),
+                            ("h", 1, scc.EXPAND_VERTEX), ("h", 1, scc.EXPAND_AUTO)]:
         st = ctx.run(algo=algo, warp=w, expand=expand)
         t.check(ctx.labels(), st, f"grid 4096 -a {algo} -w {w} expand {expand}", ctx=ctx)
 

@@ -34,9 +34,11 @@ def test_partition_fixtures(ctx, known, fixture_path, idx):
     assert_same_labels(loc, want_loc, f"local {pa['part']}")
     want, n = orc.tarjan(g)
     for t1, t2 in [(1, 1), (0, 0), (1, 0)]:
-        st = ctx.run(algo="d", trim1=t1, trim2=t2)
-        assert_same_labels(ctx.labels(), want, f"detect {pa['part']} -p {t1} -q {t2}")
-        assert st.num_sccs == n == pa["global_sccs"]
+        # the reference's vertex-per-lane mapping and the automatic expansion
+        for ex in (scc.EXPAND_VERTEX, scc.EXPAND_AUTO):
+            st = ctx.run(algo="d", trim1=t1, trim2=t2, expand=ex)
+            assert_same_labels(ctx.labels(), want, f"detect {pa['part']} -p {t1} -q {t2} expand {ex}")
+            assert st.num_sccs == n == pa["global_sccs"]
 
 
 @pytest.mark.parametrize("seed", range(8))
@@ -54,9 +56,10 @@ def test_random_partitions(ctx, seed):
     ctx.upload_partition(part)
     assert_same_labels(ctx.phase_local(), orc.local_scc(g, part)[0], f"local seed {seed}")
     want, n = orc.tarjan(g)
-    st = ctx.run(algo="d")
-    assert_same_labels(ctx.labels(), want, f"detect seed {seed} K={K}")
-    assert st.num_sccs == n
+    for ex in (scc.EXPAND_VERTEX, scc.EXPAND_AUTO):
+        st = ctx.run(algo="d", expand=ex)
+        assert_same_labels(ctx.labels(), want, f"detect seed {seed} K={K} expand {ex}")
+        assert st.num_sccs == n
 
 
 def test_planted_communities(ctx):

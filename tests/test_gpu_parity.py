@@ -31,6 +31,13 @@ def _solve(ctx, g, **kw):
     return ctx.labels(), st
 
 
+def ref_mapping(algo):
+    """The reference's own frontier mapping: a W-lane virtual warp for -a x
+    and -a y, a vertex per lane otherwise (the automatic expansion is the
+    direction-optimised search)."""
+    return scc.EXPAND_VWARP if algo in "xy" else scc.EXPAND_VERTEX
+
+
 def _check(lab, st, want, n, what):
     assert_same_labels(lab, want, what)
     assert st.num_sccs == n, what
@@ -54,11 +61,14 @@ def test_fixtures_all_algorithms(ctx, known, fixture_path, name):
     assert digest(want) == known["graphs"][name]["digest"]
     for algo, t1, t2 in itertools.product(ALGOS, [0, 1], [0, 1]):
         for W in ([1] if algo in "gh" else [1, 2, 4, 8, 16, 32, 64]):
-            lab, st = _solve(ctx, g, algo=algo, trim1=t1, trim2=t2, warp=W)
+            lab, st = _solve(ctx, g, algo=algo, trim1=t1, trim2=t2, warp=W, expand=ref_mapping(algo))
             _check(lab, st, want, n, f"{name} -a {algo} -p {t1} -q {t2} -w {W}")
             if algo in "gx":
-                lab, st = _solve(ctx, g, algo=algo, trim1=t1, trim2=t2, warp=W, sizes=False)
+                lab, st = _solve(ctx, g, algo=algo, trim1=t1, trim2=t2, warp=W, sizes=False,
+                                 expand=ref_mapping(algo))
                 _check_unsized(lab, st, want, n, f"{name} -a {algo} -p {t1} -q {t2} -w {W} unsized")
+        lab, st = _solve(ctx, g, algo=algo, trim1=t1, trim2=t2)
+        _check(lab, st, want, n, f"{name} -a {algo} -p {t1} -q {t2} default expansion")
 
 
 @pytest.mark.parametrize("seed", range(12))
@@ -69,7 +79,7 @@ def test_random_graphs(ctx, seed):
     g = random_graph(V, E, rng)
     want, n = orc.tarjan(g)
     for algo in ALGOS:
-        lab, st = _solve(ctx, g, algo=algo, warp=int(rng.choice(scc.WARP_SIZES)))
+        lab, st = _solve(ctx, g, algo=algo, warp=int(rng.choice(scc.WARP_SIZES)), expand=ref_mapping(algo))
         _check(lab, st, want, n, f"seed {seed} V={V} E={E} -a {algo}")
         lab, st = _solve(ctx, g, algo=algo, expand=scc.EXPAND_LB)
         _check(lab, st, want, n, f"seed {seed} V={V} E={E} -a {algo} LB")
