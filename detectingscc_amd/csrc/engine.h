@@ -116,7 +116,7 @@ class Engine {
     size_t nmarks = 0;
     int first_reach_marks[2] = {-1, -1};
     uint32_t num_parts = 0;
-    DevBuf buf[12];                    // detectSCC scratch (lazily sized)
+    DevBuf buf[16];                    // detectSCC scratch (lazily sized)
     std::unique_ptr<Engine> sub;       // engine for the quotient graph
 };
 

@@ -986,21 +986,33 @@ scc_status Engine::detect_phases(const scc_opts &o, int expand, int W, scc_stats
     const uint32_t QV = last[0] + last[1];
     const uint32_t *qlab = nullptr;
     if (QV) {
-        k_fill_u32(stream, cout, 0, (uint64_t)QV + 2);
-        k_fill_u32(stream, cin, 0, (uint64_t)QV + 2);
-        k_q_arcs(stream, 0, g, st, tbits, qid, cout, cin, nullptr, nullptr);
-        k_exscan(stream, cout, Qr, (uint64_t)QV + 2, tiles);
-        k_exscan(stream, cin, QBr, (uint64_t)QV + 2, tiles);
-        uint32_t QE = 0;
-        HIP_TRY(hipMemcpyAsync(&QE, Qr + QV + 1, 4, hipMemcpyDeviceToHost, stream));
+        // quotient arcs as sorted 64-bit keys: each vertex's quotient id,
+        // a count of the arcs between different local SCCs, the keys, and
+        // two radix sorts into the rows of both directions
+        uint32_t *qv;
+        unsigned long long *qn;
+        if ((s = scratch(buf[11], nv, &qv)) != SCC_OK || (s = scratch(buf[12], 1, &qn)) != SCC_OK) return s;
+        k_q_vid(stream, g, st, tbits, qid, qv);
+        k_fill_u32(stream, reinterpret_cast<uint32_t *>(qn), 0u, 2);
+        k_q_keys(stream, false, g, qv, qn, nullptr, 0);
+        unsigned long long qe = 0;
+        HIP_TRY(hipMemcpyAsync(&qe, qn, sizeof qe, hipMemcpyDeviceToHost, stream));
         HIP_TRY(hipStreamSynchronize(stream));
+        if (qe > 0xFFFFFFFFull) return fail(SCC_ERR_INVALID, "quotient graph has %llu arcs", qe);
+        const uint32_t QE = (uint32_t)qe;
         if (QE) {
             uint32_t *Qc, *QBc;
-            if ((s = scratch(buf[9], QE, &Qc)) != SCC_OK || (s = scratch(buf[10], QE, &QBc)) != SCC_OK)
+            uint64_t *keys, *sorted;
+            void *tmp;
+            const size_t tmp_bytes = csr_sort_bytes(QV, QE);
+            if ((s = scratch(buf[9], QE, &Qc)) != SCC_OK || (s = scratch(buf[10], QE, &QBc)) != SCC_OK ||
+                (s = scratch(buf[13], QE, &keys)) != SCC_OK || (s = scratch(buf[14], QE, &sorted)) != SCC_OK ||
+                (s = scratch(buf[15], tmp_bytes, reinterpret_cast<uint8_t **>(&tmp))) != SCC_OK)
                 return s;
-            HIP_TRY(hipMemcpyAsync(cout, Qr, sizeof(uint32_t) * ((size_t)QV + 2), hipMemcpyDeviceToDevice, stream));
-            HIP_TRY(hipMemcpyAsync(cin, QBr, sizeof(uint32_t) * ((size_t)QV + 2), hipMemcpyDeviceToDevice, stream));
-            k_q_arcs(stream, 1, g, st, tbits, qid, cout, cin, Qc, QBc);
+            k_fill_u32(stream, reinterpret_cast<uint32_t *>(qn), 0u, 2);
+            k_q_keys(stream, true, g, qv, qn, keys, QE);
+            if ((s = csr_from_keys(stream, QV, QE, keys, sorted, tmp, tmp_bytes, Qr, Qc, QBr, QBc)) != SCC_OK)
+                return s;
             k_q_rep(stream, g, isrep, qid, rep);
             if (!sub) {
                 sub.reset(new Engine());

@@ -16,6 +16,7 @@
 #include "engine.h"
 #include "generators.h"
 #include "host_util.h"
+#include "merge.h"
 #include "rng.h"
 
 namespace sccamd {
@@ -216,6 +217,56 @@ static scc_status sort_rows(hipStream_t st, const uint32_t *hi, const uint32_t *
         HIP_TRY(rocprim::radix_sort_keys(tmp, need, kin, kout, (unsigned int)E, 0u, 32u + bits, st));
     }
     rows_from_keys_kernel<<<gen_blocks(E + 1), GEN_BLOCK, 0, st>>>(kout, E, V, R, C);
+    return SCC_OK;
+}
+
+// From keys sorted ascending: R[u] = the number of keys whose high word is
+// below u (a binary search per u, so a graph with far more vertices than
+// arcs costs no long serial fill), and C[i] = key i's low word.
+__global__ void rows_by_search_kernel(const uint64_t *keys, uint64_t n, uint32_t V, uint32_t *R, uint32_t *C)
+{
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t u = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; u <= (uint64_t)V + 1; u += stride) {
+        uint64_t lo = 0, hi = n;   // first key with high word >= u
+        while (lo < hi) {
+            const uint64_t mid = (lo + hi) >> 1;
+            if ((keys[mid] >> 32) < u) lo = mid + 1;
+            else hi = mid;
+        }
+        R[u] = (uint32_t)lo;
+    }
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) C[i] = (uint32_t)keys[i];
+}
+
+__global__ void swap_keys_kernel(const uint64_t *in, uint64_t n, uint64_t *out)
+{
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        out[i] = (in[i] << 32) | (in[i] >> 32);
+}
+
+size_t csr_sort_bytes(uint32_t V, uint64_t n)
+{
+    size_t b = 0;
+    (void)rocprim::radix_sort_keys(nullptr, b, (uint64_t *)nullptr, (uint64_t *)nullptr,
+                                   (unsigned int)std::max<uint64_t>(n, 1), 0u, 32u + id_bits(V), (hipStream_t)0);
+    return std::max<size_t>(b, 1);
+}
+
+scc_status csr_from_keys(hipStream_t st, uint32_t V, uint64_t n, uint64_t *keys, uint64_t *out, void *tmp,
+                         size_t tmp_bytes, uint32_t *R, uint32_t *C, uint32_t *RB, uint32_t *CB)
+{
+    if (n > 0xFFFFFFFFull) return fail(SCC_ERR_INVALID, "too many arcs (%llu)", (unsigned long long)n);
+    const uint32_t bits = id_bits(V);
+    size_t need = tmp_bytes;
+    if (n) HIP_TRY(rocprim::radix_sort_keys(tmp, need, keys, out, (unsigned int)n, 0u, 32u + bits, st));
+    rows_by_search_kernel<<<gen_blocks(std::max<uint64_t>(n, (uint64_t)V + 2)), GEN_BLOCK, 0, st>>>(out, n, V, R, C);
+    if (n) {
+        // reverse keys from the sorted ones (the sort's input is not reused)
+        swap_keys_kernel<<<gen_blocks(n), GEN_BLOCK, 0, st>>>(out, n, keys);
+        need = tmp_bytes;
+        HIP_TRY(rocprim::radix_sort_keys(tmp, need, keys, out, (unsigned int)n, 0u, 32u + bits, st));
+    }
+    rows_by_search_kernel<<<gen_blocks(std::max<uint64_t>(n, (uint64_t)V + 2)), GEN_BLOCK, 0, st>>>(out, n, V, RB, CB);
     return SCC_OK;
 }
 

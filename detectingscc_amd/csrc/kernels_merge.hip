@@ -15,6 +15,8 @@
 // to the minimum vertex id of the global SCC: labels stay canonical.
 #include "merge.h"
 
+#include <algorithm>
+
 namespace sccamd {
 
 constexpr uint32_t SCAN_TILE = 1024;
@@ -173,6 +175,116 @@ __global__ void q_arcs_kernel(const uint32_t *__restrict__ Fr, const uint32_t *_
     }
 }
 
+// Quotient id of every vertex: 1 + qid of its local SCC, 0 for a vertex
+// that did not take part in the local phase (finished before it).
+__global__ void q_vid_kernel(const uint32_t *__restrict__ lab, const uint32_t *__restrict__ tbits,
+                             const uint32_t *__restrict__ qid, uint32_t *qv, uint32_t V)
+{
+    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v <= V; v += gridDim.x * blockDim.x)
+        qv[v] = (v >= 1 && !tbit(tbits, v)) ? qid[lab[v]] + 1 : 0u;
+}
+
+// Arcs between different local SCCs as 64-bit keys (source quotient id << 32
+// | target quotient id), which two radix sorts turn into the quotient CSR.
+// Counting into per-quotient-vertex rows took two atomics per arc on a few
+// hundred addresses (a community graph's local SCCs) and serialised (3.3 ms
+// per pass on 100M arcs).  Here a wave takes 64 consecutive vertices and
+// their arcs, which are contiguous in Fc: lane l takes arcs A0 + l, A0 + l +
+// 64, ... (QK_UNR of them in flight), finds the arc's source among the 64
+// row starts with a 6-step search across lanes, and compares quotient ids.
+// Counting (EMIT false) adds one atomic per workgroup; emitting appends the
+// keys through a wave queue in LDS, one reservation per flush.
+constexpr uint32_t QK_WQ = 512;   // keys per wave queue
+constexpr uint32_t QK_UNR = 4;    // arcs per lane in flight
+
+template <bool EMIT>
+__global__ __launch_bounds__(256) void q_keys_kernel(const uint32_t *__restrict__ Fr, const uint32_t *__restrict__ Fc,
+                                                     const uint32_t *__restrict__ qv, unsigned long long *n,
+                                                     uint64_t *keys, uint64_t cap, uint32_t V)
+{
+    __shared__ uint64_t s_q[4][EMIT ? QK_WQ : 1];
+    __shared__ unsigned long long s_sum[4];
+    const uint32_t lane = threadIdx.x & 63u, wv = threadIdx.x >> 6;
+    uint64_t *wq = s_q[wv];
+    uint32_t nq = 0;
+    unsigned long long cnt = 0;
+    const uint32_t nwaves = gridDim.x * 4;
+    for (uint32_t c = blockIdx.x * 4 + wv; (uint64_t)c * 64 <= V; c += nwaves) {
+        const uint32_t base = c * 64, u = base + lane;
+        const uint32_t rs = Fr[min(u, V + 1)];                      // row start (E past V)
+        const uint32_t A1 = Fr[min(base + 64, V + 1)];
+        const uint32_t qu_lane = u <= V ? qv[u] : 0u;
+        const uint32_t A0 = __shfl(rs, 0);
+        for (uint32_t j0 = A0; j0 < A1; j0 += 64 * QK_UNR) {
+            uint32_t w[QK_UNR], qu[QK_UNR];
+            bool ok[QK_UNR];
+#pragma unroll
+            for (uint32_t t = 0; t < QK_UNR; t++) {
+                const uint32_t j = j0 + t * 64 + lane;
+                ok[t] = j < A1;
+                w[t] = Fc[ok[t] ? j : A1 - 1];   // A1 > A0: a valid arc
+                // source: the last lane whose row start is <= j
+                uint32_t lo = 0, hi = 63;
+#pragma unroll
+                for (int it = 0; it < 6; it++) {
+                    const uint32_t mid = (lo + hi + 1) >> 1;
+                    if ((uint32_t)__shfl(rs, (int)mid) <= j) lo = mid;
+                    else hi = mid - 1;
+                }
+                qu[t] = __shfl(qu_lane, (int)lo);
+            }
+            uint32_t qw[QK_UNR];
+#pragma unroll
+            for (uint32_t t = 0; t < QK_UNR; t++) qw[t] = qv[w[t]];
+            if (EMIT && nq + QK_UNR * 64 > QK_WQ) {
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                unsigned long long b0 = 0;
+                if (lane == 0) b0 = atomicAdd(n, (unsigned long long)nq);
+                b0 = __shfl(b0, 0);
+                for (uint32_t i = lane; i < nq; i += 64)
+                    if (b0 + i < cap) keys[b0 + i] = wq[i];
+                nq = 0;
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            }
+#pragma unroll
+            for (uint32_t t = 0; t < QK_UNR; t++) {
+                const bool cross = ok[t] && qu[t] && qw[t] && qu[t] != qw[t];
+                if (EMIT) {
+                    const uint64_t m = __ballot(cross);
+                    if (cross) wq[nq + (uint32_t)__popcll(m & ((1ull << lane) - 1))] = ((uint64_t)qu[t] << 32) | qw[t];
+                    nq += (uint32_t)__popcll(m);
+                } else {
+                    cnt += cross;
+                }
+            }
+        }
+    }
+    if (EMIT) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        if (nq) {
+            unsigned long long b0 = 0;
+            if (lane == 0) b0 = atomicAdd(n, (unsigned long long)nq);
+            b0 = __shfl(b0, 0);
+            for (uint32_t i = lane; i < nq; i += 64)
+                if (b0 + i < cap) keys[b0 + i] = wq[i];
+        }
+    } else {
+        for (int o = 32; o > 0; o >>= 1) cnt += __shfl_xor(cnt, o);
+        if (lane == 0) s_sum[wv] = cnt;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            const unsigned long long t = s_sum[0] + s_sum[1] + s_sum[2] + s_sum[3];
+            if (t) atomicAdd(n, t);
+        }
+    }
+}
+
 __global__ void q_rep_kernel(const uint32_t *__restrict__ isrep, const uint32_t *__restrict__ qid,
                              uint32_t *rep, uint32_t V)
 {
@@ -218,6 +330,21 @@ void k_q_arcs(hipStream_t s, int pass, const DevGraph &g, const DevState &st, co
     else
         q_arcs_kernel<1><<<grid_for(g.V), 256, 0, s>>>(g.Fr, g.Fc, st.lab, tbits, qid, cnt_out, cnt_in,
                                                        Qc, QBc, g.V);
+}
+
+void k_q_vid(hipStream_t s, const DevGraph &g, const DevState &st, const uint32_t *tbits, const uint32_t *qid,
+             uint32_t *qv)
+{
+    q_vid_kernel<<<grid_for((uint64_t)g.V + 1), 256, 0, s>>>(st.lab, tbits, qid, qv, g.V);
+}
+
+void k_q_keys(hipStream_t s, bool emit, const DevGraph &g, const uint32_t *qv, unsigned long long *n,
+              uint64_t *keys, uint64_t cap)
+{
+    const uint64_t chunks = (uint64_t)g.V / 64 + 1;
+    const uint32_t blocks = (uint32_t)std::min<uint64_t>(2048, (chunks + 3) / 4);
+    if (emit) q_keys_kernel<true><<<blocks, 256, 0, s>>>(g.Fr, g.Fc, qv, n, keys, cap, g.V);
+    else q_keys_kernel<false><<<blocks, 256, 0, s>>>(g.Fr, g.Fc, qv, n, nullptr, 0, g.V);
 }
 
 void k_q_rep(hipStream_t s, const DevGraph &g, const uint32_t *isrep, const uint32_t *qid, uint32_t *rep)
