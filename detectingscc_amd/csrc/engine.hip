@@ -987,30 +987,43 @@ scc_status Engine::detect_phases(const scc_opts &o, int expand, int W, scc_stats
     const uint32_t *qlab = nullptr;
     if (QV) {
         // quotient arcs as sorted 64-bit keys: each vertex's quotient id,
-        // a count of the arcs between different local SCCs, the keys, and
-        // two radix sorts into the rows of both directions
+        // the keys of the arcs between different local SCCs, and two radix
+        // sorts into the rows of both directions.  The keys go straight into
+        // the (grow-only) buffer an earlier merge left; only a first merge,
+        // or one whose keys did not fit, counts them first.
         uint32_t *qv;
         unsigned long long *qn;
         if ((s = scratch(buf[11], nv, &qv)) != SCC_OK || (s = scratch(buf[12], 1, &qn)) != SCC_OK) return s;
         k_q_vid(stream, g, st, tbits, qid, qv);
-        k_fill_u32(stream, reinterpret_cast<uint32_t *>(qn), 0u, 2);
-        k_q_keys(stream, false, g, qv, qn, nullptr, 0);
+        auto read_count = [&](unsigned long long *out) -> scc_status {
+            HIP_TRY(hipMemcpyAsync(out, qn, sizeof *out, hipMemcpyDeviceToHost, stream));
+            HIP_TRY(hipStreamSynchronize(stream));
+            return SCC_OK;
+        };
         unsigned long long qe = 0;
-        HIP_TRY(hipMemcpyAsync(&qe, qn, sizeof qe, hipMemcpyDeviceToHost, stream));
-        HIP_TRY(hipStreamSynchronize(stream));
-        if (qe > 0xFFFFFFFFull) return fail(SCC_ERR_INVALID, "quotient graph has %llu arcs", qe);
+        uint64_t *keys = static_cast<uint64_t *>(buf[13].p);
+        uint64_t kcap = buf[13].cap / sizeof(uint64_t);
+        for (int attempt = 0;; attempt++) {
+            k_fill_u32(stream, reinterpret_cast<uint32_t *>(qn), 0u, 2);
+            if (kcap) k_q_keys(stream, true, g, qv, qn, keys, kcap);
+            else k_q_keys(stream, false, g, qv, qn, nullptr, 0);
+            if ((s = read_count(&qe)) != SCC_OK) return s;
+            if (qe > 0xFFFFFFFFull) return fail(SCC_ERR_INVALID, "quotient graph has %llu arcs", qe);
+            if (kcap && qe <= kcap) break;   // every key written
+            if (attempt == 2) return fail(SCC_ERR_DEVICE, "quotient arc count changed between passes");
+            if ((s = scratch(buf[13], std::max<uint64_t>(qe, 1), &keys)) != SCC_OK) return s;
+            kcap = buf[13].cap / sizeof(uint64_t);
+        }
         const uint32_t QE = (uint32_t)qe;
         if (QE) {
             uint32_t *Qc, *QBc;
-            uint64_t *keys, *sorted;
+            uint64_t *sorted;
             void *tmp;
             const size_t tmp_bytes = csr_sort_bytes(QV, QE);
             if ((s = scratch(buf[9], QE, &Qc)) != SCC_OK || (s = scratch(buf[10], QE, &QBc)) != SCC_OK ||
-                (s = scratch(buf[13], QE, &keys)) != SCC_OK || (s = scratch(buf[14], QE, &sorted)) != SCC_OK ||
+                (s = scratch(buf[14], QE, &sorted)) != SCC_OK ||
                 (s = scratch(buf[15], tmp_bytes, reinterpret_cast<uint8_t **>(&tmp))) != SCC_OK)
                 return s;
-            k_fill_u32(stream, reinterpret_cast<uint32_t *>(qn), 0u, 2);
-            k_q_keys(stream, true, g, qv, qn, keys, QE);
             if ((s = csr_from_keys(stream, QV, QE, keys, sorted, tmp, tmp_bytes, Qr, Qc, QBr, QBc)) != SCC_OK)
                 return s;
             k_q_rep(stream, g, isrep, qid, rep);
