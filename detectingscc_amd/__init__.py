@@ -280,8 +280,13 @@ class Stats:
     ms_phase: dict
     first_work: tuple = ()
 
-    # bytes of each first-search work unit (scc_stats.first_work)
-    WORK_BYTES = (12.0, 8.0, 4.0, 0.125, 4.0, 8.0, 8.0, 12.0)
+    # bytes of each first-search work unit (scc_stats.first_work): top-down
+    # vertex 12 (two row offsets + queue entry), arc 8 (column + visited
+    # word), claim 4 (queue entry); bottom-up swept vertex 4.125 (its visited
+    # bit and its first-parent word, read for every vertex of the sweep),
+    # candidate 0 (its first parent is in the sweep's read), first parent
+    # probed 8 (bitmap pair), row after a miss 8, further parent 12
+    WORK_BYTES = (12.0, 8.0, 4.0, 4.125, 0.0, 8.0, 8.0, 12.0)
 
     @property
     def first_work_bytes(self) -> float:

@@ -526,12 +526,18 @@ __device__ __forceinline__ void bu_level(const DirArgs &D, const uint32_t *__res
         }
         // the first parent comes from PF (coalesced; 0: no parent), not from
         // a random line of the parent list, and only a candidate whose first
-        // parent misses loads its row offsets to see whether more follow
+        // parent misses loads its row offsets to see whether more follow.
+        // PF is read for every lane beside its visited word (past V: vertex
+        // 0's entry) rather than after it for candidates only: one
+        // dependent step less per chunk for a coalesced word per vertex
+        // (R-MAT scale 24's search 0.382 -> 0.375 ms)
         uint32_t pf[BU_CHUNKS];
+#pragma unroll
+        for (uint32_t h = 0; h < BU_CHUNKS; h++) pf[h] = PF[u[h] <= V ? u[h] : 0u];
 #pragma unroll
         for (uint32_t h = 0; h < BU_CHUNKS; h++) {
             c[h] = (SINGLE || !cand[h]) ? 0u : cls[u[h]];
-            pf[h] = cand[h] ? PF[u[h]] : 0u;
+            if (!cand[h]) pf[h] = 0u;
         }
         // Every chunk's parent probe is issued before any result is used,
         // and then every chunk's row offsets, without per-lane branches: a

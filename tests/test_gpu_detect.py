@@ -62,6 +62,35 @@ def test_random_partitions(ctx, seed):
         assert st.num_sccs == n
 
 
+def test_merge_quotient_shapes(ctx):
+    """The merge's quotient graph at both extremes, in one context so its
+    grow-only key buffer is reused, grown and then left larger than needed:
+    far more quotient vertices than arcs (every vertex its own partition,
+    trim-1 off, a few thousand arcs over 100,000 vertices), then far more
+    arcs than quotient vertices (two partitions joined by many parallel
+    arcs), then the first graph again."""
+    rng = np.random.default_rng(77)
+    V = 100_000
+    src = rng.integers(1, V + 1, 3000)
+    dst = rng.integers(1, V + 1, 3000)
+    sparse = scc.from_arcs(V, np.r_[src, np.arange(1, 200)], np.r_[dst, np.arange(2, 201)])
+    part_sparse = np.zeros(V + 2, np.uint32)
+    part_sparse[1:V + 1] = np.arange(V, dtype=np.uint32)
+    n2 = 4000
+    a = rng.integers(1, n2 + 1, 60_000)
+    b = rng.integers(1, n2 + 1, 60_000)
+    dense = scc.from_arcs(n2, a, b)
+    part_dense = np.zeros(n2 + 2, np.uint32)
+    part_dense[1:n2 + 1] = (np.arange(1, n2 + 1) % 2).astype(np.uint32)
+    for g, part, t1 in [(sparse, part_sparse, 0), (dense, part_dense, 1), (sparse, part_sparse, 0)]:
+        ctx.upload(g)
+        ctx.upload_partition(part)
+        want, n = orc.tarjan(g)
+        st = ctx.run(algo="d", trim1=t1, trim2=0)
+        assert_same_labels(ctx.labels(), want, f"merge V={g.V} E={g.E} trim1={t1}")
+        assert st.num_sccs == n
+
+
 def test_planted_communities(ctx):
     g = scc.gen_communities(40000, 20, deg=8.0, inter=0.02, seed=55)
     ctx.upload(g)                       # uploads g.part too
