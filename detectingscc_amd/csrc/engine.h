@@ -19,8 +19,10 @@ class Engine {
   public:
     ~Engine();
     scc_status init(int device);
+    // `checked`: the caller has already validated the arrays (the server's
+    // submit does), so the host pass over them is skipped
     scc_status upload(uint32_t V, uint32_t E, const uint32_t *Fr, const uint32_t *Fc,
-                      const uint32_t *Br, const uint32_t *Bc);
+                      const uint32_t *Br, const uint32_t *Bc, bool checked = false);
     scc_status upload_partition(const uint32_t *part);
     scc_status upload_device(uint32_t V, uint32_t E, const uint32_t *dFr, const uint32_t *dFc,
                              const uint32_t *dBr, const uint32_t *dBc, hipStream_t from);

@@ -208,13 +208,13 @@ static scc_status validate_csr(uint32_t V, uint32_t E, const uint32_t *R, const 
 }
 
 scc_status Engine::upload(uint32_t V, uint32_t E, const uint32_t *Fr, const uint32_t *Fc,
-                          const uint32_t *Br, const uint32_t *Bc)
+                          const uint32_t *Br, const uint32_t *Bc, bool checked)
 {
     if (!Fr || !Br || (E && (!Fc || !Bc))) return fail(SCC_ERR_INVALID, "null graph array");
     if (V >= (1u << 30)) return fail(SCC_ERR_INVALID, "V=%u exceeds 2^30-1", V);
     scc_status s;
-    if ((s = validate_csr(V, E, Fr, Fc, "Fr/Fc")) != SCC_OK) return s;
-    if ((s = validate_csr(V, E, Br, Bc, "Br/Bc")) != SCC_OK) return s;
+    if (!checked && (s = validate_csr(V, E, Fr, Fc, "Fr/Fc")) != SCC_OK) return s;
+    if (!checked && (s = validate_csr(V, E, Br, Bc, "Br/Bc")) != SCC_OK) return s;
     if ((s = reserve(V, E)) != SCC_OK) return s;
     g.V = V;
     g.E = E;

@@ -54,7 +54,8 @@ class Server {
     scc_opts solo_opts{};
     uint32_t solo_vertices = 1u << 16;
     uint64_t budget = 0, inflight = 0, peak = 0;
-    bool solo_busy = false;            // a slot runs a large request
+    uint32_t solo_running = 0;         // slots running a large request
+    uint32_t solo_limit = 1;           // at most slots - 1 of them (at least 1)
     uint32_t max_batch = 64;
     std::mutex mu;
     std::condition_variable cv_work, cv_done;

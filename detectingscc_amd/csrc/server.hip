@@ -3,14 +3,14 @@
 // Requests wait in one queue in arrival order.  Each slot thread owns a
 // stream, a BatchEngine (grow-only device buffers) and, for large requests,
 // a device-wide Engine.  A free slot takes either the oldest large request
-// (V >= solo_vertices), if no other slot is running one, and runs it alone;
-// or the small requests in arrival order, skipping the large ones, up to
-// `max_batch` and the HBM budget left by what the other slots have in
-// flight, as one persistent batched launch.  Small requests thus never wait
-// behind large ones for a slot, while large requests keep their arrival
-// order among themselves, one at a time.  Requests that arrive while a
-// batch runs join the next one, so the device never waits for a batch to
-// fill.
+// (V >= solo_vertices), if fewer than slots - 1 slots (at least one) run
+// one, and runs it on its device-wide engine; or the small requests in
+// arrival order, skipping the large ones, up to `max_batch` and the HBM
+// budget left by what the other slots have in flight, as one persistent
+// batched launch.  Small requests thus never wait behind large ones for a
+// slot, and large requests are taken in arrival order (a burst of them
+// takes every slot but one).  Requests that arrive while a batch runs join
+// the next one, so the device never waits for a batch to fill.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -69,6 +69,8 @@ scc_status Server::start(uint64_t budget_bytes, uint32_t mb, uint32_t slots, uin
     busy_ms = 0;
     stopping = false;
     running = true;
+    solo_running = 0;
+    solo_limit = slots > 1 ? slots - 1 : 1;
     for (uint32_t s = 0; s < slots; s++) workers.emplace_back(&Server::worker, this, (int)s);
     return SCC_OK;
 }
@@ -138,10 +140,11 @@ bool Server::can_admit_locked() const
     bool large_seen = false, small_seen = false;
     for (uint64_t id : pending) {
         const Req *r = reqs.at(id).get();
-        if (r->solo ? (large_seen || solo_busy) : small_seen) continue;
+        const bool solo_full = solo_running >= solo_limit;
+        if (r->solo ? (large_seen || solo_full) : small_seen) continue;
         if (inflight + r->bytes <= budget) return true;
         (r->solo ? large_seen : small_seen) = true;
-        if (small_seen && (large_seen || solo_busy)) break;
+        if (small_seen && (large_seen || solo_full)) break;
     }
     return false;
 }
@@ -150,7 +153,7 @@ bool Server::can_admit_locked() const
 // that does not fit ends the batch, so small requests keep their order.
 void Server::take_locked(std::vector<Req *> &batch, uint64_t &taken)
 {
-    if (!solo_busy) {
+    if (solo_running < solo_limit) {
         for (auto it = pending.begin(); it != pending.end(); ++it) {
             Req *r = reqs.at(*it).get();
             if (!r->solo) continue;
@@ -158,7 +161,7 @@ void Server::take_locked(std::vector<Req *> &batch, uint64_t &taken)
                 batch.push_back(r);
                 taken = r->bytes;
                 pending.erase(it);
-                solo_busy = true;
+                solo_running++;
                 return;
             }
             break;
@@ -216,7 +219,8 @@ void Server::worker(int slot)
                 st = solo_eng->init(dev);
             }
             if (st == SCC_OK)
-                st = solo_eng->upload(r->V, r->E, r->Fr.data(), r->Fc.data(), r->Br.data(), r->Bc.data());
+                st = solo_eng->upload(r->V, r->E, r->Fr.data(), r->Fc.data(), r->Br.data(), r->Bc.data(),
+                                      true);   // checked when it was submitted
             scc_stats sst;
             if (st == SCC_OK) st = solo_eng->run(solo_opts, &sst);
             if (st == SCC_OK) st = solo_eng->labels(lab.data());
@@ -283,7 +287,7 @@ void Server::worker(int slot)
             batches++;
             if (G == 1 && batch[0]->solo) {
                 solo_runs++;
-                solo_busy = false;
+                solo_running--;
             }
             inflight -= taken;
         }
