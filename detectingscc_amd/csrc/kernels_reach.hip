@@ -726,7 +726,9 @@ __global__ __launch_bounds__(256) void expand_do_kernel(DoArgs a, uint32_t bpd)
 // cycle, one vertex per level, 2.6 us per level against 6.2 us device-wide;
 // the 4096^2 grid's frontiers of ~2,000 vertices, 27 us against 7.2 us), so
 // the host starts it on frontiers of at most NL_ENTER vertices and it hands
-// back beyond NL_SPILL.
+// back beyond NL_SPILL.  The host uses it for multi-class searches only: a
+// single-class search's top-down levels continue narrow frontiers inside
+// the device-wide launch (lb_level's local continuation), which is faster.
 constexpr uint32_t NL_THREADS = 1024;
 constexpr uint32_t NL_CAP = 6144;                 // frontier entries held in LDS
 constexpr uint32_t NL_ENTER = 256;                // host: start on frontiers this small
