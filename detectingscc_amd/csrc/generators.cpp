@@ -96,7 +96,7 @@ void rmat_arcs(uint32_t scale, uint32_t vcap, uint64_t E, double a, double b, do
     }, threads);
 }
 
-// Graph `index` of the batched mix: V log-uniform in
+// Graph `index` of the batched mix (the many-graph workload): V log-uniform in
 // [2^lgmin, 2^lgmax], mean out-degree uniform in [dmin, dmax]; 40% ER,
 // 40% R-MAT (a=.57 b=.19 c=.19, ids above V redrawn, seeded permutation),
 // 20% planted directed cycles of length 2..512 carrying about half of the
