@@ -497,16 +497,26 @@ void k_trim1(hipStream_t s, const DevGraph &g, DevState &st, const uint32_t *in,
 // ---------------------------------------------------------- trim-2 ---
 // scc_kernels.cu:236-357.  Live same-class degree capped at 2; *k gets
 // the last live neighbour seen.
+// Two arcs per step (both columns, then both class words): the answer
+// usually rests on the first two arcs, so a step of two halves the chain
+// of dependent loads without reading arcs that are not needed; an index
+// past the end repeats the last arc and is not counted.
 __device__ __forceinline__ int live_deg2(const uint32_t *__restrict__ R,
                                          const uint32_t *__restrict__ C, const uint32_t *cls,
                                          uint32_t v, uint32_t c, uint32_t *k)
 {
     int d = 0;
-    for (uint32_t i = R[v], e = R[v + 1]; i < e; i++) {
-        uint32_t w = C[i];
-        if (cls[w] != c) continue;
-        if (++d == 2) return 2;
-        *k = w;
+    for (uint32_t i = R[v], e = R[v + 1]; i < e; i += 2) {
+        const uint32_t w0 = C[i], w1 = C[min(i + 1, e - 1)];
+        const uint32_t x0 = cls[w0], x1 = cls[w1];
+        if (x0 == c) {
+            if (++d == 2) return 2;
+            *k = w0;
+        }
+        if (i + 1 < e && x1 == c) {
+            if (++d == 2) return 2;
+            *k = w1;
+        }
     }
     return d;
 }
