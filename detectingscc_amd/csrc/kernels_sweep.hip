@@ -607,16 +607,28 @@ __global__ __launch_bounds__(256) void init_vis_kernel(const uint32_t *__restric
     uint64_t best = 0;
     // VIS_CHUNKS chunks per step keep that many loads in flight per wave
     for (uint32_t ch0 = wave; ch0 < nchunks; ch0 += VIS_CHUNKS * nwaves) {
-        uint32_t c[VIS_CHUNKS], dp[VIS_CHUNKS];
+        // every chunk's loads are issued before any is used, without
+        // per-lane branches (a load and its use in one branch waited for
+        // each in turn): a lane past the end reads vertex 0's entries
+        uint32_t c[VIS_CHUNKS], dp[VIS_CHUNKS], x[VIS_CHUNKS];
+        bool in[VIS_CHUNKS];
 #pragma unroll
         for (uint32_t h = 0; h < VIS_CHUNKS; h++) {
             const uint32_t ch = ch0 + h * nwaves, v = ch * 64 + lane();
-            const bool in = ch < nchunks && v <= V;
-            if (live_bits)   // one class: the live bitmap (2 words a chunk instead of 64)
-                c[h] = in && ((live_bits[v >> 5] >> (v & 31)) & 1u) ? 0u : DEAD;
+            in[h] = ch < nchunks && v <= V;
+            const uint32_t vv = in[h] ? v : 0u;
+            // one class: the live bitmap (2 words a chunk instead of 64)
+            x[h] = live_bits ? live_bits[vv >> 5] : cls[vv];
+            if (ELECT) dp[h] = DP[vv];
+        }
+#pragma unroll
+        for (uint32_t h = 0; h < VIS_CHUNKS; h++) {
+            const uint32_t v = (ch0 + h * nwaves) * 64 + lane();
+            if (live_bits)
+                c[h] = in[h] && ((x[h] >> (v & 31)) & 1u) ? 0u : DEAD;
             else
-                c[h] = in ? cls[v] : DEAD;
-            if (ELECT) dp[h] = in ? DP[v] : 0u;
+                c[h] = in[h] ? x[h] : DEAD;
+            if (ELECT && !in[h]) dp[h] = 0u;
         }
 #pragma unroll
         for (uint32_t h = 0; h < VIS_CHUNKS; h++) {
@@ -938,8 +950,10 @@ __global__ __launch_bounds__(256) void update_words_kernel(uint32_t *cls, uint64
     uint32_t live = 0, found = 0, mmin = DEAD;
     const uint32_t ngroups = (V >> 5) + 1;
     for (uint32_t gi = blockIdx.x * blockDim.x + threadIdx.x; gi < ngroups; gi += gridDim.x * blockDim.x) {
+        // the group's three words in flight together (the visited pairs are
+        // read even for a group with no live vertex, which then adds and
+        // writes nothing, rather than after its live word)
         const uint32_t lw = live_bits[gi];
-        if (!lw) continue;
         const uint64_t f = reinterpret_cast<const uint64_t *>(fvis)[gi];
         const uint64_t b = reinterpret_cast<const uint64_t *>(bvis)[gi];
         const uint32_t fv = (uint32_t)f & lw, bv = (uint32_t)b & lw;   // live vertices are not finished
@@ -979,7 +993,7 @@ void k_update_words(hipStream_t s, const DevGraph &g, DevState &st, const uint32
 {
     const uint64_t ngroups = ((uint64_t)g.V >> 5) + 1;
     uint64_t b = (ngroups + SWEEP_BLOCK - 1) / SWEEP_BLOCK;
-    b = b < 1 ? 1 : (b > 1024 ? 1024 : b);
+    b = b < 1 ? 1 : (b > 4096 ? 4096 : b);   // a group per lane up to 2^25 vertices
     update_words_kernel<<<(uint32_t)b, SWEEP_BLOCK, 0, s>>>(st.cls, st.piv, st.fvis, st.bvis, live_bits, st.ctr,
                                                             g.V, st.aux);
 }
@@ -1082,6 +1096,28 @@ void k_minlabel_init(hipStream_t s, const DevGraph &g, DevState &st)
     minlabel_init_kernel<<<sweep_blocks(g.V), SWEEP_BLOCK, 0, s>>>(st.cls, g.V);
 }
 
+// m folded (minimum, or maximum with MAX) with val[w] over the arcs
+// [b, e) of C: K arcs per step, all K columns and then all K values in
+// flight before any is used (a step past e repeats the last arc, which
+// leaves a minimum or maximum unchanged).
+template <bool MAX, uint32_t K>
+__device__ __forceinline__ uint32_t fold_arcs(const uint32_t *__restrict__ C, const uint32_t *val, uint32_t b,
+                                              uint32_t e, uint32_t m)
+{
+    for (uint32_t i = b; i < e; i += K) {
+        uint32_t w[K], x[K];
+#pragma unroll
+        for (uint32_t k = 0; k < K; k++) w[k] = C[min(i + k, e - 1)];
+#pragma unroll
+        for (uint32_t k = 0; k < K; k++) x[k] = val[w[k]];
+#pragma unroll
+        for (uint32_t k = 0; k < K; k++) m = MAX ? (x[k] > m ? x[k] : m) : (x[k] < m ? x[k] : m);
+    }
+    return m;
+}
+constexpr uint32_t MINLABEL_K = 4;   // arcs per step of the min-label propagation
+constexpr uint32_t COLOR_K = 4;      // and of the colour propagation
+
 __global__ void minlabel_prop_kernel(const uint32_t *__restrict__ Fr,
                                      const uint32_t *__restrict__ Fc, uint32_t *cls, uint32_t *ctr,
                                      uint32_t V)
@@ -1089,14 +1125,14 @@ __global__ void minlabel_prop_kernel(const uint32_t *__restrict__ Fr,
     uint32_t changed = 0;
     WAVE_LOOP(v, valid, V)
     {
-        if (!valid) continue;
-        const uint32_t m0 = cls[v];
-        if (m0 == DEAD) continue;
-        uint32_t m = m0;
-        for (uint32_t i = Fr[v], e = Fr[v + 1]; i < e; i++) {
-            uint32_t x = cls[Fc[i]];
-            m = x < m ? x : m;
-        }
+        // label and row offsets together (past V, vertex 0's: the dummy,
+        // dead); an inactive lane folds an empty range, so the offsets are
+        // used on every path and are not loaded after the label test
+        const uint32_t vv = valid ? v : 0u;
+        const uint32_t m0 = cls[vv], b = Fr[vv], e = Fr[vv + 1];
+        const bool act = valid && m0 != DEAD;
+        uint32_t m = fold_arcs<false, MINLABEL_K>(Fc, cls, b, act ? e : b, m0);
+        if (!act) continue;
         for (int it = 0; it < 32; it++) {
             uint32_t x = cls[m];
             if (x >= m) break;
@@ -1151,14 +1187,13 @@ __global__ void color_prop_kernel(const uint32_t *__restrict__ Fr, const uint32_
     uint32_t changed = 0;
     WAVE_LOOP(v, valid, V)
     {
-        if (!valid) continue;
-        const uint32_t m0 = aux[v];
-        if (!m0) continue;
-        uint32_t m = m0;
-        for (uint32_t i = Fr[v], e = Fr[v + 1]; i < e; i++) {
-            uint32_t x = aux[Fc[i]];
-            m = x > m ? x : m;
-        }
+        // colour and row offsets together (past V, vertex 0's: colour 0);
+        // an inactive lane folds an empty range, as in minlabel_prop_kernel
+        const uint32_t vv = valid ? v : 0u;
+        const uint32_t m0 = aux[vv], b = Fr[vv], e = Fr[vv + 1];
+        const bool act = valid && m0 != 0;
+        uint32_t m = fold_arcs<true, COLOR_K>(Fc, aux, b, act ? e : b, m0);
+        if (!act) continue;
         for (int it = 0; it < 32; it++) {
             uint32_t x = aux[m];
             if (x <= m) break;
