@@ -180,8 +180,15 @@ __global__ void q_arcs_kernel(const uint32_t *__restrict__ Fr, const uint32_t *_
 __global__ void q_vid_kernel(const uint32_t *__restrict__ lab, const uint32_t *__restrict__ tbits,
                              const uint32_t *__restrict__ qid, uint32_t *qv, uint32_t V)
 {
-    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v <= V; v += gridDim.x * blockDim.x)
-        qv[v] = (v >= 1 && !tbit(tbits, v)) ? qid[lab[v]] + 1 : 0u;
+    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v <= V; v += gridDim.x * blockDim.x) {
+        // the finished bit and the label go out together; a finished
+        // vertex's label (LAB_FINAL set) is not an index, vertex 0's stands in
+        const bool fin = tbit(tbits, v);
+        const uint32_t l = lab[v];
+        const bool in = v >= 1 && !fin;
+        const uint32_t q = qid[in ? l : 0u];
+        qv[v] = in ? q + 1 : 0u;
+    }
 }
 
 // Arcs between different local SCCs as 64-bit keys (source quotient id << 32
