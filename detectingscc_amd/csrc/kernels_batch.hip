@@ -152,11 +152,15 @@ template <int BS> __device__ uint32_t g_trim1(Lds &L, const G1 &g, bool fixpoint
     for (uint32_t pass = 0; pass <= g.V; pass++) {   // each pass removes a vertex or ends
         uint32_t removed = 0;
         for (uint32_t v = threadIdx.x + 1; v <= g.V; v += BS) {
+            // class and both rows' offsets together; a dead vertex scans
+            // empty ranges, so the offsets are used on every path and are
+            // not loaded after the class test or the first scan
             const uint32_t c = g.cls[v];
-            if (c == DEAD) continue;
-            bool keep = g_any_cls(g.Bc, g.Br[v], g.Br[v + 1], g.cls, c);
-            if (keep) keep = g_any_cls(g.Fc, g.Fr[v], g.Fr[v + 1], g.cls, c);
-            if (!keep) {
+            const uint32_t bb = g.Br[v], be = g.Br[v + 1], fb = g.Fr[v], fe = g.Fr[v + 1];
+            const bool act = c != DEAD;
+            const bool kin = g_any_cls(g.Bc, bb, act ? be : bb, g.cls, c);
+            const bool keep = kin && g_any_cls(g.Fc, fb, fe, g.cls, c);
+            if (act && !keep) {
                 g.lab[v] = v | LAB_FINAL;
                 __hip_atomic_store(&g.cls[v], DEAD, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
                 removed++;
@@ -404,9 +408,12 @@ template <int BS> __device__ void g_minlabel(Lds &L, const G1 &g)
     for (uint32_t it = 0; it <= g.V; it++) {   // labels only decrease
         uint32_t changed = 0;
         for (uint32_t v = threadIdx.x + 1; v <= g.V; v += BS) {
-            const uint32_t m0 = g.cls[v];
-            if (m0 == DEAD) continue;
-            uint32_t m = g_fold<false>(g.Fc, g.Fr[v], g.Fr[v + 1], g.cls, m0);
+            // label and row offsets together (an inactive lane folds an
+            // empty range, so the offsets are used on every path)
+            const uint32_t m0 = g.cls[v], b = g.Fr[v], e = g.Fr[v + 1];
+            const bool act = m0 != DEAD;
+            uint32_t m = g_fold<false>(g.Fc, b, act ? e : b, g.cls, m0);
+            if (!act) continue;
             for (int it = 0; it < 32; it++) {
                 const uint32_t x = g.cls[m];
                 if (x >= m) break;
@@ -437,9 +444,11 @@ template <int BS> __device__ void g_color_loop(Lds &L, const G1 &g, uint32_t W, 
         for (uint32_t it = 0; it <= g.V; it++) {   // colours only increase
             uint32_t changed = 0;
             for (uint32_t v = threadIdx.x + 1; v <= g.V; v += BS) {
-                const uint32_t m0 = g.aux[v];
-                if (!m0) continue;
-                uint32_t m = g_fold<true>(g.Fc, g.Fr[v], g.Fr[v + 1], g.aux, m0);
+                // as in g_minlabel
+                const uint32_t m0 = g.aux[v], b = g.Fr[v], e = g.Fr[v + 1];
+                const bool act = m0 != 0;
+                uint32_t m = g_fold<true>(g.Fc, b, act ? e : b, g.aux, m0);
+                if (!act) continue;
                 for (int it = 0; it < 32; it++) {
                     const uint32_t x = g.aux[m];
                     if (x <= m) break;
