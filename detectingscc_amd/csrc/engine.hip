@@ -955,8 +955,10 @@ scc_status Engine::detect_phases(const scc_opts &o, int expand, int W, scc_stats
     scc_status s;
     if (o.trim1 && (s = trim1(true, true, true)) != SCC_OK) return s;
     if (o.trim2) {
+        // after the single-class trim-1 every live vertex is in class 0 and
+        // its live bitmap is exact (nothing reads the bitmap after trim-2)
         mark(SCC_PHASE_TRIM2);
-        k_trim2(stream, g, st);
+        k_trim2(stream, g, st, o.trim1 ? live_bits() : nullptr);
     }
     if (o.trim1 && (s = trim1(true)) != SCC_OK) return s;
     mark(SCC_PHASE_LOCAL);

@@ -497,23 +497,26 @@ void k_trim1(hipStream_t s, const DevGraph &g, DevState &st, const uint32_t *in,
 // ---------------------------------------------------------- trim-2 ---
 // scc_kernels.cu:236-357.  Live same-class degree capped at 2; *k gets
 // the last live neighbour seen.
-// Two arcs per step (both columns, then both class words): the answer
+// Two arcs per step (both columns, then both class tests): the answer
 // usually rests on the first two arcs, so a step of two halves the chain
 // of dependent loads without reading arcs that are not needed; an index
-// past the end repeats the last arc and is not counted.
-__device__ __forceinline__ int live_deg2(const uint32_t *__restrict__ R,
-                                         const uint32_t *__restrict__ C, const uint32_t *cls,
-                                         uint32_t v, uint32_t c, uint32_t *k)
+// past the end repeats the last arc and is not counted.  With BITS (every
+// live vertex in one class) the test is the live bitmap, which stays in L2,
+// instead of a random line of the class words.
+template <bool BITS>
+__device__ __forceinline__ int live_deg2(const uint32_t *__restrict__ R, const uint32_t *__restrict__ C,
+                                         const uint32_t *cls, const uint32_t *live, uint32_t v, uint32_t c,
+                                         uint32_t *k)
 {
     int d = 0;
     for (uint32_t i = R[v], e = R[v + 1]; i < e; i += 2) {
         const uint32_t w0 = C[i], w1 = C[min(i + 1, e - 1)];
-        const uint32_t x0 = cls[w0], x1 = cls[w1];
-        if (x0 == c) {
+        const bool s0 = same_live<BITS>(cls, live, w0, c), s1 = same_live<BITS>(cls, live, w1, c);
+        if (s0) {
             if (++d == 2) return 2;
             *k = w0;
         }
-        if (i + 1 < e && x1 == c) {
+        if (i + 1 < e && s1) {
             if (++d == 2) return 2;
             *k = w1;
         }
@@ -531,24 +534,32 @@ __device__ __forceinline__ bool has_arc(const uint32_t *__restrict__ R,
 
 // A pair {v,k} whose only live same-class in-arcs (or out-arcs) are the
 // ones between them is an SCC.  The claim is a CAS on the smaller id, so the
-// pair is recorded once and no flag update is lost (defect D5).
+// pair is recorded once and no flag update is lost (defect D5).  With BITS
+// every live vertex is in class 0 and the live bitmap `live` is exact for
+// the class words as the pass starts (a single-class trim-1 just ran); the
+// pass removes pairs from the class words only, so a pair removed meanwhile
+// still reads as live, which can only hide a pair, never make one.
+template <bool BITS>
 __global__ void trim2_kernel(const uint32_t *__restrict__ Fr, const uint32_t *__restrict__ Fc,
                              const uint32_t *__restrict__ Br, const uint32_t *__restrict__ Bc,
-                             uint32_t *cls, uint32_t *lab, uint32_t V, uint32_t *ctr)
+                             uint32_t *cls, const uint32_t *__restrict__ live, uint32_t *lab, uint32_t V,
+                             uint32_t *ctr)
 {
     uint32_t pairs = 0;
     WAVE_LOOP(v, valid, V)
     {
         if (!valid) continue;
-        const uint32_t c = cls[v];
-        if (c == DEAD) continue;
+        const uint32_t c = BITS ? 0u : cls[v];
+        if (BITS ? !same_live<true>(cls, live, v, 0u) : c == DEAD) continue;
+        // a single live in-neighbour k that v also points to and whose only
+        // live in-neighbour is v, or the same with out-neighbours
         uint32_t k = 0, kk = 0;
-        bool hit = live_deg2(Br, Bc, cls, v, c, &k) == 1 && has_arc(Fr, Fc, v, k) &&
-                   live_deg2(Br, Bc, cls, k, c, &kk) == 1;
+        bool hit = live_deg2<BITS>(Br, Bc, cls, live, v, c, &k) == 1 && has_arc(Fr, Fc, v, k) &&
+                   live_deg2<BITS>(Br, Bc, cls, live, k, c, &kk) == 1;
         if (!hit) {
             k = 0;
-            hit = live_deg2(Fr, Fc, cls, v, c, &k) == 1 && has_arc(Br, Bc, v, k) &&
-                  live_deg2(Fr, Fc, cls, k, c, &kk) == 1;
+            hit = live_deg2<BITS>(Fr, Fc, cls, live, v, c, &k) == 1 && has_arc(Br, Bc, v, k) &&
+                  live_deg2<BITS>(Fr, Fc, cls, live, k, c, &kk) == 1;
         }
         if (hit) {
             uint32_t lo = v < k ? v : k, hi = v < k ? k : v;
@@ -563,10 +574,14 @@ __global__ void trim2_kernel(const uint32_t *__restrict__ Fr, const uint32_t *__
     spread_add32(ctr, C_TRIM2, pairs);
 }
 
-void k_trim2(hipStream_t s, const DevGraph &g, DevState &st)
+void k_trim2(hipStream_t s, const DevGraph &g, DevState &st, const uint32_t *live_bits)
 {
-    trim2_kernel<<<sweep_blocks(g.V), SWEEP_BLOCK, 0, s>>>(g.Fr, g.Fc, g.Br, g.Bc, st.cls, st.lab,
-                                                           g.V, st.ctr);
+    if (live_bits)
+        trim2_kernel<true><<<sweep_blocks(g.V), SWEEP_BLOCK, 0, s>>>(g.Fr, g.Fc, g.Br, g.Bc, st.cls, live_bits,
+                                                                     st.lab, g.V, st.ctr);
+    else
+        trim2_kernel<false><<<sweep_blocks(g.V), SWEEP_BLOCK, 0, s>>>(g.Fr, g.Fc, g.Br, g.Bc, st.cls, nullptr,
+                                                                      st.lab, g.V, st.ctr);
 }
 
 // A search starts with no hub pending and the slots of its queue counts

@@ -360,7 +360,9 @@ void k_trim1(hipStream_t s, const DevGraph &g, DevState &st, const uint32_t *in,
 uint32_t slot_cap(uint32_t V, uint32_t grid);
 uint32_t trim_slot_cap(uint32_t V);
 size_t trim_list_words(uint32_t V);
-void k_trim2(hipStream_t s, const DevGraph &g, DevState &st);
+// live_bits: every live vertex is in class 0 and the bitmap (as the
+// single-class trim-1 leaves it) is exact; tested instead of the class words
+void k_trim2(hipStream_t s, const DevGraph &g, DevState &st, const uint32_t *live_bits = nullptr);
 // Visited bitmaps of a search and C_NLIVE; with elect_first (every live
 // vertex in class 0) also the first pivot, seeded in both directions.
 // live_bits: every live vertex is in class 0 and the bitmap (1 bit per

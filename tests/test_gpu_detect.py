@@ -33,7 +33,9 @@ def test_partition_fixtures(ctx, known, fixture_path, idx):
     assert n_loc == pa["local_sccs"]
     assert_same_labels(loc, want_loc, f"local {pa['part']}")
     want, n = orc.tarjan(g)
-    for t1, t2 in [(1, 1), (0, 0), (1, 0)]:
+    # trim-2 tests the live bitmap after the single-class trim-1 (-p 1 -q 1)
+    # and the class words without it (-p 0 -q 1)
+    for t1, t2 in [(1, 1), (0, 0), (1, 0), (0, 1)]:
         # the reference's vertex-per-lane mapping and the automatic expansion
         for ex in (scc.EXPAND_VERTEX, scc.EXPAND_AUTO):
             st = ctx.run(algo="d", trim1=t1, trim2=t2, expand=ex)
