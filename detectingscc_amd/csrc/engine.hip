@@ -960,7 +960,10 @@ scc_status Engine::detect_phases(const scc_opts &o, int expand, int W, scc_stats
         mark(SCC_PHASE_TRIM2);
         k_trim2(stream, g, st, o.trim1 ? live_bits() : nullptr);
     }
-    if (o.trim1 && (s = trim1(true)) != SCC_OK) return s;
+    // every live vertex is still in class 0 (trim-2 only removes), so the
+    // second fixpoint runs single-class too: its first pass rebuilds the
+    // live bitmap from the class words, and it tests neighbours in the bitmap
+    if (o.trim1 && (s = trim1(true, true)) != SCC_OK) return s;
     mark(SCC_PHASE_LOCAL);
     const size_t nv = (size_t)g.V + 1;
     uint32_t *tbits, *isrep, *qid, *rep, *cout, *cin, *Qr, *QBr, *tiles;
