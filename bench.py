@@ -52,6 +52,7 @@ WORKLOADS = {
     # measured alone (one JSON line each, value = graphs/s)
     "batch4096": "the 4096-graph mix, batched persistent form vs one graph per launch",
     "serve": "Poisson request stream into the continuous-batching server",
+    "part200": "-a d on 8,388,608 V with 200 planted communities, partitioned by community",
 }
 
 
@@ -382,6 +383,16 @@ def main():
             "warmup": 1, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "int32",
             "data": "synthetic", "config": {"workload": part.pop("workload"), "parallelism": "single GPU"},
             "detail": part}), flush=True)
+        return
+
+    if args.workload == "part200":
+        part = bench_partitioned(steps=args.steps)
+        print(json.dumps({
+            "metric": "SCC edges/s of -a d (local SCCs per partition, then the merge)",
+            "value": part["edges_per_s"], "unit": "edges/s", "n_gpus": 1, "steps": args.steps, "warmup": 1,
+            "ms_per_step": part["ms"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+            "dtype": "int32", "data": "synthetic",
+            "config": {"workload": part.pop("workload"), "parallelism": "single GPU"}, "detail": part}), flush=True)
         return
 
     ctx = scc.Context(0)
