@@ -757,19 +757,26 @@ void k_init_vis(hipStream_t s, const DevGraph &g, DevState &st, bool elect_first
 // pollForPivots / pollForFirstPivot (scc_kernels.cu:57-74, 643-656) with a
 // 64-bit key so the winner is the true argmax and every class elects one
 // vertex (defects D4, D10).  A wave whose live lanes share a class does one
-// atomic.
-__global__ void elect_kernel(const uint32_t *__restrict__ Fr, const uint32_t *__restrict__ Br,
-                             const uint32_t *__restrict__ cls, unsigned long long *piv, uint32_t V)
+// atomic.  The key's degree product is DP[v] (built for every live vertex
+// at the start of the decomposition), read with the class word rather than
+// four row offsets after it.  Each chunk reads its classes' current maxima
+// just before its atomics, so most lanes see a maximum an earlier chunk has
+// raised and issue none (deciding four chunks against one read of the
+// maxima, with their atomics issued together, doubled the time on a graph
+// with about 600 classes: more lanes act on maxima that are out of date).
+__global__ void elect_kernel(const uint32_t *__restrict__ DP, const uint32_t *__restrict__ cls,
+                             unsigned long long *piv, uint32_t V)
 {
     WAVE_LOOP(v, valid, V)
     {
-        const uint32_t c = valid ? cls[v] : DEAD;
+        const uint32_t vv = valid ? v : 0u;   // vertex 0: the dummy, dead
+        const uint32_t c = cls[vv], dp = DP[vv];
         const bool ok = c != DEAD;
         const uint64_t live = __ballot(ok);
         if (!live) continue;
         const int L = __ffsll((unsigned long long)live) - 1;
         const uint32_t c0 = __shfl(c, L);
-        const unsigned long long key = ok ? pivot_key(Fr, Br, v) : 0ull;
+        const unsigned long long key = ok ? (((uint64_t)dp << 32) | (uint64_t)(0xFFFFFFFFu - v)) : 0ull;
         if (__all(!ok || c == c0)) {
             unsigned long long m = key;
             for (int o = 32; o > 0; o >>= 1) {
@@ -787,7 +794,7 @@ __global__ void elect_kernel(const uint32_t *__restrict__ Fr, const uint32_t *__
 // its maximum key first and issues one atomicMax.
 void k_elect(hipStream_t s, const DevGraph &g, DevState &st)
 {
-    elect_kernel<<<sweep_blocks(g.V), SWEEP_BLOCK, 0, s>>>(g.Fr, g.Br, st.cls,
+    elect_kernel<<<sweep_blocks(g.V), SWEEP_BLOCK, 0, s>>>(g.DP, st.cls,
                                                            reinterpret_cast<unsigned long long *>(st.piv), g.V);
 }
 
