@@ -469,6 +469,7 @@ constexpr uint32_t BU_SHORT = 1;   // the first parent, read from the first-pare
 constexpr uint32_t BU_GROUP = 4;
 constexpr uint32_t BU_WQ = 1024;
 constexpr uint32_t BU_CHUNKS = 2;   // 64-vertex chunks per wave iteration
+constexpr uint32_t BU_SPARSE = 16;  // unvisited live vertices under V / BU_SPARSE: candidates alone read PF
 
 // One 64-bit load gives p's visited-or-finished and finished bits.
 template <bool SINGLE>
@@ -494,7 +495,7 @@ template <bool SINGLE, bool COUNT = false>
 __device__ __forceinline__ void bu_level(const DirArgs &D, const uint32_t *__restrict__ PR,
                                          const uint32_t *__restrict__ PC, const uint32_t *__restrict__ PF,
                                          const uint32_t *cls, uint32_t V, int L, uint32_t bid, uint32_t bpd,
-                                         uint32_t *lds, uint32_t qcap, uint32_t *anom, WorkCount &wc)
+                                         uint32_t *lds, uint32_t qcap, uint32_t *anom, bool sparse, WorkCount &wc)
 {
     const uint32_t slot = bid % kSpread;
     uint32_t *cnt_out = &D.cslot[CT(((L + 1) % 3) * kSpread + slot)];
@@ -531,9 +532,18 @@ __device__ __forceinline__ void bu_level(const DirArgs &D, const uint32_t *__res
         // 0's entry) rather than after it for candidates only: one
         // dependent step less per chunk for a coalesced word per vertex
         // (R-MAT scale 24's search 0.382 -> 0.375 ms)
+        // Once few vertices are left unvisited (`sparse`) the stream of PF
+        // would mostly be words of visited vertices: then only candidates
+        // read theirs, after the visited word, and only the lines holding a
+        // candidate's word are fetched.
         uint32_t pf[BU_CHUNKS];
+        if (sparse) {
 #pragma unroll
-        for (uint32_t h = 0; h < BU_CHUNKS; h++) pf[h] = PF[u[h] <= V ? u[h] : 0u];
+            for (uint32_t h = 0; h < BU_CHUNKS; h++) pf[h] = cand[h] ? PF[u[h]] : 0u;
+        } else {
+#pragma unroll
+            for (uint32_t h = 0; h < BU_CHUNKS; h++) pf[h] = PF[u[h] <= V ? u[h] : 0u];
+        }
 #pragma unroll
         for (uint32_t h = 0; h < BU_CHUNKS; h++) {
             c[h] = (SINGLE || !cand[h]) ? 0u : cls[u[h]];
@@ -692,7 +702,8 @@ __global__ __launch_bounds__(256) void expand_do_kernel(DoArgs a, uint32_t bpd)
         lb_level<SINGLE, COUNT>(D, a.cls, L, bid, bpd, s_lds, n, slotted ? s_pre : nullptr, a.qcap, wc);
     else if (st.mode == DIR_BU)
         bu_level<SINGLE, COUNT>(D, a.PR[dir], a.PC[dir], a.PF[dir], a.cls, a.V, L, bid, bpd, s_lds, a.qcap,
-                                a.ctr + CT(C_ANOM), wc);
+                                a.ctr + CT(C_ANOM),
+                                (int64_t)a.ctr[CT(C_NLIVE)] - (int64_t)st.visited < (int64_t)(a.V / BU_SPARSE), wc);
     if (COUNT) add_work(a.ctr, wc);
 }
 
