@@ -283,9 +283,11 @@ class Stats:
     # bytes of each first-search work unit (scc_stats.first_work): top-down
     # vertex 12 (two row offsets + queue entry), arc 8 (column + visited
     # word), claim 4 (queue entry); bottom-up swept vertex 4.125 (its visited
-    # bit and its first-parent word, read for every vertex of the sweep),
-    # candidate 0 (its first parent is in the sweep's read), first parent
-    # probed 8 (bitmap pair), row after a miss 8, further parent 12
+    # bit and its first-parent word: every vertex of a dense sweep, only the
+    # candidates of a sparse one, whose other vertices read just their bit,
+    # at most V/8 bytes per level and direction left uncounted), candidate 0
+    # (its first parent is in the sweep's read), first parent probed 8
+    # (bitmap pair), row after a miss 8, further parent 12
     WORK_BYTES = (12.0, 8.0, 4.0, 4.125, 0.0, 8.0, 8.0, 12.0)
 
     @property

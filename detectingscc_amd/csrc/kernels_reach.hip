@@ -596,7 +596,9 @@ __device__ __forceinline__ void bu_level(const DirArgs &D, const uint32_t *__res
             hits[h] = __ballot(found[h]);
             need_long[h] = more[h] && e[h] - b[h] > BU_SHORT;
             if (COUNT) {
-                wc.n[W_BUS] += u[h] <= V;
+                // a vertex whose visited bit and first-parent word were read:
+                // every vertex of a dense sweep, a candidate of a sparse one
+                wc.n[W_BUS] += sparse ? cand[h] : u[h] <= V;
                 wc.n[W_BUC] += cand[h];
                 wc.n[W_BUF] += pf[h] != 0;
                 wc.n[W_BUR] += more[h];
