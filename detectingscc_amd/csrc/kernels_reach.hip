@@ -469,7 +469,30 @@ constexpr uint32_t BU_SHORT = 1;   // the first parent, read from the first-pare
 constexpr uint32_t BU_GROUP = 4;
 constexpr uint32_t BU_WQ = 1024;
 constexpr uint32_t BU_CHUNKS = 2;   // 64-vertex chunks per wave iteration
-constexpr uint32_t BU_SPARSE = 16;  // unvisited live vertices under V / BU_SPARSE: candidates alone read PF
+constexpr uint32_t BU_SPARSE = 16;  // unvisited live vertices under V / BU_SPARSE: a sparse level
+// 64-vertex chunks a sparse step scans.  Its candidate masks and counts are
+// wave-uniform and live in scalar registers, which every level of the
+// kernel then reserves: with 4, 6 or 8 chunks (86 to 102 SGPRs) R-MAT scale
+// 24's dense bottom-up level took 251 us instead of 225 (the compiler still
+// reported 8 waves per SIMD for 86 and 94); with 2 (78 SGPRs, the LDS size
+// unchanged too) it did not slow down.
+constexpr uint32_t BU_SP_CHUNKS = 2;
+constexpr uint32_t BU_WH = (BU_SP_CHUNKS > BU_CHUNKS ? BU_SP_CHUNKS : BU_CHUNKS) * 2;   // hit words per wave
+
+// Position of the r-th set bit of m (r < popcount(m)), by halving.
+__device__ __forceinline__ uint32_t nth_bit(uint64_t m, uint32_t r)
+{
+    uint32_t pos = 0;
+#pragma unroll
+    for (uint32_t w = 32; w >= 1; w >>= 1) {
+        const uint32_t c = (uint32_t)__popcll(m & ((1ull << w) - 1));
+        const bool skip = r >= c;
+        r -= skip ? c : 0u;
+        m = skip ? m >> w : m;
+        pos += skip ? w : 0u;
+    }
+    return pos;
+}
 
 // One 64-bit load gives p's visited-or-finished and finished bits.
 template <bool SINGLE>
@@ -481,11 +504,23 @@ __device__ __forceinline__ bool visited_parent(const uint32_t *vis, const uint32
 }
 
 // LDS words of a bottom-up level: each wave's queue, long list and the
-// long-list hits of its chunks (two words per chunk).
-constexpr uint32_t BU_LDS = 4 * BU_WQ + 4 * BU_CHUNKS * 64 + 4 * BU_CHUNKS * 2;
+// hits of its chunks (two words per chunk) that lanes other than the
+// vertex's own found.
+constexpr uint32_t BU_LDS = 4 * BU_WQ + 4 * BU_CHUNKS * 64 + 4 * BU_WH;
 
 // One bottom-up level of direction D (parents in PR/PC) for workgroup `bid`
 // of `bpd`; `lds` holds BU_LDS words.  Called by the whole workgroup.
+//
+// A sparse level (few unvisited vertices left) has one or two candidates
+// per 64-vertex chunk, and sweeping it a chunk per lane set ran the whole
+// chunk body for them: the level was bound by instructions, not memory
+// (43 us for 0.2M L2 requests on R-MAT scale 24).  A sparse step instead
+// reads the visited words of BU_SP_CHUNKS of the wave's chunks, lists their
+// candidates across the lanes (candidate i of the step on lane i % 64, its
+// chunk from the prefix of the chunks' candidate counts and its position
+// from nth_bit) and probes them 64 at a time, one per lane; hits are ORed
+// into per-chunk masks in LDS and stored by the lanes that own the chunks'
+// visited words, as in a dense step.
 //
 // The vertices a pass visits go to the region of slot bid % kSpread of the
 // output queue (slot_cap(V, bpd) = qcap entries: the slot's workgroups own
@@ -507,116 +542,15 @@ __device__ __forceinline__ void bu_level(const DirArgs &D, const uint32_t *__res
     const uint32_t wv = threadIdx.x >> 6, lane = wlane();
     uint32_t *wq = lds + wv * BU_WQ;
     uint32_t *wl = lds + 4 * BU_WQ + wv * BU_CHUNKS * 64;
-    uint32_t *wh = lds + 4 * BU_WQ + 4 * BU_CHUNKS * 64 + wv * BU_CHUNKS * 2;
+    uint32_t *wh = lds + 4 * BU_WQ + 4 * BU_CHUNKS * 64 + wv * BU_WH;
     uint32_t nq = 0;
     const uint32_t sub = lane % BU_GROUP, grp = lane / BU_GROUP;
     const uint32_t gshift = lane & ~(BU_GROUP - 1);
     const uint32_t stride = bpd * 256;   // one chunk of every wave of the grid
-    for (uint32_t base = (bid * 4 + wv) * 64; base <= V; base += BU_CHUNKS * stride) {
-        if (nq + 2 * BU_CHUNKS * 64 > BU_WQ) wave_flush(wq, nq, cnt_out, qout, qcap, anom);
-        uint32_t u[BU_CHUNKS], word[BU_CHUNKS], c[BU_CHUNKS], b[BU_CHUNKS], e[BU_CHUNKS];
-        bool cand[BU_CHUNKS], found[BU_CHUNKS], need_long[BU_CHUNKS];
-        uint64_t hits[BU_CHUNKS];
-#pragma unroll
-        for (uint32_t h = 0; h < BU_CHUNKS; h++) {
-            // lane l of chunk h holds vertex base + h * stride + l (vertex 0,
-            // the dummy, is finished)
-            u[h] = base + h * stride + lane;
-            word[h] = u[h] <= V ? D.vis[vword(u[h])] : ~0u;
-            cand[h] = u[h] != 0 && !(word[h] & (1u << (u[h] & 31)));
-        }
-        // the first parent comes from PF (coalesced; 0: no parent), not from
-        // a random line of the parent list, and only a candidate whose first
-        // parent misses loads its row offsets to see whether more follow.
-        // PF is read for every lane beside its visited word (past V: vertex
-        // 0's entry) rather than after it for candidates only: one
-        // dependent step less per chunk for a coalesced word per vertex
-        // (R-MAT scale 24's search 0.382 -> 0.375 ms)
-        // Once few vertices are left unvisited (`sparse`) the stream of PF
-        // would mostly be words of visited vertices: then only candidates
-        // read theirs, after the visited word, and only the lines holding a
-        // candidate's word are fetched.
-        uint32_t pf[BU_CHUNKS];
-        if (sparse) {
-#pragma unroll
-            for (uint32_t h = 0; h < BU_CHUNKS; h++) pf[h] = cand[h] ? PF[u[h]] : 0u;
-        } else {
-#pragma unroll
-            for (uint32_t h = 0; h < BU_CHUNKS; h++) pf[h] = PF[u[h] <= V ? u[h] : 0u];
-        }
-#pragma unroll
-        for (uint32_t h = 0; h < BU_CHUNKS; h++) {
-            c[h] = (SINGLE || !cand[h]) ? 0u : cls[u[h]];
-            if (!cand[h]) pf[h] = 0u;
-        }
-        // Every chunk's parent probe is issued before any result is used,
-        // and then every chunk's row offsets, without per-lane branches: a
-        // lane with nothing to load reads vertex 0's entries (the dummy:
-        // group 0 of the bitmap, rows 0 and 1), which are always in bounds.
-        // A wave with nothing to probe (or no row to read) skips the loads.
-        // With a probe and its rows per chunk in one branch, each load
-        // waited for the one before it.
-        bool anyp = false;
-#pragma unroll
-        for (uint32_t h = 0; h < BU_CHUNKS; h++) anyp |= pf[h] != 0;
-        uint64_t pw[BU_CHUNKS] = {};
-        if (__ballot(anyp)) {
-#pragma unroll
-            for (uint32_t h = 0; h < BU_CHUNKS; h++) pw[h] = reinterpret_cast<const uint64_t *>(D.vis)[pf[h] >> 5];
-        }
-        bool more[BU_CHUNKS];
-#pragma unroll
-        for (uint32_t h = 0; h < BU_CHUNKS; h++) {
-            // visited in this search (visited-or-finished and not finished),
-            // and of the candidate's class
-            const bool vp = ((uint32_t)pw[h] & ~(uint32_t)(pw[h] >> 32)) & (1u << (pf[h] & 31));
-            found[h] = pf[h] && vp && (SINGLE || cls[pf[h]] == c[h]);
-            more[h] = pf[h] && !found[h];
-        }
-        bool anym = false;
-#pragma unroll
-        for (uint32_t h = 0; h < BU_CHUNKS; h++) {
-            b[h] = e[h] = 0;
-            anym |= more[h];
-        }
-        if (__ballot(anym)) {
-#pragma unroll
-            for (uint32_t h = 0; h < BU_CHUNKS; h++) {
-                const uint32_t r = more[h] ? u[h] : 0u;
-                b[h] = PR[r];
-                e[h] = PR[r + 1];
-            }
-#pragma unroll
-            for (uint32_t h = 0; h < BU_CHUNKS; h++) {
-                if (!more[h]) b[h] = e[h] = 0;
-            }
-        }
-#pragma unroll
-        for (uint32_t h = 0; h < BU_CHUNKS; h++) {
-            hits[h] = __ballot(found[h]);
-            need_long[h] = more[h] && e[h] - b[h] > BU_SHORT;
-            if (COUNT) {
-                // a vertex whose visited bit and first-parent word were read:
-                // every vertex of a dense sweep, a candidate of a sparse one
-                wc.n[W_BUS] += sparse ? cand[h] : u[h] <= V;
-                wc.n[W_BUC] += cand[h];
-                wc.n[W_BUF] += pf[h] != 0;
-                wc.n[W_BUR] += more[h];
-                wc.n[W_CLAIM] += found[h];
-            }
-        }
-        uint32_t nlong = 0;
-#pragma unroll
-        for (uint32_t h = 0; h < BU_CHUNKS; h++) {
-            wave_push(wq, nq, found[h], u[h]);
-            const uint64_t lm = __ballot(need_long[h]);
-            if (need_long[h]) wl[nlong + __popcll(lm & ((1ull << lane) - 1))] = u[h];
-            nlong += (uint32_t)__popcll(lm);
-        }
-        if (lane < BU_CHUNKS * 2) wh[lane] = 0;
-        wave_sync();
-        // long parent lists: 64 / BU_GROUP vertices per round, each group
-        // stops at its first hit
+    const uint64_t *vis64 = reinterpret_cast<const uint64_t *>(D.vis);
+    // long parent lists of wl[0, nlong): 64 / BU_GROUP vertices per round,
+    // each group stops at its first hit; a hit goes to its chunk's mask in wh
+    auto long_lists = [&](uint32_t base, uint32_t nlong) {
         for (uint32_t k0 = 0; k0 < nlong; k0 += 64 / BU_GROUP) {
             const uint32_t k = k0 + grp;
             bool hit = false;
@@ -640,12 +574,189 @@ __device__ __forceinline__ void bu_level(const DirArgs &D, const uint32_t *__res
             }
             wave_push(wq, nq, claimed, x);
         }
-        wave_sync();   // long-list hits are in wh; the next iteration rewrites wl
+    };
+    if (!sparse) {
+        for (uint32_t base = (bid * 4 + wv) * 64; base <= V; base += BU_CHUNKS * stride) {
+            if (nq + 2 * BU_CHUNKS * 64 > BU_WQ) wave_flush(wq, nq, cnt_out, qout, qcap, anom);
+            uint32_t u[BU_CHUNKS], word[BU_CHUNKS], c[BU_CHUNKS], b[BU_CHUNKS], e[BU_CHUNKS];
+            bool cand[BU_CHUNKS], found[BU_CHUNKS], need_long[BU_CHUNKS];
+            uint64_t hits[BU_CHUNKS];
 #pragma unroll
-        for (uint32_t h = 0; h < BU_CHUNKS; h++) {
-            // lanes 0 and 32 own the chunk's two visited words
-            const uint32_t m = (lane == 0 ? (uint32_t)hits[h] : (uint32_t)(hits[h] >> 32)) | wh[h * 2 + (lane >> 5)];
-            if ((lane & 31) == 0 && m) D.vis[vword(u[h])] = word[h] | m;
+            for (uint32_t h = 0; h < BU_CHUNKS; h++) {
+                // lane l of chunk h holds vertex base + h * stride + l (vertex 0,
+                // the dummy, is finished)
+                u[h] = base + h * stride + lane;
+                word[h] = u[h] <= V ? D.vis[vword(u[h])] : ~0u;
+                cand[h] = u[h] != 0 && !(word[h] & (1u << (u[h] & 31)));
+            }
+            // the first parent comes from PF (coalesced; 0: no parent), not from
+            // a random line of the parent list, and only a candidate whose first
+            // parent misses loads its row offsets to see whether more follow.
+            // PF is read for every lane beside its visited word (past V: vertex
+            // 0's entry) rather than after it for candidates only: one
+            // dependent step less per chunk for a coalesced word per vertex
+            // (R-MAT scale 24's search 0.382 -> 0.375 ms)
+            uint32_t pf[BU_CHUNKS];
+#pragma unroll
+            for (uint32_t h = 0; h < BU_CHUNKS; h++) pf[h] = PF[u[h] <= V ? u[h] : 0u];
+#pragma unroll
+            for (uint32_t h = 0; h < BU_CHUNKS; h++) {
+                c[h] = (SINGLE || !cand[h]) ? 0u : cls[u[h]];
+                if (!cand[h]) pf[h] = 0u;
+            }
+            // Every chunk's parent probe is issued before any result is used,
+            // and then every chunk's row offsets, without per-lane branches: a
+            // lane with nothing to load reads vertex 0's entries (the dummy:
+            // group 0 of the bitmap, rows 0 and 1), which are always in bounds.
+            // A wave with nothing to probe (or no row to read) skips the loads.
+            // With a probe and its rows per chunk in one branch, each load
+            // waited for the one before it.
+            bool anyp = false;
+#pragma unroll
+            for (uint32_t h = 0; h < BU_CHUNKS; h++) anyp |= pf[h] != 0;
+            uint64_t pw[BU_CHUNKS] = {};
+            if (__ballot(anyp)) {
+#pragma unroll
+                for (uint32_t h = 0; h < BU_CHUNKS; h++) pw[h] = reinterpret_cast<const uint64_t *>(D.vis)[pf[h] >> 5];
+            }
+            bool more[BU_CHUNKS];
+#pragma unroll
+            for (uint32_t h = 0; h < BU_CHUNKS; h++) {
+                // visited in this search (visited-or-finished and not finished),
+                // and of the candidate's class
+                const bool vp = ((uint32_t)pw[h] & ~(uint32_t)(pw[h] >> 32)) & (1u << (pf[h] & 31));
+                found[h] = pf[h] && vp && (SINGLE || cls[pf[h]] == c[h]);
+                more[h] = pf[h] && !found[h];
+            }
+            bool anym = false;
+#pragma unroll
+            for (uint32_t h = 0; h < BU_CHUNKS; h++) {
+                b[h] = e[h] = 0;
+                anym |= more[h];
+            }
+            if (__ballot(anym)) {
+#pragma unroll
+                for (uint32_t h = 0; h < BU_CHUNKS; h++) {
+                    const uint32_t r = more[h] ? u[h] : 0u;
+                    b[h] = PR[r];
+                    e[h] = PR[r + 1];
+                }
+#pragma unroll
+                for (uint32_t h = 0; h < BU_CHUNKS; h++) {
+                    if (!more[h]) b[h] = e[h] = 0;
+                }
+            }
+#pragma unroll
+            for (uint32_t h = 0; h < BU_CHUNKS; h++) {
+                hits[h] = __ballot(found[h]);
+                need_long[h] = more[h] && e[h] - b[h] > BU_SHORT;
+                if (COUNT) {
+                    // a vertex whose visited bit and first-parent word were read
+                    wc.n[W_BUS] += u[h] <= V;
+                    wc.n[W_BUC] += cand[h];
+                    wc.n[W_BUF] += pf[h] != 0;
+                    wc.n[W_BUR] += more[h];
+                    wc.n[W_CLAIM] += found[h];
+                }
+            }
+            uint32_t nlong = 0;
+#pragma unroll
+            for (uint32_t h = 0; h < BU_CHUNKS; h++) {
+                wave_push(wq, nq, found[h], u[h]);
+                const uint64_t lm = __ballot(need_long[h]);
+                if (need_long[h]) wl[nlong + __popcll(lm & ((1ull << lane) - 1))] = u[h];
+                nlong += (uint32_t)__popcll(lm);
+            }
+            if (lane < BU_CHUNKS * 2) wh[lane] = 0;
+            wave_sync();
+            long_lists(base, nlong);
+            wave_sync();   // long-list hits are in wh; the next iteration rewrites wl
+#pragma unroll
+            for (uint32_t h = 0; h < BU_CHUNKS; h++) {
+                // lanes 0 and 32 own the chunk's two visited words
+                const uint32_t m = (lane == 0 ? (uint32_t)hits[h] : (uint32_t)(hits[h] >> 32)) | wh[h * 2 + (lane >> 5)];
+                if ((lane & 31) == 0 && m) D.vis[vword(u[h])] = word[h] | m;
+            }
+        }
+    } else {
+        for (uint32_t base = (bid * 4 + wv) * 64; base <= V; base += BU_SP_CHUNKS * stride) {
+            // a step pushes each of its candidates at most once
+            if (nq + BU_SP_CHUNKS * 64 > BU_WQ) wave_flush(wq, nq, cnt_out, qout, qcap, anom);
+            uint32_t word[BU_SP_CHUNKS];
+#pragma unroll
+            for (uint32_t h = 0; h < BU_SP_CHUNKS; h++) {
+                const uint32_t uh = base + h * stride + lane;
+                word[h] = uh <= V ? D.vis[vword(uh)] : ~0u;
+            }
+            uint64_t cm[BU_SP_CHUNKS];
+            uint32_t pre[BU_SP_CHUNKS + 1];
+            pre[0] = 0;
+#pragma unroll
+            for (uint32_t h = 0; h < BU_SP_CHUNKS; h++) {
+                const uint32_t uh = base + h * stride + lane;   // vertex 0, the dummy, is finished
+                cm[h] = __ballot(uh != 0 && !(word[h] & (1u << (uh & 31))));
+                pre[h + 1] = pre[h] + (uint32_t)__popcll(cm[h]);
+            }
+            const uint32_t total = pre[BU_SP_CHUNKS];
+            if (!total) continue;
+            if (lane < BU_SP_CHUNKS * 2) wh[lane] = 0;
+            wave_sync();
+            for (uint32_t b0 = 0; b0 < total; b0 += 64) {
+                const uint32_t i = b0 + lane;
+                const bool ok = i < total;
+                // chunk h holds candidates [pre[h], pre[h + 1]) of the step
+                uint32_t h = 0;
+#pragma unroll
+                for (uint32_t k = 1; k < BU_SP_CHUNKS; k++) h += i >= pre[k];
+                uint64_t m = cm[0];
+                uint32_t r0 = 0;
+#pragma unroll
+                for (uint32_t k = 1; k < BU_SP_CHUNKS; k++) {
+                    m = h == k ? cm[k] : m;
+                    r0 = h == k ? pre[k] : r0;
+                }
+                const uint32_t pos = ok ? nth_bit(m, i - r0) : 0u;
+                const uint32_t u = ok ? base + h * stride + pos : 0u;   // an idle lane reads vertex 0's words
+                // the first parent and the class together, then the parent's
+                // bitmap pair, then (after a miss) the row offsets
+                uint32_t pf = PF[u];
+                const uint32_t c = SINGLE ? 0u : cls[u];
+                if (!ok) pf = 0u;
+                uint64_t pw = 0;
+                if (__ballot(pf != 0)) pw = vis64[pf >> 5];
+                const bool vp = ((uint32_t)pw & ~(uint32_t)(pw >> 32)) & (1u << (pf & 31));
+                const bool found = pf && vp && (SINGLE || cls[pf] == c);
+                const bool more = pf && !found;
+                uint32_t rb = 0, re = 0;
+                if (__ballot(more)) {
+                    const uint32_t r = more ? u : 0u;
+                    rb = PR[r];
+                    re = PR[r + 1];
+                    if (!more) rb = re = 0;
+                }
+                const bool need_long = more && re - rb > BU_SHORT;
+                if (COUNT) {
+                    wc.n[W_BUS] += ok;   // a candidate of a sparse step read its bit and first parent
+                    wc.n[W_BUC] += ok;
+                    wc.n[W_BUF] += pf != 0;
+                    wc.n[W_BUR] += more;
+                    wc.n[W_CLAIM] += found;
+                }
+                if (found) atomicOr(&wh[h * 2 + (pos >> 5)], 1u << (pos & 31));   // LDS
+                wave_push(wq, nq, found, u);
+                const uint64_t lm = __ballot(need_long);
+                if (need_long) wl[__popcll(lm & ((1ull << lane) - 1))] = u;
+                wave_sync();
+                long_lists(base, (uint32_t)__popcll(lm));
+                wave_sync();   // the hits are in wh; the next batch rewrites wl
+            }
+#pragma unroll
+            for (uint32_t h = 0; h < BU_SP_CHUNKS; h++) {
+                // lanes 0 and 32 own the chunk's two visited words
+                const uint32_t mh = wh[h * 2 + (lane >> 5)];
+                if ((lane & 31) == 0 && mh) D.vis[vword(base + h * stride + lane)] = word[h] | mh;
+            }
+            wave_sync();   // the next step clears wh
         }
     }
     wave_flush_block(wq, nq, cnt_out, qout, qcap, anom);
