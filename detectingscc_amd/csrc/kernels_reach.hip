@@ -472,11 +472,8 @@ constexpr uint32_t BU_CHUNKS = 2;   // 64-vertex chunks per wave iteration
 constexpr uint32_t BU_SPARSE = 16;  // unvisited live vertices under V / BU_SPARSE: a sparse level
 // 64-vertex chunks a sparse step scans.  Its candidate masks and counts are
 // wave-uniform and live in scalar registers, which every level of the
-// kernel then reserves: with 4, 6 or 8 chunks (86 to 102 SGPRs) R-MAT scale
-// 24's dense bottom-up level took 251 us instead of 225 (the compiler still
-// reported 8 waves per SIMD for 86 and 94); with 2 (78 SGPRs, the LDS size
-// unchanged too) it did not slow down.
-constexpr uint32_t BU_SP_CHUNKS = 2;
+// kernel then reserves; expand_do_kernel is held to 80 of them.
+constexpr uint32_t BU_SP_CHUNKS = 6;
 constexpr uint32_t BU_WH = (BU_SP_CHUNKS > BU_CHUNKS ? BU_SP_CHUNKS : BU_CHUNKS) * 2;   // hit words per wave
 
 // Position of the r-th set bit of m (r < popcount(m)), by halving.
@@ -784,7 +781,12 @@ __device__ __forceinline__ void add_work(uint32_t *ctr, const WorkCount &wc)
 }
 
 template <bool SINGLE, bool COUNT>
-__global__ __launch_bounds__(256) void expand_do_kernel(DoArgs a, uint32_t bpd)
+// At most 80 scalar registers (the compiler moves the rest to vector lanes):
+// with 86 to 102 (the sparse steps' uniform masks) R-MAT scale 24's dense
+// bottom-up level took 251 us instead of 225, although the compiler
+// reported 8 waves per SIMD for 86 and 94; held to 80, the same code runs
+// the dense level in 224 us and the two sparse levels in 52 + 23 us.
+__global__ __launch_bounds__(256) __attribute__((amdgpu_num_sgpr(80))) void expand_do_kernel(DoArgs a, uint32_t bpd)
 {
     __shared__ uint32_t s_lds[BU_LDS > LB_LDS ? BU_LDS : LB_LDS];
     __shared__ uint32_t s_pre[kSpread + 2];
