@@ -71,7 +71,8 @@ class Engine {
     DirArgs dir_args(int d, uint32_t level) const;
     // single: every live vertex has class 0 (the first trim of a
     // decomposition); none_dead: besides, the state is fresh (no vertex dead,
-    // all in class 0, labels v | LAB_FINAL; its per-vertex words may be unwritten)
+    // all in class 0, labels v | LAB_FINAL; its per-vertex words may be unwritten;
+    // every counter zero: the state initialisation was the last thing queued)
     // wait = false: one queued batch of passes, no look at the counters
     scc_status trim1(bool fixpoint, bool single = false, bool none_dead = false, bool wait = true,
                      bool labels = true);   // labels = false: a fresh state's labels stay unwritten

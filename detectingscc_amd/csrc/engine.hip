@@ -677,7 +677,9 @@ static constexpr uint32_t kTrimQueued = 4;
 scc_status Engine::trim1(bool fixpoint, bool single, bool none_dead, bool wait, bool labels)
 {
     mark(SCC_PHASE_TRIM1);
-    zero_ctr(C_TLIST, 3);
+    // a fresh state comes straight from the state initialisation, which
+    // zeroed every counter line (the removal-list counts and their slots too)
+    if (!none_dead) zero_ctr(C_TLIST, 3);
     // the single-class live bitmap lives in the byte scratch array, free
     // during a decomposition; the first pivot round reads it
     uint32_t *live = single ? live_bits() : nullptr;
