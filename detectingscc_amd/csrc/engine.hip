@@ -244,7 +244,12 @@ scc_status Engine::upload_partition(const uint32_t *part)
         for (uint64_t v = b + 1; v <= e; v++)
             dense[v] = (uint32_t)(std::lower_bound(ids.begin(), ids.end(), part[v]) - ids.begin());
     });
-    HIP_TRY(hipMemcpy(st.part, dense.data(), sizeof(uint32_t) * ((size_t)g.V + 2), hipMemcpyHostToDevice));
+    // on the engine's stream (non-blocking, so not ordered with the null
+    // stream): after any queued work that reads the partition, before the
+    // next run's; the synchronisation keeps `dense` alive until copied
+    HIP_TRY(hipMemcpyAsync(st.part, dense.data(), sizeof(uint32_t) * ((size_t)g.V + 2), hipMemcpyHostToDevice,
+                           stream));
+    HIP_TRY(hipStreamSynchronize(stream));
     num_parts = (uint32_t)ids.size();
     has_part = true;
     return SCC_OK;
@@ -1281,7 +1286,10 @@ scc_status Engine::phase_first_pivot(const uint8_t *dead, uint32_t *pivot)
     if ((s = read_ctr()) != SCC_OK) return s;
     *pivot = 0;
     if (h_ctr[CT(C_NPIV)] > 1) return fail(SCC_ERR_DEVICE, "first-pivot election chose %u vertices", h_ctr[CT(C_NPIV)]);
-    if (h_ctr[CT(C_NPIV)] == 1) HIP_TRY(hipMemcpy(pivot, st.pivlist, sizeof(uint32_t), hipMemcpyDeviceToHost));
+    if (h_ctr[CT(C_NPIV)] == 1) {
+        HIP_TRY(hipMemcpyAsync(pivot, st.pivlist, sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
+        HIP_TRY(hipStreamSynchronize(stream));
+    }
     return SCC_OK;
 }
 
